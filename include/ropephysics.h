@@ -1,0 +1,208 @@
+/* ropephysics.h -- C ABI of the MI355X (gfx950) batched Cosserat-rod stepper.
+ *
+ * This is the drop-in boundary for RopePhysics' batched GPU step. The
+ * reference exposes it as the C++ class crs::gpu::Batch
+ * (src/gpu/gpu_solver.h:66-145) over an nvcc-only inner ABI
+ * (src/gpu/gpu_kernels.h:147-172). Here the same operations are plain
+ * extern "C" functions over pointers and sizes; the C++ facade in
+ * ropephysics/batch.hpp maps the reference's Batch method set onto them.
+ *
+ * Conventions
+ *   - Every call returns an int status (RP_OK == 0) unless stated; on failure
+ *     rp_last_error() (thread-local) says why. Nothing fails silently: wrong
+ *     sizes, out-of-range rods and device errors are reported.
+ *   - The library owns all device memory. Host buffers are only read or
+ *     written during the call that receives them.
+ *   - A batch is used from one host thread at a time. Batches are
+ *     independent; each has its own HIP stream, so work on two batches
+ *     overlaps on the device. rp_batch_sync waits for one batch only.
+ *   - Arrays are FP32. Vectors are xyz triples, quaternions wxyz quadruples
+ *     (body -> world). Per-element arrays of a rod range are packed in rod
+ *     order, each rod contributing its own element count (ragged CSR).
+ */
+#ifndef ROPEPHYSICS_H
+#define ROPEPHYSICS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RP_ABI_VERSION 1
+
+enum rp_status {
+    RP_OK = 0,
+    RP_ERR_INVALID = -1,  /* bad argument, size or index */
+    RP_ERR_DEVICE = -2,   /* no usable device, or a HIP call failed */
+    RP_ERR_MEMORY = -3,   /* device or host allocation failed */
+    RP_ERR_CAPACITY = -4, /* a rod does not fit the chosen strategy's LDS budget */
+    RP_ERR_STATE = -5     /* call not valid in the batch's current state */
+};
+
+/* Execution strategy.
+ *   PLAIN  one rod per 128-thread workgroup, one launch per step, state
+ *          re-read from and written to HBM every step: the reference's fused
+ *          launch shape (src/gpu/gpu_solver.cu:531-681), kept as the baseline.
+ *   TILED  rods of any sizes packed side by side into wavefront lane groups,
+ *          state resident in LDS and VGPRs across every substep of many steps
+ *          per launch; launches replayed from a HIP graph.
+ *   AUTO   TILED when every rod fits it, else PLAIN. */
+enum rp_strategy { RP_STRATEGY_PLAIN = 0, RP_STRATEGY_TILED = 1, RP_STRATEGY_AUTO = 2 };
+
+enum rp_prim_type { RP_PLANE = 0, RP_SPHERE = 1, RP_CAPSULE = 2, RP_BOX = 3 };
+
+/* Same fields, order and meaning as the reference's TPrimitive<float>
+ * (src/core/geometry.h:17-65). */
+typedef struct rp_primitive {
+    int type;
+    float a[3];           /* plane: point; sphere: centre; capsule: end A; box: centre */
+    float b[3];           /* capsule: end B */
+    float normal[3];      /* plane: unit outward normal */
+    float half_extents[3];/* box */
+    float rotation[4];    /* box orientation, wxyz */
+    float radius;         /* sphere / capsule */
+    float friction;
+} rp_primitive;
+
+/* One collision world (reference CollisionWorld, src/core/collision.h:83-106),
+ * with the values the reference derives on the host already resolved. */
+typedef struct rp_world_desc {
+    int num_prims;
+    const rp_primitive* prims;
+    float contact_margin_radii; /* contactMarginRadii: a particle within this many rod radii of a
+                                   surface gets a contact (reference default 0.25) */
+    int self_collision;    /* 0 / 1 */
+    float self_friction;
+    int self_skip;         /* selfSkip; the gap used is max(self_skip, floor(2r / l_min) + 2) */
+    int hash_table_size;   /* spatial-hash buckets (reference default 4096) */
+    int self_capacity;     /* self contacts stored per rod, 0 for max(16, segments / 2);
+                              more are counted as overflow */
+} rp_world_desc;
+
+/* A ragged batch: rods may differ in every size. Element arrays are indexed
+ * globally (rod r owns [x_start[r], x_start[r+1])); constraint endpoint
+ * indices are rod-local. Constraints may be given in any order; the library
+ * computes the colouring (greedy, in index order) itself. */
+typedef struct rp_batch_desc {
+    int num_rods;
+    const int* particle_start; /* [num_rods + 1] */
+    const int* frame_start;    /* [num_rods + 1]: segment frames then fixed ghost frames */
+    const int* stretch_start;  /* [num_rods + 1] */
+    const int* bend_start;     /* [num_rods + 1] */
+    const float* inv_mass;     /* [P] */
+    const float* inv_inertia;  /* [F * 3] body-frame diagonal */
+    const float* inertia;      /* [F * 3] */
+    const int* stretch_p0;     /* [S] */
+    const int* stretch_p1;     /* [S] */
+    const int* stretch_seg;    /* [S] */
+    const float* stretch_rest; /* [S] */
+    const float* stretch_compliance; /* [S * 3] */
+    const int* bend_a;         /* [B] */
+    const int* bend_b;         /* [B] */
+    const float* bend_rest;    /* [B] */
+    const float* bend_compliance;    /* [B * 3] */
+    const float* bend_rest_darboux;  /* [B * 3] */
+    const float* radius;       /* [num_rods] */
+    int num_worlds;            /* 0: no contacts at all */
+    const rp_world_desc* worlds;
+    const int* world_of_rod;   /* [num_rods], or NULL: every rod uses world 0 */
+    int strategy;              /* rp_strategy */
+    int tile_config;           /* 0: autotune; else an id from rp_tile_config_name */
+} rp_batch_desc;
+
+/* BatchParams (src/gpu/gpu_solver.h:52-60). */
+typedef struct rp_step_params {
+    float dt;
+    int substeps;
+    int iterations;
+    float gravity[3];
+    float linear_damping;
+    float angular_damping;
+    int contact_interval;
+} rp_step_params;
+
+typedef struct rp_device_info {
+    char name[256];
+    char arch[64];
+    int compute_units;
+    int wavefront_size;
+    size_t lds_per_workgroup;
+    size_t total_global_mem;
+    int clock_khz;
+    int max_tiled_segments;  /* longest chain rod the TILED path accepts (no contacts) */
+} rp_device_info;
+
+typedef struct rp_counters {
+    long long self_contact_overflow; /* summed over rods and steps */
+    int strategy;                    /* resolved strategy */
+    int tile_config;                 /* resolved tile configuration id */
+    int workgroups;                  /* per launch */
+    int lanes_busy;                  /* lanes assigned to rods, over all workgroups */
+    int lanes_total;
+    size_t lds_bytes_per_workgroup;  /* largest */
+    int launches_per_step;           /* PLAIN: 1; TILED: 1 / steps per launch */
+    long long steps_taken;
+    size_t device_bytes;             /* device memory held by the batch */
+} rp_counters;
+
+/* cudaAvailable / deviceName / diagnostic / maxFusedSegments
+ * (src/gpu/gpu_solver.h:44-48,150). */
+int rp_device_query(rp_device_info* out);
+
+/* Batch::create (src/gpu/gpu_batch.cpp:150-316): allocate, colour, pack and
+ * upload topology; state starts zeroed (upload it with rp_batch_set_state). */
+int rp_batch_create(const rp_batch_desc* desc, void** out_batch);
+
+/* Batch::~Batch / destroy (src/gpu/gpu_batch.cpp:132-138). */
+void rp_batch_destroy(void* batch);
+
+/* Batch::upload / uploadRod (src/gpu/gpu_batch.cpp:318-370), in bulk for rods
+ * [first_rod, first_rod + num_rods). x, v: particles * 3; q: frames * 4;
+ * omega: frames * 3. NULL leaves a field unchanged. */
+int rp_batch_set_state(void* batch, int first_rod, int num_rods, const float* x, const float* v, const float* q,
+                       const float* omega);
+
+/* Batch::setLoads / setKinematicVelocities (src/gpu/gpu_batch.cpp:372-391,
+ * 418-432): external force and kinematic velocity per particle, external
+ * torque (world frame) per frame, and -- when fixed_q is given, frames * 4 --
+ * the orientation of every zero-inertia (fixed) frame. NULL leaves a field
+ * unchanged. */
+int rp_batch_set_loads(void* batch, int first_rod, int num_rods, const float* force, const float* kin_vel,
+                       const float* torque, const float* fixed_q);
+
+/* Batch::setMaterialScales (src/gpu/gpu_batch.cpp:434-443): per-rod Young's
+ * modulus scale (every compliance is divided by it) and friction scale, each
+ * [num_rods] or NULL to leave unchanged. Non-positive youngs scales are an
+ * error. */
+int rp_batch_set_scales(void* batch, const float* youngs_scale, const float* friction_scale);
+
+/* Batch::step (src/gpu/gpu_batch.cpp:457-473), num_steps times, asynchronous
+ * on the batch's stream. */
+int rp_batch_step(void* batch, const rp_step_params* params, int num_steps);
+
+/* Batch::synchronize (src/gpu/gpu_batch.cpp:475), scoped to this batch. */
+int rp_batch_sync(void* batch);
+
+/* Batch::download (src/gpu/gpu_batch.cpp:393-416), in bulk; waits for the
+ * batch's pending steps. NULL skips a field. */
+int rp_batch_get_state(void* batch, int first_rod, int num_rods, float* x, float* v, float* q, float* omega);
+
+/* Batch::selfContactOverflow, launchesPerStep, sharedBytesPerBlock
+ * (src/gpu/gpu_batch.cpp:140-148,477-484). */
+int rp_batch_counters(void* batch, rp_counters* out);
+
+/* Name of a tile configuration id (for logs and the autotuner's report);
+ * NULL for an unknown id. */
+const char* rp_tile_config_name(int id);
+
+/* Number of tile configuration ids the TILED strategy knows (ids 1..n). */
+int rp_tile_config_count(void);
+
+/* Thread-local message for the last failed call on this thread. */
+const char* rp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ROPEPHYSICS_H */

@@ -1,0 +1,127 @@
+"""ctypes binding of the C ABI in include/ropephysics.h.
+
+Loads ``libropephysics.so`` built in-tree by ``ropephysics_amd.build``. There is
+no fallback: if the library is missing or no GPU is usable, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libropephysics.so")
+
+RP_OK, RP_ERR_INVALID, RP_ERR_DEVICE, RP_ERR_MEMORY, RP_ERR_CAPACITY, RP_ERR_STATE = 0, -1, -2, -3, -4, -5
+STRATEGY = {"plain": 0, "tiled": 1, "auto": 2}
+STRATEGY_NAME = {v: k for k, v in STRATEGY.items()}
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+
+
+class RpPrimitive(C.Structure):
+    _fields_ = [("type", C.c_int), ("a", C.c_float * 3), ("b", C.c_float * 3), ("normal", C.c_float * 3),
+                ("half_extents", C.c_float * 3), ("rotation", C.c_float * 4), ("radius", C.c_float),
+                ("friction", C.c_float)]
+
+
+class RpWorldDesc(C.Structure):
+    _fields_ = [("num_prims", C.c_int), ("prims", C.POINTER(RpPrimitive)), ("contact_margin_radii", C.c_float),
+                ("self_collision", C.c_int), ("self_friction", C.c_float), ("self_skip", C.c_int),
+                ("hash_table_size", C.c_int), ("self_capacity", C.c_int)]
+
+
+class RpBatchDesc(C.Structure):
+    _fields_ = [("num_rods", C.c_int), ("particle_start", _ip), ("frame_start", _ip), ("stretch_start", _ip),
+                ("bend_start", _ip), ("inv_mass", _fp), ("inv_inertia", _fp), ("inertia", _fp),
+                ("stretch_p0", _ip), ("stretch_p1", _ip), ("stretch_seg", _ip), ("stretch_rest", _fp),
+                ("stretch_compliance", _fp), ("bend_a", _ip), ("bend_b", _ip), ("bend_rest", _fp),
+                ("bend_compliance", _fp), ("bend_rest_darboux", _fp), ("radius", _fp), ("num_worlds", C.c_int),
+                ("worlds", C.POINTER(RpWorldDesc)), ("world_of_rod", _ip), ("strategy", C.c_int),
+                ("tile_config", C.c_int)]
+
+
+class RpStepParams(C.Structure):
+    _fields_ = [("dt", C.c_float), ("substeps", C.c_int), ("iterations", C.c_int), ("gravity", C.c_float * 3),
+                ("linear_damping", C.c_float), ("angular_damping", C.c_float), ("contact_interval", C.c_int)]
+
+
+class RpDeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 256), ("arch", C.c_char * 64), ("compute_units", C.c_int),
+                ("wavefront_size", C.c_int), ("lds_per_workgroup", C.c_size_t), ("total_global_mem", C.c_size_t),
+                ("clock_khz", C.c_int), ("max_tiled_segments", C.c_int)]
+
+
+class RpCounters(C.Structure):
+    _fields_ = [("self_contact_overflow", C.c_longlong), ("strategy", C.c_int), ("tile_config", C.c_int),
+                ("workgroups", C.c_int), ("lanes_busy", C.c_int), ("lanes_total", C.c_int),
+                ("lds_bytes_per_workgroup", C.c_size_t), ("launches_per_step", C.c_int),
+                ("steps_taken", C.c_longlong), ("device_bytes", C.c_size_t)]
+
+
+# Every symbol include/ropephysics.h declares, with its signature.
+SIGNATURES = {
+    "rp_device_query": (C.c_int, [C.POINTER(RpDeviceInfo)]),
+    "rp_batch_create": (C.c_int, [C.POINTER(RpBatchDesc), C.POINTER(C.c_void_p)]),
+    "rp_batch_destroy": (None, [C.c_void_p]),
+    "rp_batch_set_state": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _fp, _fp, _fp, _fp]),
+    "rp_batch_set_loads": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _fp, _fp, _fp, _fp]),
+    "rp_batch_set_scales": (C.c_int, [C.c_void_p, _fp, _fp]),
+    "rp_batch_step": (C.c_int, [C.c_void_p, C.POINTER(RpStepParams), C.c_int]),
+    "rp_batch_sync": (C.c_int, [C.c_void_p]),
+    "rp_batch_get_state": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _fp, _fp, _fp, _fp]),
+    "rp_batch_counters": (C.c_int, [C.c_void_p, C.POINTER(RpCounters)]),
+    "rp_tile_config_name": (C.c_char_p, [C.c_int]),
+    "rp_tile_config_count": (C.c_int, []),
+    "rp_last_error": (C.c_char_p, []),
+}
+
+_LIB = None
+
+
+class RopePhysicsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ropephysics error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """The loaded HIP library. Raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RopePhysicsError(RP_ERR_DEVICE, f"{LIB_PATH} is not built; run ropephysics_amd.build.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc: int) -> None:
+    if rc != RP_OK:
+        raise RopePhysicsError(rc, lib().rp_last_error().decode())
+
+
+def fptr(a):
+    return None if a is None else a.ctypes.data_as(_fp)
+
+
+def iptr(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+def device_info() -> dict:
+    info = RpDeviceInfo()
+    check(lib().rp_device_query(C.byref(info)))
+    return {"name": info.name.decode(), "arch": info.arch.decode(), "compute_units": info.compute_units,
+            "wavefront_size": info.wavefront_size, "lds_per_workgroup": info.lds_per_workgroup,
+            "total_global_mem": info.total_global_mem, "clock_khz": info.clock_khz,
+            "max_tiled_segments": info.max_tiled_segments}
+
+
+def tile_configs() -> dict:
+    L = lib()
+    return {i: L.rp_tile_config_name(i).decode() for i in range(1, L.rp_tile_config_count() + 1)}

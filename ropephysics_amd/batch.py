@@ -1,0 +1,246 @@
+"""Python mirror of the reference's ``crs::gpu::Batch`` over the C ABI.
+
+Reference interface: ``src/gpu/gpu_solver.h:66-145``. Method names follow it
+(``create``/``step``/``synchronize``/``download``/``download_positions``/
+``upload``/``set_loads``/``upload_rod``/``set_kinematic_velocities``/
+``set_material_scales``/``self_contact_overflow``/``launches_per_step``), with
+two extensions: a batch may hold rods of different shapes (a list of rods
+instead of one replicated prototype), and each rod may collide with its own
+world.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from .rod import CollisionWorld, Rod, SolverParams
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _world_desc(w: CollisionWorld):
+    prims = (_abi.RpPrimitive * max(1, len(w.primitives)))()
+    for k, p in enumerate(w.primitives):
+        c = prims[k]
+        c.type = int(p.type)
+        c.a[:] = list(map(float, p.a))
+        c.b[:] = list(map(float, p.b))
+        c.normal[:] = list(map(float, p.normal))
+        c.half_extents[:] = list(map(float, p.half_extents))
+        c.rotation[:] = list(map(float, p.rotation))
+        c.radius = float(p.radius)
+        c.friction = float(p.friction)
+    d = _abi.RpWorldDesc(len(w.primitives), C.cast(prims, C.POINTER(_abi.RpPrimitive)),
+                         float(w.contact_margin_radii), int(w.self_collision), float(w.self_friction),
+                         int(w.self_skip), int(w.hash_table_size), 0)
+    return d, prims
+
+
+def step_params(p: SolverParams) -> _abi.RpStepParams:
+    g = (C.c_float * 3)(*map(float, p.gravity))
+    return _abi.RpStepParams(float(p.dt), int(p.substeps), int(p.iterations), g, float(p.linear_damping),
+                             float(p.angular_damping), int(p.contact_interval))
+
+
+class Batch:
+    """A batch of independent rods stepped on the GPU.
+
+    ``Batch(prototype, num_rods)`` replicates one rod as the reference's
+    ``Batch::create`` does; ``Batch([rod_a, rod_b, ...])`` packs rods of any
+    shapes. ``worlds`` is one ``CollisionWorld`` for every rod or a list, with
+    ``world_of_rod`` choosing per rod.
+    """
+
+    def __init__(self, rods, num_rods: int | None = None, strategy: str = "auto", world=None,
+                 world_of_rod=None, tile_config: int = 0):
+        self._h = None
+        if isinstance(rods, Rod):
+            rods = [rods] * int(num_rods if num_rods is not None else 1)
+        self.rods = list(rods)
+        R = len(self.rods)
+        if R == 0:
+            raise ValueError("a batch needs at least one rod")
+        nP = np.array([r.num_particles for r in self.rods])
+        nF = np.array([r.num_frames for r in self.rods])
+        nS = np.array([r.num_segments for r in self.rods])
+        nB = np.array([len(r.bA) for r in self.rods])
+        self.particle_start = _i32(np.concatenate([[0], np.cumsum(nP)]))
+        self.frame_start = _i32(np.concatenate([[0], np.cumsum(nF)]))
+        self.stretch_start = _i32(np.concatenate([[0], np.cumsum(nS)]))
+        self.bend_start = _i32(np.concatenate([[0], np.cumsum(nB)]))
+        uniq = {id(r): r for r in self.rods}
+        same = len(uniq) == 1
+        proto = self.rods[0]
+
+        def cat(name, comps=None):
+            if same:
+                a = getattr(proto, name)
+                return np.tile(a, (R, 1)) if a.ndim == 2 else np.tile(a, R)
+            return np.concatenate([getattr(r, name) for r in self.rods])
+
+        self._keep = []
+        arr = {
+            "inv_mass": _f32(cat("invMass")), "inv_inertia": _f32(cat("invInertia")), "inertia": _f32(cat("inertia")),
+            "stretch_p0": _i32(cat("sP0")), "stretch_p1": _i32(cat("sP1")), "stretch_seg": _i32(cat("sSeg")),
+            "stretch_rest": _f32(cat("sRest")), "stretch_compliance": _f32(cat("sComp")),
+            "bend_a": _i32(cat("bA")), "bend_b": _i32(cat("bB")), "bend_rest": _f32(cat("bRest")),
+            "bend_compliance": _f32(cat("bComp")), "bend_rest_darboux": _f32(cat("bRestD")),
+            "radius": _f32([r.material.radius for r in self.rods]),
+        }
+        self._keep.append(arr)
+        if world is None:
+            worlds = []
+        elif isinstance(world, CollisionWorld):
+            worlds = [world]
+        else:
+            worlds = list(world)
+        wdescs = (_abi.RpWorldDesc * max(1, len(worlds)))()
+        for i, w in enumerate(worlds):
+            d, prims = _world_desc(w)
+            self._keep.append(prims)
+            wdescs[i] = d
+        self._keep.append(wdescs)
+        wor = None
+        if world_of_rod is not None:
+            wor = _i32(world_of_rod)
+            self._keep.append(wor)
+        desc = _abi.RpBatchDesc()
+        desc.num_rods = R
+        desc.particle_start = _abi.iptr(self.particle_start)
+        desc.frame_start = _abi.iptr(self.frame_start)
+        desc.stretch_start = _abi.iptr(self.stretch_start)
+        desc.bend_start = _abi.iptr(self.bend_start)
+        for k, a in arr.items():
+            setattr(desc, k, _abi.fptr(a) if a.dtype == np.float32 else _abi.iptr(a))
+        desc.num_worlds = len(worlds)
+        desc.worlds = C.cast(wdescs, C.POINTER(_abi.RpWorldDesc))
+        desc.world_of_rod = _abi.iptr(wor)
+        desc.strategy = _abi.STRATEGY[strategy]
+        desc.tile_config = int(tile_config)
+        h = C.c_void_p()
+        _abi.check(_abi.lib().rp_batch_create(C.byref(desc), C.byref(h)))
+        self._h = h
+        self.num_rods = R
+        self.upload_all(self.rods)
+
+    def __del__(self):
+        self.destroy()
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            _abi.lib().rp_batch_destroy(self._h)
+            self._h = None
+
+    # ------------------------------------------------------------------ state and loads
+
+    def _range_sizes(self, first, count):
+        p0, p1 = self.particle_start[first], self.particle_start[first + count]
+        f0, f1 = self.frame_start[first], self.frame_start[first + count]
+        return int(p1 - p0), int(f1 - f0)
+
+    def set_state(self, first: int, count: int, x=None, v=None, q=None, omega=None):
+        """Bulk state upload for rods [first, first + count), packed per rod."""
+        P, F = self._range_sizes(first, count)
+        arrs = [None if a is None else _f32(a).reshape(-1) for a in (x, v, q, omega)]
+        for a, n in zip(arrs, (P * 3, P * 3, F * 4, F * 3)):
+            if a is not None and a.size != n:
+                raise ValueError(f"state array has {a.size} values, expected {n}")
+        _abi.check(_abi.lib().rp_batch_set_state(self._h, first, count, *[_abi.fptr(a) for a in arrs]))
+
+    def get_state(self, first: int = 0, count: int | None = None) -> dict:
+        """Bulk read-back of x, v, q, omega for rods [first, first + count)."""
+        count = self.num_rods - first if count is None else count
+        P, F = self._range_sizes(first, count)
+        out = {"x": np.empty((P, 3), np.float32), "v": np.empty((P, 3), np.float32),
+               "q": np.empty((F, 4), np.float32), "omega": np.empty((F, 3), np.float32)}
+        _abi.check(_abi.lib().rp_batch_get_state(self._h, first, count, *[_abi.fptr(out[k]) for k in
+                                                                            ("x", "v", "q", "omega")]))
+        return out
+
+    def set_loads_range(self, first: int, count: int, force=None, kin_vel=None, torque=None, fixed_q=None):
+        P, F = self._range_sizes(first, count)
+        arrs = [None if a is None else _f32(a).reshape(-1) for a in (force, kin_vel, torque, fixed_q)]
+        for a, n in zip(arrs, (P * 3, P * 3, F * 3, F * 4)):
+            if a is not None and a.size != n:
+                raise ValueError(f"load array has {a.size} values, expected {n}")
+        _abi.check(_abi.lib().rp_batch_set_loads(self._h, first, count, *[_abi.fptr(a) for a in arrs]))
+
+    def upload_all(self, rods):
+        """Every rod's full state and loads, one bulk copy per field."""
+        if len(rods) != self.num_rods:
+            raise ValueError("need one rod per batch entry")
+        same = len({id(r) for r in rods}) == 1
+
+        def cat(name):
+            if same:
+                a = getattr(rods[0], name)
+                return np.tile(a, (self.num_rods, 1))
+            return np.concatenate([getattr(r, name) for r in rods])
+
+        self.set_state(0, self.num_rods, cat("x"), cat("v"), cat("q"), cat("omega"))
+        self.set_loads_range(0, self.num_rods, cat("extForce"), cat("kinVel"), cat("extTorque"))
+
+    # Reference method set (gpu_solver.h:74-105).
+    def upload(self, rod: Rod):
+        self.upload_all([rod] * self.num_rods)
+
+    def upload_rod(self, index: int, rod: Rod):
+        self.set_state(index, 1, rod.x, rod.v, rod.q, rod.omega)
+        self.set_loads(index, rod)
+
+    def set_loads(self, index: int, rod: Rod):
+        self.set_loads_range(index, 1, rod.extForce, rod.kinVel, rod.extTorque, rod.q)
+
+    def set_kinematic_velocities(self, rod_major):
+        self.set_loads_range(0, self.num_rods, kin_vel=rod_major)
+
+    def set_material_scales(self, youngs_scale=None, friction_scale=None):
+        ys = None if youngs_scale is None or len(youngs_scale) == 0 else _f32(youngs_scale)
+        fs = None if friction_scale is None or len(friction_scale) == 0 else _f32(friction_scale)
+        for a in (ys, fs):
+            if a is not None and a.size != self.num_rods:
+                raise ValueError(f"scale array has {a.size} values for {self.num_rods} rods")
+        _abi.check(_abi.lib().rp_batch_set_scales(self._h, _abi.fptr(ys), _abi.fptr(fs)))
+
+    def step(self, params: SolverParams, steps: int = 1):
+        sp = step_params(params)
+        _abi.check(_abi.lib().rp_batch_step(self._h, C.byref(sp), int(steps)))
+
+    def synchronize(self):
+        _abi.check(_abi.lib().rp_batch_sync(self._h))
+
+    def download(self, index: int) -> dict:
+        return self.get_state(index, 1)
+
+    def download_positions(self) -> np.ndarray:
+        P = int(self.particle_start[-1])
+        x = np.empty((P, 3), np.float32)
+        _abi.check(_abi.lib().rp_batch_get_state(self._h, 0, self.num_rods, _abi.fptr(x), None, None, None))
+        return x
+
+    def counters(self) -> dict:
+        c = _abi.RpCounters()
+        _abi.check(_abi.lib().rp_batch_counters(self._h, C.byref(c)))
+        d = {k: getattr(c, k) for k, _ in c._fields_}
+        d["strategy"] = _abi.STRATEGY_NAME.get(d["strategy"], d["strategy"])
+        if d["tile_config"]:
+            d["tile_config_name"] = _abi.lib().rp_tile_config_name(d["tile_config"]).decode()
+        return d
+
+    def self_contact_overflow(self) -> int:
+        return int(self.counters()["self_contact_overflow"])
+
+    def launches_per_step(self) -> int:
+        return int(self.counters()["launches_per_step"])
+
+    def rod_slice(self, arr, index, kind="particle"):
+        s = self.particle_start if kind == "particle" else self.frame_start
+        return arr[s[index]:s[index + 1]]

@@ -1,0 +1,622 @@
+// Batched XPBD Cosserat-rod stepper kernels for gfx950 (CDNA4).
+//
+// Two launch shapes over the same per-constraint arithmetic (rp_math.h):
+//
+//  k_plain_step   one rod per 128-lane workgroup, whole state staged in LDS,
+//                 one launch per step, state re-read from and written back to
+//                 HBM every step. This is the reference's fused launch shape
+//                 (src/gpu/gpu_solver.cu:531-681) and serves as the baseline.
+//
+//  k_tiled_step   rods of any length packed into lane groups of 64-lane
+//                 wavefronts. Positions and orientations, which constraints
+//                 share across lanes, live in LDS; everything a lane owns
+//                 (velocities, substep-start state, multipliers, per-
+//                 constraint constants) lives in its VGPRs for the whole
+//                 launch, which runs many steps. HBM is touched once per
+//                 launch. A lane group walks each colour in trips of its
+//                 own width, so rods of uneven segment counts share one
+//                 launch without padding to the longest.
+#include <hip/hip_runtime.h>
+
+#include "rp_layout.h"
+#include "rp_math.h"
+#include "rp_phases.h"
+
+namespace rp {
+
+// ------------------------------------------------------------------ PLAIN
+
+// LDS carve-up of one rod for the PLAIN kernel, in bytes.
+struct PlainLayout {
+    int x, xp, v, f, kv, q, qp, om, tq, iI, I, lamS, lamB, cont, prims, self, total;
+};
+
+__host__ __device__ inline PlainLayout plain_layout(int nP, int nF, int nS, int nB, int nPrim, int self, int table,
+                                                    int cap) {
+    PlainLayout L;
+    int o = 0;
+    L.x = o;
+    o += nP * 16;
+    L.xp = o;
+    o += nP * 16;
+    L.v = o;
+    o += nP * 16;
+    L.f = o;
+    o += nP * 16;
+    L.kv = o;
+    o += nP * 16;
+    L.q = o;
+    o += nF * 16;
+    L.qp = o;
+    o += nF * 16;
+    L.om = o;
+    o += nF * 16;
+    L.tq = o;
+    o += nF * 16;
+    L.iI = o;
+    o += nF * 16;
+    L.I = o;
+    o += nF * 16;
+    L.lamS = o;
+    o += nS * 16;
+    L.lamB = o;
+    o += nB * 16;
+    L.cont = o;
+    o += nP * nPrim * kContactFloats * 4;
+    L.prims = o;
+    o += nPrim * kPrimFloats * 4;
+    L.self = o;
+    if (self) o += self_lds_bytes(nS, table, cap);
+    L.total = (o + 15) & ~15;
+    return L;
+}
+
+__global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa) {
+    extern __shared__ float4 smem4[];
+    char* smem = reinterpret_cast<char*>(smem4);
+    const int r = blockIdx.x;
+    if (r >= d.numRods) return;
+    const RodInfo ri = d.rods[r];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nP = ri.nP, nF = ri.nF, nS = ri.nS, nB = ri.nB, nPrim = ri.nPrim;
+    const PlainLayout L = plain_layout(nP, nF, nS, nB, nPrim, ri.self, ri.table, ri.selfCap);
+    float4* x4 = reinterpret_cast<float4*>(smem + L.x);
+    float4* xp4 = reinterpret_cast<float4*>(smem + L.xp);
+    float4* v4 = reinterpret_cast<float4*>(smem + L.v);
+    float4* f4 = reinterpret_cast<float4*>(smem + L.f);
+    float4* kv4 = reinterpret_cast<float4*>(smem + L.kv);
+    float4* q4s = reinterpret_cast<float4*>(smem + L.q);
+    float4* qp4 = reinterpret_cast<float4*>(smem + L.qp);
+    float4* om4 = reinterpret_cast<float4*>(smem + L.om);
+    float4* tq4 = reinterpret_cast<float4*>(smem + L.tq);
+    float4* iI4 = reinterpret_cast<float4*>(smem + L.iI);
+    float4* I4 = reinterpret_cast<float4*>(smem + L.I);
+    float4* lamS = reinterpret_cast<float4*>(smem + L.lamS);
+    float4* lamB = reinterpret_cast<float4*>(smem + L.lamB);
+    float* cont = reinterpret_cast<float*>(smem + L.cont);
+    float* prims = reinterpret_cast<float*>(smem + L.prims);
+    SelfLds sl;
+    if (ri.self) sl = self_carve(smem + L.self, nS, ri.table, ri.selfCap);
+
+    const int p0 = ri.pStart, f0 = ri.fStart;
+    for (int i = tid; i < nP; i += nt) {
+        x4[i] = vf4(ld3(d.x, p0 + i), d.invMass[p0 + i]);
+        v4[i] = vf4(ld3(d.v, p0 + i), 0.0f);
+        f4[i] = vf4(ld3(d.force, p0 + i), 0.0f);
+        kv4[i] = vf4(ld3(d.kinVel, p0 + i), 0.0f);
+    }
+    for (int j = tid; j < nF; j += nt) {
+        q4s[j] = qf4(ld4q(d.q, f0 + j));
+        om4[j] = vf4(ld3(d.omega, f0 + j), 0.0f);
+        tq4[j] = vf4(ld3(d.torque, f0 + j), 0.0f);
+        iI4[j] = reinterpret_cast<const float4*>(d.invInertia4)[f0 + j];
+        I4[j] = reinterpret_cast<const float4*>(d.inertia4)[f0 + j];
+    }
+    for (int k = tid; k < nPrim; k += nt) stage_prim(prims + k * kPrimFloats, d.prims[ri.primOff + k]);
+    if (ri.self && tid == 0) {
+        sl.misc[0] = 0;
+        sl.misc[1] = 0;
+    }
+    const float cscale = d.complianceScale[r];
+    const float fscale = d.frictionScale[r];
+    const int* colS = d.colStart + ri.colOff;
+    const int* colB = colS + ri.nSCol + 1;
+    const int4* sIdx = reinterpret_cast<const int4*>(d.sIdx) + ri.sStart;
+    const float4* sPar = reinterpret_cast<const float4*>(d.sPar) + ri.sStart;
+    const int4* bIdx = reinterpret_cast<const int4*>(d.bIdx) + ri.bStart;
+    const float4* bPar = reinterpret_cast<const float4*>(d.bPar) + ri.bStart;
+    const float4* bRst = reinterpret_cast<const float4*>(d.bRest4) + ri.bStart;
+    const int* segEnd = d.segEnd + 2 * ri.sStart;
+    const v3 g = mk(sa.gx, sa.gy, sa.gz);
+    const float h = sa.h;
+    __syncthreads();
+
+    for (int step = 0; step < sa.steps; ++step) {
+        for (int sub = 0; sub < sa.substeps; ++sub) {
+            const bool regen = sub % sa.contactInterval == 0;
+            for (int k = tid; k < nS; k += nt) lamS[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int k = tid; k < nB; k += nt) lamB[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (nPrim > 0)
+                for (int i = tid; i < nP; i += nt)
+                    world_contacts_begin(cont + i * nPrim * kContactFloats, prims, nPrim, f4v(x4[i]), x4[i].w,
+                                         ri.radius, ri.margin, fscale, regen);
+            if (ri.self) {
+                if (regen) {
+                    if (tid == 0) self_build_hash(sl, x4, segEnd, nS, ri.table, ri.radius);
+                    __syncthreads();
+                    for (int j = tid; j < nS; j += nt)
+                        self_query(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
+                                   ri.selfFriction * fscale, j, false);
+                    __syncthreads();
+                    if (tid == 0) self_allocate(sl, nS, ri.selfCap);
+                    __syncthreads();
+                    for (int j = tid; j < nS; j += nt)
+                        self_query(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
+                                   ri.selfFriction * fscale, j, true);
+                } else if (tid == 0) {
+                    self_reset(sl);
+                }
+                __syncthreads();
+            }
+            for (int i = tid; i < nP; i += nt) {
+                v3 x = f4v(x4[i]), v = f4v(v4[i]), xp;
+                const float w = x4[i].w;
+                predict_particle(x, v, xp, w, f4v(f4[i]), f4v(kv4[i]), h, g);
+                x4[i] = vf4(x, w);
+                v4[i] = vf4(v, 0.0f);
+                xp4[i] = vf4(xp, w);
+            }
+            for (int j = tid; j < nF; j += nt) {
+                q4 q = f4q(q4s[j]), qp;
+                v3 om = f4v(om4[j]);
+                predict_frame(q, om, qp, f4v(iI4[j]), f4v(I4[j]), f4v(tq4[j]), h);
+                q4s[j] = qf4(q);
+                qp4[j] = qf4(qp);
+                om4[j] = vf4(om, 0.0f);
+            }
+            __syncthreads();
+
+            for (int it = 0; it < sa.iterations; ++it) {
+                for (int c = 0; c < ri.nSCol; ++c) {
+                    for (int k = colS[c] + tid; k < colS[c + 1]; k += nt) {
+                        const int4 id = sIdx[k];
+                        const float4 par = sPar[k];
+                        v3 xa = f4v(x4[id.x]), xb = f4v(x4[id.y]);
+                        const float wa = x4[id.x].w, wb = x4[id.y].w;
+                        q4 q = f4q(q4s[id.z]);
+                        v3 lam = f4v(lamS[k]);
+                        if (project_stretch(xa, xb, q, lam, wa, wb, f4v(iI4[id.z]), par.x, mk(par.y, par.z, par.w),
+                                            sa.invH2, cscale)) {
+                            x4[id.x] = vf4(xa, wa);
+                            x4[id.y] = vf4(xb, wb);
+                            q4s[id.z] = qf4(q);
+                            lamS[k] = vf4(lam, 0.0f);
+                        }
+                    }
+                    __syncthreads();
+                }
+                for (int c = 0; c < ri.nBCol; ++c) {
+                    for (int k = colB[c] + tid; k < colB[c + 1]; k += nt) {
+                        const int4 id = bIdx[k];
+                        const float4 par = bPar[k];
+                        q4 qa = f4q(q4s[id.x]), qb = f4q(q4s[id.y]);
+                        v3 lam = f4v(lamB[k]);
+                        if (project_bend(qa, qb, lam, f4v(iI4[id.x]), f4v(iI4[id.y]), par.x, mk(par.y, par.z, par.w),
+                                         f4v(bRst[k]), sa.invH2, cscale)) {
+                            q4s[id.x] = qf4(qa);
+                            q4s[id.y] = qf4(qb);
+                            lamB[k] = vf4(lam, 0.0f);
+                        }
+                    }
+                    __syncthreads();
+                }
+                if (nPrim > 0) {
+                    for (int i = tid; i < nP; i += nt) {
+                        v3 x = f4v(x4[i]);
+                        const float w = x4[i].w;
+                        if (w <= 0.0f) continue;
+                        world_contacts_project(cont + i * nPrim * kContactFloats, nPrim, x, f4v(xp4[i]), w);
+                        x4[i] = vf4(x, w);
+                    }
+                    __syncthreads();
+                }
+                if (ri.self) {
+                    if (tid == 0) self_project(sl, x4, xp4);
+                    __syncthreads();
+                }
+            }
+
+            for (int i = tid; i < nP; i += nt) {
+                const v3 v = finish_particle(f4v(x4[i]), f4v(xp4[i]), x4[i].w, f4v(kv4[i]), h, sa.linDecay);
+                v4[i] = vf4(v, 0.0f);
+            }
+            for (int j = tid; j < nF; j += nt) {
+                const v3 om = finish_frame(f4q(q4s[j]), f4q(qp4[j]), f4v(iI4[j]), h, sa.angDecay);
+                om4[j] = vf4(om, 0.0f);
+            }
+            __syncthreads();
+        }
+    }
+
+    for (int i = tid; i < nP; i += nt) {
+        st3(d.x, p0 + i, f4v(x4[i]));
+        st3(d.v, p0 + i, f4v(v4[i]));
+    }
+    for (int j = tid; j < nF; j += nt) {
+        st4q(d.q, f0 + j, f4q(q4s[j]));
+        st3(d.omega, f0 + j, f4v(om4[j]));
+    }
+    if (ri.self && tid == 0) d.selfOverflow[r] += sl.misc[1];
+}
+
+// ------------------------------------------------------------------ TILED
+
+// LDS carve-up of one rod slot for the TILED kernel, in bytes.
+struct TiledLayout {
+    int x, q, f, kv, tq, cont, prims, xp, self, total;
+};
+
+__host__ __device__ inline TiledLayout tiled_layout(int nP, int nF, int nS, int nPrim, int self, int table, int cap) {
+    TiledLayout L;
+    int o = 0;
+    L.x = o;
+    o += nP * 16;
+    L.q = o;
+    o += nF * 16;
+    L.f = o;
+    o += nP * 16;
+    L.kv = o;
+    o += nP * 16;
+    L.tq = o;
+    o += nF * 16;
+    L.cont = o;
+    o += nP * nPrim * kContactFloats * 4;
+    L.prims = o;
+    o += nPrim * kPrimFloats * 4;
+    L.xp = o;
+    if (self) o += nP * 16;
+    L.self = o;
+    if (self) o += self_lds_bytes(nS, table, cap);
+    L.total = (o + 15) & ~15;
+    return L;
+}
+
+// TP: trips over particles and frames; TC: trips per colour per family.
+// A lane group of G lanes covers up to TP*G particles and frames and TC*G
+// constraints of each colour; the host sizes G per rod to satisfy that.
+template <int TP, int TC>
+__global__ __launch_bounds__(256) void k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles,
+                                                    const TileSlot* slots) {
+    extern __shared__ float4 smem4[];
+    char* smem = reinterpret_cast<char*>(smem4);
+    const Tile tile = tiles[blockIdx.x];
+    const int lane = threadIdx.x;
+
+    // Find this lane's rod slot.
+    int rod = -1, l = 0, G = 1, ldsOff = 0;
+    for (int s = 0; s < tile.numSlots; ++s) {
+        const TileSlot ts = slots[tile.firstSlot + s];
+        if (lane >= ts.lane0 && lane < ts.lane0 + ts.lanes) {
+            rod = ts.rod;
+            l = lane - ts.lane0;
+            G = ts.lanes;
+            ldsOff = ts.lds;
+        }
+    }
+    const bool active = rod >= 0;
+    RodInfo ri;
+    if (active) {
+        ri = d.rods[rod];
+    } else {
+        ri.nP = ri.nF = ri.nS = ri.nB = 0;
+        ri.nPrim = 0;
+        ri.self = 0;
+        ri.table = 1;
+        ri.selfCap = 0;
+        ri.pStart = ri.fStart = ri.sStart = ri.bStart = 0;
+        ri.colOff = 0;
+        ri.radius = ri.margin = ri.selfFriction = 0.0f;
+        ri.selfGap = 1;
+        ri.primOff = 0;
+    }
+    const int nP = ri.nP, nF = ri.nF, nS = ri.nS, nPrim = ri.nPrim;
+    const TiledLayout L = tiled_layout(nP, nF, nS, nPrim, ri.self, ri.table, ri.selfCap);
+    char* base = smem + ldsOff;
+    float4* x4 = reinterpret_cast<float4*>(base + L.x);
+    float4* q4s = reinterpret_cast<float4*>(base + L.q);
+    float4* f4 = reinterpret_cast<float4*>(base + L.f);
+    float4* kv4 = reinterpret_cast<float4*>(base + L.kv);
+    float4* tq4 = reinterpret_cast<float4*>(base + L.tq);
+    float* cont = reinterpret_cast<float*>(base + L.cont);
+    float* prims = reinterpret_cast<float*>(base + L.prims);
+    float4* xp4 = reinterpret_cast<float4*>(base + L.xp);
+    SelfLds sl;
+    if (ri.self) sl = self_carve(base + L.self, nS, ri.table, ri.selfCap);
+    // Any lane group in the workgroup may run self-collision; the barriers
+    // around it must be reached by every lane of the workgroup.
+    const bool anySelf = __syncthreads_or(ri.self) != 0;
+    const bool anyPrim = __syncthreads_or(nPrim > 0) != 0;
+
+    const int p0 = ri.pStart, f0 = ri.fStart;
+    // ---- lane-owned state and constants, loaded once per launch
+    v3 vel[TP], xpv[TP];
+    float wP[TP];
+    v3 om[TP], iIF[TP], IF[TP];
+    q4 qpv[TP];
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+        const int i = l + G * t;
+        if (active && i < nP) {
+            vel[t] = ld3(d.v, p0 + i);
+            wP[t] = d.invMass[p0 + i];
+            x4[i] = vf4(ld3(d.x, p0 + i), wP[t]);
+            f4[i] = vf4(ld3(d.force, p0 + i), 0.0f);
+            kv4[i] = vf4(ld3(d.kinVel, p0 + i), 0.0f);
+        }
+        const int j = l + G * t;
+        if (active && j < nF) {
+            om[t] = ld3(d.omega, f0 + j);
+            const float4 a = reinterpret_cast<const float4*>(d.invInertia4)[f0 + j];
+            const float4 b = reinterpret_cast<const float4*>(d.inertia4)[f0 + j];
+            iIF[t] = f4v(a);
+            IF[t] = f4v(b);
+            q4s[j] = qf4(ld4q(d.q, f0 + j));
+            tq4[j] = vf4(ld3(d.torque, f0 + j), 0.0f);
+        }
+    }
+    for (int k = l; active && k < nPrim; k += G) stage_prim(prims + k * kPrimFloats, d.prims[ri.primOff + k]);
+
+    // Stretch constraints: colour c, trip t -> position colS[c] + l + G t.
+    int sA[2][TC], sB[2][TC], sF[2][TC];
+    float sRest[2][TC];
+    v3 sComp[2][TC], sIIv[2][TC], sLam[2][TC];
+    bool sOn[2][TC];
+    int bA_[2][TC], bB_[2][TC];
+    float bRest[2][TC];
+    v3 bComp[2][TC], bRD[2][TC], bIa[2][TC], bIb[2][TC], bLam[2][TC];
+    bool bOn[2][TC];
+    const int* colS = d.colStart + ri.colOff;
+    const int* colB = colS + ri.nSCol + 1;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int t = 0; t < TC; ++t) {
+            sOn[c][t] = false;
+            bOn[c][t] = false;
+            if (active && c < ri.nSCol) {
+                const int k = colS[c] + l + G * t;
+                if (k < colS[c + 1]) {
+                    const int4 id = reinterpret_cast<const int4*>(d.sIdx)[ri.sStart + k];
+                    const float4 par = reinterpret_cast<const float4*>(d.sPar)[ri.sStart + k];
+                    sOn[c][t] = true;
+                    sA[c][t] = id.x;
+                    sB[c][t] = id.y;
+                    sF[c][t] = id.z;
+                    sRest[c][t] = par.x;
+                    sComp[c][t] = mk(par.y, par.z, par.w);
+                    sIIv[c][t] = f4v(reinterpret_cast<const float4*>(d.invInertia4)[f0 + id.z]);
+                }
+            }
+            if (active && c < ri.nBCol) {
+                const int k = colB[c] + l + G * t;
+                if (k < colB[c + 1]) {
+                    const int4 id = reinterpret_cast<const int4*>(d.bIdx)[ri.bStart + k];
+                    const float4 par = reinterpret_cast<const float4*>(d.bPar)[ri.bStart + k];
+                    bOn[c][t] = true;
+                    bA_[c][t] = id.x;
+                    bB_[c][t] = id.y;
+                    bRest[c][t] = par.x;
+                    bComp[c][t] = mk(par.y, par.z, par.w);
+                    bRD[c][t] = f4v(reinterpret_cast<const float4*>(d.bRest4)[ri.bStart + k]);
+                    bIa[c][t] = f4v(reinterpret_cast<const float4*>(d.invInertia4)[f0 + id.x]);
+                    bIb[c][t] = f4v(reinterpret_cast<const float4*>(d.invInertia4)[f0 + id.y]);
+                }
+            }
+        }
+    const float cscale = active ? d.complianceScale[rod] : 1.0f;
+    const float fscale = active ? d.frictionScale[rod] : 1.0f;
+    const int* segEnd = d.segEnd + 2 * ri.sStart;
+    if (ri.self && l == 0) {
+        sl.misc[0] = 0;
+        sl.misc[1] = 0;
+    }
+    const v3 g = mk(sa.gx, sa.gy, sa.gz);
+    const float h = sa.h;
+    __syncthreads();
+
+    for (int step = 0; step < sa.steps; ++step) {
+        for (int sub = 0; sub < sa.substeps; ++sub) {
+            const bool regen = sub % sa.contactInterval == 0;
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int t = 0; t < TC; ++t) {
+                    sLam[c][t] = mk(0.f, 0.f, 0.f);
+                    bLam[c][t] = mk(0.f, 0.f, 0.f);
+                }
+            if (anyPrim && nPrim > 0) {
+#pragma unroll
+                for (int t = 0; t < TP; ++t) {
+                    const int i = l + G * t;
+                    if (active && i < nP)
+                        world_contacts_begin(cont + i * nPrim * kContactFloats, prims, nPrim, f4v(x4[i]), wP[t],
+                                             ri.radius, ri.margin, fscale, regen);
+                }
+            }
+            if (anySelf) {
+                if (regen) {
+                    if (ri.self && l == 0) self_build_hash(sl, x4, segEnd, nS, ri.table, ri.radius);
+                    __syncthreads();
+                    if (ri.self)
+                        for (int j = l; j < nS; j += G)
+                            self_query(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
+                                       ri.selfFriction * fscale, j, false);
+                    __syncthreads();
+                    if (ri.self && l == 0) self_allocate(sl, nS, ri.selfCap);
+                    __syncthreads();
+                    if (ri.self)
+                        for (int j = l; j < nS; j += G)
+                            self_query(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
+                                       ri.selfFriction * fscale, j, true);
+                } else if (ri.self && l == 0) {
+                    self_reset(sl);
+                }
+                __syncthreads();
+            }
+            // ---- predict
+#pragma unroll
+            for (int t = 0; t < TP; ++t) {
+                const int i = l + G * t;
+                if (active && i < nP) {
+                    v3 x = f4v(x4[i]);
+                    predict_particle(x, vel[t], xpv[t], wP[t], f4v(f4[i]), f4v(kv4[i]), h, g);
+                    x4[i] = vf4(x, wP[t]);
+                    if (ri.self) xp4[i] = vf4(xpv[t], wP[t]);
+                }
+                const int j = l + G * t;
+                if (active && j < nF) {
+                    q4 q = f4q(q4s[j]);
+                    predict_frame(q, om[t], qpv[t], iIF[t], IF[t], f4v(tq4[j]), h);
+                    q4s[j] = qf4(q);
+                }
+            }
+            __syncthreads();
+
+            for (int it = 0; it < sa.iterations; ++it) {
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+#pragma unroll
+                    for (int t = 0; t < TC; ++t) {
+                        if (sOn[c][t]) {
+                            const float4 ax = x4[sA[c][t]], bx = x4[sB[c][t]];
+                            v3 xa = f4v(ax), xb = f4v(bx);
+                            q4 q = f4q(q4s[sF[c][t]]);
+                            if (project_stretch(xa, xb, q, sLam[c][t], ax.w, bx.w, sIIv[c][t], sRest[c][t],
+                                                sComp[c][t], sa.invH2, cscale)) {
+                                x4[sA[c][t]] = vf4(xa, ax.w);
+                                x4[sB[c][t]] = vf4(xb, bx.w);
+                                q4s[sF[c][t]] = qf4(q);
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+#pragma unroll
+                    for (int t = 0; t < TC; ++t) {
+                        if (bOn[c][t]) {
+                            q4 qa = f4q(q4s[bA_[c][t]]), qb = f4q(q4s[bB_[c][t]]);
+                            if (project_bend(qa, qb, bLam[c][t], bIa[c][t], bIb[c][t], bRest[c][t], bComp[c][t],
+                                             bRD[c][t], sa.invH2, cscale)) {
+                                q4s[bA_[c][t]] = qf4(qa);
+                                q4s[bB_[c][t]] = qf4(qb);
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+                if (anyPrim) {
+                    if (nPrim > 0) {
+#pragma unroll
+                        for (int t = 0; t < TP; ++t) {
+                            const int i = l + G * t;
+                            if (active && i < nP && wP[t] > 0.0f) {
+                                v3 x = f4v(x4[i]);
+                                world_contacts_project(cont + i * nPrim * kContactFloats, nPrim, x, xpv[t], wP[t]);
+                                x4[i] = vf4(x, wP[t]);
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+                if (anySelf) {
+                    if (ri.self && l == 0) self_project(sl, x4, xp4);
+                    __syncthreads();
+                }
+            }
+
+            // ---- finish: velocities from positions; nothing shared is written
+#pragma unroll
+            for (int t = 0; t < TP; ++t) {
+                const int i = l + G * t;
+                if (active && i < nP)
+                    vel[t] = finish_particle(f4v(x4[i]), xpv[t], wP[t], f4v(kv4[i]), h, sa.linDecay);
+                const int j = l + G * t;
+                if (active && j < nF) om[t] = finish_frame(f4q(q4s[j]), qpv[t], iIF[t], h, sa.angDecay);
+            }
+        }
+    }
+
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+        const int i = l + G * t;
+        if (active && i < nP) {
+            st3(d.x, p0 + i, f4v(x4[i]));
+            st3(d.v, p0 + i, vel[t]);
+        }
+        const int j = l + G * t;
+        if (active && j < nF) {
+            st4q(d.q, f0 + j, f4q(q4s[j]));
+            st3(d.omega, f0 + j, om[t]);
+        }
+    }
+    if (ri.self && l == 0) d.selfOverflow[rod] += sl.misc[1];
+}
+
+// Copy the orientation of zero-inertia frames from `src` (frames * 4, packed
+// for frames [f0, f0 + count)).
+__global__ void k_apply_fixed_frames(float* q, const float* src, const float* invInertia4, int f0, int count) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count) return;
+    const float4 iI = reinterpret_cast<const float4*>(invInertia4)[f0 + j];
+    if (norm2(mk(iI.x, iI.y, iI.z)) == 0.0f)
+        for (int c = 0; c < 4; ++c) q[4 * (f0 + j) + c] = src[4 * j + c];
+}
+
+}  // namespace rp
+
+// ------------------------------------------------------------------ host-side launch wrappers
+
+namespace rp {
+
+int plain_lds_bytes(int nP, int nF, int nS, int nB, int nPrim, int self, int table, int cap) {
+    return plain_layout(nP, nF, nS, nB, nPrim, self, table, cap).total;
+}
+
+int tiled_lds_bytes(int nP, int nF, int nS, int nPrim, int self, int table, int cap) {
+    return tiled_layout(nP, nF, nS, nPrim, self, table, cap).total;
+}
+
+hipError_t launch_plain(const DeviceArrays& d, const StepArgs& sa, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(k_plain_step, dim3(d.numRods), dim3(128), lds, s, d, sa);
+    return hipGetLastError();
+}
+
+template <int TP, int TC>
+static hipError_t launch_tiled_t(const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
+                                 const TileSlot* slots, int numTiles, int threads, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((k_tiled_step<TP, TC>), dim3(numTiles), dim3(threads), lds, s, d, sa, tiles, slots);
+    return hipGetLastError();
+}
+
+// Trip shapes compiled into the library; (TP, TC) pairs.
+hipError_t launch_tiled(int tp, int tc, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
+                        const TileSlot* slots, int numTiles, int threads, size_t lds, hipStream_t s) {
+    if (tp == 2 && tc == 1) return launch_tiled_t<2, 1>(d, sa, tiles, slots, numTiles, threads, lds, s);
+    if (tp == 3 && tc == 1) return launch_tiled_t<3, 1>(d, sa, tiles, slots, numTiles, threads, lds, s);
+    if (tp == 3 && tc == 2) return launch_tiled_t<3, 2>(d, sa, tiles, slots, numTiles, threads, lds, s);
+    if (tp == 5 && tc == 2) return launch_tiled_t<5, 2>(d, sa, tiles, slots, numTiles, threads, lds, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_fixed_frames(float* q, const float* src, const float* invInertia4, int f0, int count,
+                               hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_apply_fixed_frames, dim3((count + 255) / 256), dim3(256), 0, s, q, src, invInertia4, f0,
+                       count);
+    return hipGetLastError();
+}
+
+}  // namespace rp

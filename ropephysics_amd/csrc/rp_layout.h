@@ -1,0 +1,105 @@
+// Device-memory layout shared by the host side (rp_batch.cpp) and the
+// kernels (rp_kernels.hip). Plain structs only.
+//
+// HBM holds every per-element array rod after rod (ragged CSR order):
+//   state   x[P*3] v[P*3] q[F*4] omega[F*3]         (read and written)
+//   loads   force[P*3] kinVel[P*3] torque[F*3]      (read)
+//   consts  invMass[P] invInertia4[F] inertia4[F]   (read; float4, w unused)
+//   stretch sIdx[S] (p0, p1, seg, -) and sPar[S] (rest, compliance xyz),
+//           stored in each rod's colour order, colour after colour
+//   bend    bIdx[B] (a, b, -, -), bPar[B] (rest, compliance xyz) and
+//           bRest4[B] (rest Darboux xyz, -), same ordering
+//   segEnd[S] (p0, p1) in the rod's original constraint order, for the
+//           self-collision broadphase, which visits segments in index order
+#pragma once
+
+namespace rp {
+
+constexpr int kMaxColours = 8;     // per family, PLAIN path
+constexpr int kTiledColours = 2;   // per family, TILED path
+constexpr int kMaxPrims = 8;       // per world
+constexpr int kPrimFloats = 20;    // one primitive staged in LDS
+constexpr int kContactFloats = 8;  // one world-contact slot
+constexpr int kSelfFloats = 16;    // one self-contact slot
+
+struct RodInfo {
+    int pStart, nP, fStart, nF;
+    int sStart, nS, bStart, nB;
+    int world;     // index into worlds, -1 for none
+    int nPrim;     // primitives of that world
+    int self;      // self-collision on
+    int selfGap;   // minimum segment index gap
+    int selfCap;   // pool capacity
+    int table;     // hash buckets
+    int colOff;    // into colStart: stretch colours then bend colours
+    int nSCol, nBCol;
+    int primOff;   // into prims
+    float radius;
+    float margin;  // contact margin (margin radii * radius)
+    float selfFriction;
+    int pad[3];
+};
+
+struct Prim {
+    int type;
+    float a[3], b[3], normal[3], half[3], rot[4];
+    float radius, friction;
+};
+
+// A lane group of a TILED workgroup: `lanes` consecutive lanes starting at
+// `lane0` step rod `rod`, whose LDS region starts at byte `lds`.
+struct TileSlot {
+    int rod;
+    int lane0;
+    int lanes;
+    int lds;
+};
+
+struct Tile {
+    int firstSlot;
+    int numSlots;
+    int ldsBytes;
+    int pad;
+};
+
+struct DeviceArrays {
+    // state
+    float* x;
+    float* v;
+    float* q;
+    float* omega;
+    // loads
+    const float* force;
+    const float* kinVel;
+    const float* torque;
+    // constants
+    const float* invMass;
+    const float* invInertia4;
+    const float* inertia4;
+    const int* sIdx;      // int4
+    const float* sPar;    // float4
+    const int* bIdx;      // int4
+    const float* bPar;    // float4
+    const float* bRest4;  // float4
+    const int* segEnd;    // int2
+    const int* colStart;
+    const RodInfo* rods;
+    const Prim* prims;
+    float* complianceScale;
+    float* frictionScale;
+    int* selfOverflow;
+    int numRods;
+};
+
+struct StepArgs {
+    float h;
+    float invH2;
+    float gx, gy, gz;
+    float linDecay, angDecay;
+    int substeps;
+    int iterations;
+    int contactInterval;
+    int steps;  // steps in this launch
+};
+
+}  // namespace rp
