@@ -144,6 +144,8 @@ typedef struct rp_counters {
     int launches_per_step;           /* PLAIN: 1; TILED: 1 / steps per launch */
     long long steps_taken;
     size_t device_bytes;             /* device memory held by the batch */
+    float last_step_gpu_ms;          /* with timing on: HIP-event time of the last rp_batch_step */
+    int last_step_launches;          /* kernel launches in the last rp_batch_step */
 } rp_counters;
 
 /* cudaAvailable / deviceName / diagnostic / maxFusedSegments
@@ -191,6 +193,10 @@ int rp_batch_get_state(void* batch, int first_rod, int num_rods, float* x, float
 /* Batch::selfContactOverflow, launchesPerStep, sharedBytesPerBlock
  * (src/gpu/gpu_batch.cpp:140-148,477-484). */
 int rp_batch_counters(void* batch, rp_counters* out);
+
+/* Record HIP events around every rp_batch_step on the batch's stream
+ * (enable = 1) so rp_counters reports its device time. Off by default. */
+int rp_batch_set_timing(void* batch, int enable);
 
 /* Name of a tile configuration id (for logs and the autotuner's report);
  * NULL for an unknown id. */

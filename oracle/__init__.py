@@ -121,6 +121,9 @@ def lib(kind: str = "oracle"):
     L.orc_cell_coord.argtypes = [C.c_int, C.c_double, C.c_double]
     L.orc_self_gap.argtypes = [vp, C.POINTER(WorldC)]
     L.orc_colour.argtypes = [vp, C.c_int, _ip, _ip, C.c_int]
+    if hasattr(L, "orc_count_ops"):
+        L.orc_count_ops.argtypes = [vp, C.POINTER(ParamsC), C.POINTER(WorldC), C.c_int,
+                                    C.POINTER(C.c_longlong)]
     _LIBS[kind] = L
     return L
 
@@ -274,6 +277,16 @@ class Rod:
         w, s = C.c_int(), C.c_int()
         self.L.orc_contact_counts(self.h, C.byref(w), C.byref(s))
         return w.value, s.value
+
+    def count_ops(self, params: Params, world: World | None = None, steps: int = 1) -> dict:
+        """Floating-point operations of `steps` steps of a copy of this FP64 rod."""
+        out = (C.c_longlong * 5)()
+        pc = params.c()
+        wc = world.c() if world is not None else None
+        rc = self.L.orc_count_ops(self.h, C.byref(pc), C.byref(wc) if wc is not None else None, int(steps), out)
+        if rc != 0:
+            raise RuntimeError("op counting needs an FP64 oracle rod")
+        return dict(zip(("add", "mul", "div", "sqrt", "cmp"), list(out)))
 
     def self_gap(self, world: World) -> int:
         wc = world.c()

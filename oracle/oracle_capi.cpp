@@ -5,6 +5,7 @@
 #include <thread>
 #include <vector>
 
+#include "opcount.hpp"
 #include "rp_oracle.hpp"
 
 namespace {
@@ -222,6 +223,61 @@ void stepHolder(Holder<T>& h, const orc_params* p, const orc_world* w, int steps
 
 Handle* H(void* p) { return static_cast<Handle*>(p); }
 
+template <typename To, typename From>
+std::vector<To> conv(const std::vector<From>& v) {
+    std::vector<To> r(v.size());
+    for (size_t i = 0; i < v.size(); ++i) r[i] = To(double(v[i]));
+    return r;
+}
+template <typename To, typename From>
+std::vector<orc::V3<To>> convV(const std::vector<orc::V3<From>>& v) {
+    std::vector<orc::V3<To>> r(v.size());
+    for (size_t i = 0; i < v.size(); ++i) r[i] = orc::V3<To>(To(double(v[i].x)), To(double(v[i].y)), To(double(v[i].z)));
+    return r;
+}
+template <typename To, typename From>
+std::vector<orc::Q4<To>> convQ(const std::vector<orc::Q4<From>>& v) {
+    std::vector<orc::Q4<To>> r(v.size());
+    for (size_t i = 0; i < v.size(); ++i)
+        r[i] = orc::Q4<To>(To(double(v[i].w)), To(double(v[i].x)), To(double(v[i].y)), To(double(v[i].z)));
+    return r;
+}
+
+orc::Rod<orc::Cnt> countingRod(const orc::Rod<double>& s) {
+    using orc::Cnt;
+    orc::Rod<Cnt> r;
+    r.x = convV<Cnt>(s.x);
+    r.xPrev = convV<Cnt>(s.xPrev);
+    r.v = convV<Cnt>(s.v);
+    r.extForce = convV<Cnt>(s.extForce);
+    r.kinVel = convV<Cnt>(s.kinVel);
+    r.invMass = conv<Cnt>(s.invMass);
+    r.mass = conv<Cnt>(s.mass);
+    r.q = convQ<Cnt>(s.q);
+    r.qPrev = convQ<Cnt>(s.qPrev);
+    r.omega = convV<Cnt>(s.omega);
+    r.invInertia = convV<Cnt>(s.invInertia);
+    r.inertia = convV<Cnt>(s.inertia);
+    r.extTorque = convV<Cnt>(s.extTorque);
+    r.sP0 = s.sP0;
+    r.sP1 = s.sP1;
+    r.sSeg = s.sSeg;
+    r.sRest = conv<Cnt>(s.sRest);
+    r.sComp = convV<Cnt>(s.sComp);
+    r.sLam = convV<Cnt>(s.sLam);
+    r.bA = s.bA;
+    r.bB = s.bB;
+    r.bRest = conv<Cnt>(s.bRest);
+    r.bRestD = convV<Cnt>(s.bRestD);
+    r.bComp = convV<Cnt>(s.bComp);
+    r.bLam = convV<Cnt>(s.bLam);
+    r.mat.youngs = s.mat.youngs;
+    r.mat.poisson = s.mat.poisson;
+    r.mat.density = s.mat.density;
+    r.mat.radius = s.mat.radius;
+    return r;
+}
+
 }  // namespace
 
 extern "C" {
@@ -413,6 +469,28 @@ int orc_self_gap(void* rod, const orc_world* world) {
     Handle* h = H(rod);
     if (h->precision == 32) return orc::selfIndexGap(h->f.rod, worldOf<float>(*world));
     return orc::selfIndexGap(h->d.rod, worldOf<double>(*world));
+}
+
+int orc_count_ops(void* rod, const orc_params* p, const orc_world* world, int steps, long long out[5]) {
+    Handle* h = H(rod);
+    if (h->precision != 64) return -1;
+    orc::Rod<orc::Cnt> r = countingRod(h->d.rod);
+    orc::Context<orc::Cnt> ctx;
+    const orc::Params<orc::Cnt> pp = paramsOf<orc::Cnt>(*p);
+    orc::tally() = orc::OpTally();
+    if (world) {
+        const orc::World<orc::Cnt> w = worldOf<orc::Cnt>(*world);
+        for (int s = 0; s < steps; ++s) orc::step(r, pp, &w, ctx);
+    } else {
+        for (int s = 0; s < steps; ++s) orc::step(r, pp, static_cast<const orc::World<orc::Cnt>*>(nullptr), ctx);
+    }
+    const orc::OpTally t = orc::tally();
+    out[0] = t.add;
+    out[1] = t.mul;
+    out[2] = t.div;
+    out[3] = t.sqrt;
+    out[4] = t.cmp;
+    return 0;
 }
 
 int orc_colour(void* rod, int family, int* order, int* start, int maxColours) {

@@ -89,6 +89,10 @@ void orc_seg_seg(int precision, const double p1[3], const double q1[3], const do
 unsigned orc_hash_cell(int ix, int iy, int iz, int table);
 int orc_cell_coord(int precision, double v, double cell);
 int orc_self_gap(void* rod, const orc_world* world);
+/* Step a copy of an FP64 rod with the operation-counting scalar and report
+ * {adds (incl. subtracts), multiplies, divides, square roots, compares}.
+ * The rod itself is not advanced. Returns 0, or -1 for an FP32 rod. */
+int orc_count_ops(void* rod, const orc_params* p, const orc_world* world, int steps, long long out[5]);
 /* Greedy colouring of a rod's stretch (family 0) or bend (family 1)
  * constraints: writes order[count] and start[colours + 1] (start capacity
  * maxColours + 1); returns the colour count. */

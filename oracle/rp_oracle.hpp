@@ -37,6 +37,12 @@ inline float oSqrt(float v) { return std::sqrt(v); }
 inline double oSqrt(double v) { return std::sqrt(v); }
 inline float oFloor(float v) { return std::floor(v); }
 inline double oFloor(double v) { return std::floor(v); }
+inline float oExp(float v) { return std::exp(v); }
+inline double oExp(double v) { return std::exp(v); }
+inline float oCos(float v) { return std::cos(v); }
+inline double oCos(double v) { return std::cos(v); }
+inline float oSin(float v) { return std::sin(v); }
+inline double oSin(double v) { return std::sin(v); }
 
 template <typename T>
 constexpr T kPiT = T(3.14159265358979323846);
@@ -228,7 +234,7 @@ template <typename T> inline Q4<T> quatFromMat(const M3<T>& R) {
 
 template <typename T> inline Q4<T> quatAxisAngle(V3<T> axis, T angle) {
     const T h = angle * T(0.5);
-    return Q4<T>(std::cos(h), normalize(axis) * std::sin(h));
+    return Q4<T>(oCos(h), normalize(axis) * oSin(h));
 }
 
 template <typename T> inline Q4<T> hemi(Q4<T> q, Q4<T> ref) { return dot(q, ref) < T(0) ? q * T(-1) : q; }
@@ -689,7 +695,7 @@ struct Hash {
 template <typename T> inline int selfIndexGap(const Rod<T>& rod, const World<T>& w) {
     T lMin = rod.sRest.empty() ? T(1) : rod.sRest[0];
     for (T l : rod.sRest) lMin = std::min(lMin, l);
-    const int geometric = int(std::floor(2 * rod.mat.radius / lMin)) + 2;
+    const int geometric = static_cast<int>(oFloor(T(2) * rod.mat.radius / lMin)) + 2;
     return std::max(w.selfSkip, geometric);
 }
 
@@ -907,8 +913,8 @@ inline void substep(Rod<T>& rod, const Params<T>& p, T h, std::vector<Contact<T>
         projectBend(rod, h, p.complianceScale, bc);
         if (contacts) projectContacts(rod, *contacts);
     }
-    const T linDecay = std::exp(-p.linDamp * h);
-    const T angDecay = std::exp(-p.angDamp * h);
+    const T linDecay = oExp(-p.linDamp * h);
+    const T angDecay = oExp(-p.angDamp * h);
     for (int i = 0; i < nP; ++i) {
         if (rod.invMass[i] == T(0)) {
             rod.v[i] = rod.kinVel[i];

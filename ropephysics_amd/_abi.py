@@ -56,7 +56,8 @@ class RpCounters(C.Structure):
     _fields_ = [("self_contact_overflow", C.c_longlong), ("strategy", C.c_int), ("tile_config", C.c_int),
                 ("workgroups", C.c_int), ("lanes_busy", C.c_int), ("lanes_total", C.c_int),
                 ("lds_bytes_per_workgroup", C.c_size_t), ("launches_per_step", C.c_int),
-                ("steps_taken", C.c_longlong), ("device_bytes", C.c_size_t)]
+                ("steps_taken", C.c_longlong), ("device_bytes", C.c_size_t), ("last_step_gpu_ms", C.c_float),
+                ("last_step_launches", C.c_int)]
 
 
 # Every symbol include/ropephysics.h declares, with its signature.
@@ -71,6 +72,7 @@ SIGNATURES = {
     "rp_batch_sync": (C.c_int, [C.c_void_p]),
     "rp_batch_get_state": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _fp, _fp, _fp, _fp]),
     "rp_batch_counters": (C.c_int, [C.c_void_p, C.POINTER(RpCounters)]),
+    "rp_batch_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "rp_tile_config_name": (C.c_char_p, [C.c_int]),
     "rp_tile_config_count": (C.c_int, []),
     "rp_last_error": (C.c_char_p, []),

@@ -40,7 +40,7 @@ def _world_desc(w: CollisionWorld):
         c.friction = float(p.friction)
     d = _abi.RpWorldDesc(len(w.primitives), C.cast(prims, C.POINTER(_abi.RpPrimitive)),
                          float(w.contact_margin_radii), int(w.self_collision), float(w.self_friction),
-                         int(w.self_skip), int(w.hash_table_size), 0)
+                         int(w.self_skip), int(w.hash_table_size), int(w.self_capacity))
     return d, prims
 
 
@@ -213,6 +213,10 @@ class Batch:
     def step(self, params: SolverParams, steps: int = 1):
         sp = step_params(params)
         _abi.check(_abi.lib().rp_batch_step(self._h, C.byref(sp), int(steps)))
+
+    def set_timing(self, enable: bool = True):
+        """Time every step call with HIP events on the batch's stream."""
+        _abi.check(_abi.lib().rp_batch_set_timing(self._h, int(enable)))
 
     def synchronize(self):
         _abi.check(_abi.lib().rp_batch_sync(self._h))

@@ -124,9 +124,16 @@ struct rp_batch_impl {
     long long stepsTaken = 0;
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::vector<int> colStart;  // host copy of the colour-start table
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool evPending = false;
+    int lastLaunches = 0;
+    float lastMs = 0.0f;
 
     ~rp_batch_impl() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -818,46 +825,75 @@ int rp_batch_step(void* batch, const rp_step_params* params, int num_steps) {
     if (num_steps == 0) return RP_OK;
     int rc = 0;
     if (b.strategy == RP_STRATEGY_TILED && (rc = resolve_tiles(b, p))) return rc;
-    // Replay the launch sequence from a HIP graph cached per parameter set.
-    GraphKey key;
-    std::memset(&key, 0, sizeof(key));
-    key.strategy = b.strategy;
-    key.config = b.tiles ? b.tiles->config : 0;
-    key.steps = num_steps;
-    key.dt = p.dt;
-    key.gx = p.gravity[0];
-    key.gy = p.gravity[1];
-    key.gz = p.gravity[2];
-    key.lin = p.linear_damping;
-    key.ang = p.angular_damping;
-    key.substeps = p.substeps;
-    key.iterations = p.iterations;
-    key.interval = p.contact_interval;
-    auto it = b.graphs.find(key);
-    if (it == b.graphs.end()) {
-        hipGraph_t g;
-        RP_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
-        rc = launch_steps(b, p, num_steps, b.strategy, b.tiles.get());
-        hipGraph_t captured = nullptr;
-        const hipError_t ec = hipStreamEndCapture(b.stream, &captured);
-        if (rc) {
-            if (captured) (void)hipGraphDestroy(captured);
-            return rc;
-        }
-        if (ec != hipSuccess) return fail(RP_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
-        g = captured;
-        hipGraphExec_t exec;
-        const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ei != hipSuccess) return fail(RP_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
-        if (b.graphs.size() > 64) {
-            for (auto& kv : b.graphs) (void)hipGraphExecDestroy(kv.second);
-            b.graphs.clear();
-        }
-        it = b.graphs.emplace(key, exec).first;
+    // Steps run in chunks of stepsPerLaunch: one TILED launch, or that many
+    // PLAIN launches. Each chunk shape is captured once into a HIP graph
+    // (cached per parameter set) and replayed.
+    const int chunk = b.stepsPerLaunch;
+    const int full = num_steps / chunk, rem = num_steps % chunk;
+    hipGraphExec_t gFull = nullptr, gRem = nullptr;
+    if (b.graphs.size() > 62) {  // bound the cache; a call adds at most two graphs
+        RP_HIP(hipStreamSynchronize(b.stream));
+        for (auto& kv : b.graphs) (void)hipGraphExecDestroy(kv.second);
+        b.graphs.clear();
     }
-    RP_HIP(hipGraphLaunch(it->second, b.stream));
+    for (int pass = 0; pass < 2; ++pass) {
+        const int n = pass == 0 ? chunk : rem;
+        if ((pass == 0 && full == 0) || n == 0) continue;
+        GraphKey key;
+        std::memset(&key, 0, sizeof(key));
+        key.strategy = b.strategy;
+        key.config = b.tiles ? b.tiles->config : 0;
+        key.steps = n;
+        key.dt = p.dt;
+        key.gx = p.gravity[0];
+        key.gy = p.gravity[1];
+        key.gz = p.gravity[2];
+        key.lin = p.linear_damping;
+        key.ang = p.angular_damping;
+        key.substeps = p.substeps;
+        key.iterations = p.iterations;
+        key.interval = p.contact_interval;
+        auto it = b.graphs.find(key);
+        if (it == b.graphs.end()) {
+            RP_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+            rc = launch_steps(b, p, n, b.strategy, b.tiles.get());
+            hipGraph_t captured = nullptr;
+            const hipError_t ec = hipStreamEndCapture(b.stream, &captured);
+            if (rc) {
+                if (captured) (void)hipGraphDestroy(captured);
+                return rc;
+            }
+            if (ec != hipSuccess) return fail(RP_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+            hipGraphExec_t exec;
+            const hipError_t ei = hipGraphInstantiate(&exec, captured, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(captured);
+            if (ei != hipSuccess)
+                return fail(RP_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
+            it = b.graphs.emplace(key, exec).first;
+        }
+        (pass == 0 ? gFull : gRem) = it->second;
+    }
+    if (b.timing) RP_HIP(hipEventRecord(b.ev0, b.stream));
+    for (int i = 0; i < full; ++i) RP_HIP(hipGraphLaunch(gFull, b.stream));
+    if (rem) RP_HIP(hipGraphLaunch(gRem, b.stream));
+    if (b.timing) {
+        RP_HIP(hipEventRecord(b.ev1, b.stream));
+        b.evPending = true;
+    }
+    b.lastLaunches = b.strategy == RP_STRATEGY_PLAIN ? num_steps
+                                                     : (num_steps + b.stepsPerLaunch - 1) / b.stepsPerLaunch;
     b.stepsTaken += num_steps;
+    return RP_OK;
+}
+
+int rp_batch_set_timing(void* batch, int enable) {
+    if (!batch) return fail(RP_ERR_INVALID, "null batch");
+    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
+    if (enable && !b.ev0) {
+        RP_HIP(hipEventCreate(&b.ev0));
+        RP_HIP(hipEventCreate(&b.ev1));
+    }
+    b.timing = enable != 0;
     return RP_OK;
 }
 
@@ -890,6 +926,12 @@ int rp_batch_counters(void* batch, rp_counters* out) {
     RP_HIP(hipStreamSynchronize(b.stream));
     RP_HIP(hipMemcpy(ovf.data(), b.d.selfOverflow, b.numRods * 4, hipMemcpyDeviceToHost));
     for (int v : ovf) out->self_contact_overflow += v;
+    if (b.timing && b.evPending) {
+        RP_HIP(hipEventElapsedTime(&b.lastMs, b.ev0, b.ev1));
+        b.evPending = false;
+    }
+    out->last_step_gpu_ms = b.lastMs;
+    out->last_step_launches = b.lastLaunches;
     out->strategy = b.strategy;
     out->steps_taken = b.stepsTaken;
     out->device_bytes = b.deviceBytes;

@@ -426,6 +426,9 @@ class CollisionWorld:
     self_skip: int = 2
     contact_margin_radii: float = 0.25
     hash_table_size: int = 4096
+    # Self contacts stored per rod by the batch (0: max(16, segments / 2), the
+    # reference's pool); contacts beyond it are counted as overflow.
+    self_capacity: int = 0
 
 
 @dataclass

@@ -1,0 +1,296 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+The bar, per scenario:
+  * bit for bit equal to the oracle's FP32 instantiation run in the GPU's
+    colour order (the arithmetic order is the reference's, contraction off on
+    both sides), every field, every step checked;
+  * within tolerance of the FP64 oracle: 1e-7 m after one step (the
+    reference's own one-step bar, src/gpu/gpu_cases.cpp:285) and within the
+    float envelope over long horizons;
+  * both strategies, every tile shape, and a rod alone or packed among others
+    give identical bits; repeated runs are bitwise identical.
+Reference scenarios follow src/gpu/gpu_cases.cpp:126-548.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import ropephysics_amd as rp
+from helpers import (REFERENCE_MATERIAL, benchmark_params, benchmark_rod, bits, compare_bitwise,
+                     compliance_scale_f32, max_pos_err, oracle_params, oracle_state, oracle_twin, oracle_world)
+
+pytestmark = pytest.mark.gpu
+
+STRATEGIES = ["plain", "tiled"]
+
+CONTACT_WORLD = rp.CollisionWorld(primitives=[
+    rp.Primitive.plane((0, 0, -0.0051), (0, 0, 1), 0.5),
+    rp.Primitive.sphere((0.5, 0, -0.0851), 0.08, 0.3),
+])
+
+ALL_PRIMS_WORLD = rp.CollisionWorld(primitives=[
+    rp.Primitive.plane((0, 0, -0.0051), (0, 0, 1), 0.5),
+    rp.Primitive.sphere((0.5, 0, -0.0851), 0.08, 0.3),
+    rp.Primitive.capsule((0.25, -0.1, -0.016), (0.25, 0.1, -0.016), 0.01, 0.4),
+    rp.Primitive.box((0.8, 0.0, -0.026), (0.05, 0.05, 0.02), (0.9659258262890683, 0, 0, 0.25881904510252074), 0.6),
+])
+
+
+def run_pair(rods, world, params, steps, strategy, checkpoints=None, scales=None, tile_config=0):
+    """Step rods on the GPU and each rod on the FP32 oracle; return the number
+    of differing FP32 words per field, summed over rods and checkpoints, plus
+    the worst FP64 error at the end."""
+    b = rp.Batch(rods, strategy=strategy, world=world, tile_config=tile_config)
+    ys, fs = (None, None) if scales is None else scales
+    if scales is not None:
+        b.set_material_scales(ys, fs)
+    checkpoints = checkpoints or [steps]
+    twins = []
+    for i, r in enumerate(rods):
+        cs = compliance_scale_f32(ys[i]) if ys is not None else 1.0
+        f = float(np.float32(fs[i])) if fs is not None else 1.0
+        twins.append((oracle_twin(r, 32), oracle_twin(r, 64), oracle_params(params, compliance_scale=cs,
+                                                                             friction_scale=f)))
+    ow = oracle_world(world)
+    diffs = {"x": 0, "v": 0, "q": 0, "omega": 0}
+    done = 0
+    for cp in checkpoints:
+        b.step(params, cp - done)
+        for o32, o64, op in twins:
+            o32.step(op, ow, cp - done)
+            o64.step(op, ow, cp - done)
+        done = cp
+        b.synchronize()
+        for i, (o32, _, _) in enumerate(twins):
+            d = compare_bitwise(b.download(i), oracle_state(o32))
+            for k in diffs:
+                diffs[k] += d[k]
+    err64 = max(max_pos_err(b.download(i)["x"], o64.get()["x"]) for i, (_, o64, _) in enumerate(twins))
+    return diffs, err64, b
+
+
+def assert_bitwise(diffs):
+    assert all(v == 0 for v in diffs.values()), diffs
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
+@pytest.mark.parametrize("n", [16, 48, 128])
+def test_gravity_cantilever(strategy, n):
+    diffs, err64, _ = run_pair([benchmark_rod(n)], None, benchmark_params(), 200, strategy,
+                               checkpoints=[1, 10, 50, 200])
+    assert_bitwise(diffs)
+    assert err64 < 1e-3  # reference: float drift at 200 steps (gpu_cases.cpp:303-304)
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
+@pytest.mark.parametrize("n", [16, 48, 128])
+def test_one_step_within_1e7_of_fp64(strategy, n):
+    _, err64, _ = run_pair([benchmark_rod(n)], None, benchmark_params(), 1, strategy)
+    assert err64 < 1e-7
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
+def test_loaded_driven_rod(strategy):
+    """Tip force, mid torque, a pinned root moving kinematically and the root's
+    ghost frame twisted a little every step through set_loads
+    (gpu_cases.cpp:205-228)."""
+    n = 48
+    rod = benchmark_rod(n)
+    rod.extForce[-1] = (0.0, 0.02, 0.05)
+    rod.extTorque[n // 2] = (2e-3, 0.0, -1e-3)
+    rod.kinVel[0] = (0.0, 0.05, 0.1)
+    ghost = rod.num_frames - 1
+    rest = tuple(rod.q[ghost])
+    params = benchmark_params()
+    b = rp.Batch([rod], strategy=strategy)
+    o32 = oracle_twin(rod, 32)
+    op = oracle_params(params)
+    for s in range(60):
+        qd = rp.rod._qnormalize(rp.rod._qmul(rp.quat_axis_angle((1, 0, 0), 0.5e-3 * (s + 1)), rest))
+        rod.q[ghost] = qd
+        b.set_loads(0, rod)
+        a = o32.get()
+        a["q"][ghost] = np.float32(np.array(qd))
+        o32.set({"q": a["q"]})
+        b.step(params, 1)
+        o32.step(op, None, 1)
+    b.synchronize()
+    assert_bitwise(compare_bitwise(b.download(0), oracle_state(o32)))
+    # The loads matter: the rod is far from the unloaded one.
+    plain = oracle_twin(benchmark_rod(n), 32)
+    plain.step(op, None, 60)
+    assert max_pos_err(b.download(0)["x"], plain.get()["x"]) > 1e-3
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
+def test_contact_floor_and_sphere(strategy):
+    diffs, _, _ = run_pair([benchmark_rod(48)], CONTACT_WORLD, benchmark_params(), 200, strategy,
+                           checkpoints=[1, 50, 200])
+    assert_bitwise(diffs)
+    o = oracle_twin(benchmark_rod(48), 32)
+    o.step(oracle_params(benchmark_params()), oracle_world(CONTACT_WORLD), 50)
+    assert o.contact_counts()[0] >= 5  # the scene really is in contact
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
+def test_all_primitive_types(strategy):
+    rods = [benchmark_rod(40), benchmark_rod(64)]
+    diffs, _, _ = run_pair(rods, ALL_PRIMS_WORLD, benchmark_params(iterations=2), 150, strategy,
+                           checkpoints=[20, 150])
+    assert_bitwise(diffs)
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
+def test_per_rod_stiffness_and_friction_scales(strategy):
+    """setMaterialScales: stiffness on a swinging cantilever, friction on an
+    incline that holds at mu = 0.5 and slides at half (gpu_cases.cpp:316-419)."""
+    rods = [benchmark_rod(16), benchmark_rod(16)]
+    diffs, _, b = run_pair(rods, None, benchmark_params(), 20, strategy, scales=([1.0, 3.0], None))
+    assert_bitwise(diffs)
+    x = b.get_state()["x"]
+    assert max_pos_err(x[:17], x[17:]) > 1e-3
+    tilt = np.deg2rad(20.0)
+    normal = (np.sin(tilt), 0.0, np.cos(tilt))
+    incline = rp.CollisionWorld(primitives=[rp.Primitive.plane((0, 0, 0), normal, 0.5)])
+    start = tuple(np.array(normal) * (REFERENCE_MATERIAL.radius + 1e-4) + np.array([0, -0.5, 0]))
+    rod = rp.make_straight_rod(16, 1.0, REFERENCE_MATERIAL, start, (0, 1, 0))
+    diffs, _, b = run_pair([rod, rod], incline, benchmark_params(), 150, strategy, scales=([1.0, 1.0], [1.0, 0.5]))
+    assert_bitwise(diffs)
+    x = b.get_state()["x"]
+    assert max_pos_err(x[:17], x[17:]) > 1e-3
+
+
+def helix_rope():
+    n_turn, turns = 20, 3
+    t = np.arange(n_turn * turns + 1) / n_turn * 2 * np.pi
+    pts = np.stack([0.03 * np.cos(t), 0.03 * np.sin(t), 0.02 + 0.0095 * t / (2 * np.pi)], axis=-1)
+    return rp.make_rod_from_centerline(pts, rp.RodMaterial(2e6, 0.35, 1000.0, 5e-3), (0, 0, 1))
+
+
+SELF_WORLD = rp.CollisionWorld(primitives=[rp.Primitive.plane((0, 0, 0), (0, 0, 1), 0.6)], self_collision=True,
+                               self_friction=0.3, hash_table_size=256)
+SELF_PARAMS = rp.SolverParams(dt=1e-3, substeps=16, iterations=4, gravity=(0, 0, -9.81), linear_damping=1.0,
+                              angular_damping=1.0)
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
+def test_self_collision_coil(strategy):
+    rope = helix_rope()
+    diffs, _, b = run_pair([rope], SELF_WORLD, SELF_PARAMS, 40, strategy, checkpoints=[1, 5, 40])
+    assert_bitwise(diffs)
+    assert b.self_contact_overflow() == 0
+    no_self = rp.CollisionWorld(primitives=SELF_WORLD.primitives)
+    o = oracle_twin(rope, 32)
+    o.step(oracle_params(SELF_PARAMS), oracle_world(no_self), 40)
+    assert max_pos_err(b.download(0)["x"], o.get()["x"]) > 1e-3  # self-collision mattered
+
+
+def test_self_collision_overflow_is_counted():
+    """Contacts beyond the per-rod pool are counted, never silently dropped
+    (gpu_solver.cu:311-320)."""
+    rope = helix_rope()
+    b = rp.Batch([rope], strategy="plain", world=SELF_WORLD)
+    b.step(SELF_PARAMS, 2)
+    assert b.self_contact_overflow() == 0
+    tiny = rp.CollisionWorld(primitives=SELF_WORLD.primitives, self_collision=True, self_friction=0.3,
+                             hash_table_size=256, self_capacity=1)
+    for strategy in STRATEGIES:
+        tb = rp.Batch([rope], strategy=strategy, world=tiny)
+        tb.step(SELF_PARAMS, 2)
+        assert tb.self_contact_overflow() > 0
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
+def test_ragged_batch_and_alone_equals_mixed(strategy):
+    sizes = [7, 20, 33, 64, 97, 130, 160, 48]
+    rods = [benchmark_rod(n) for n in sizes]
+    diffs, _, b = run_pair(rods, CONTACT_WORLD, benchmark_params(), 30, strategy)
+    assert_bitwise(diffs)
+    mixed = b.get_state()
+    for i in (0, 3, 6):
+        alone = rp.Batch([rods[i]], strategy=strategy, world=CONTACT_WORLD)
+        alone.step(benchmark_params(), 30)
+        s = alone.download(0)
+        for k in ("x", "v", "q", "omega"):
+            part = b.rod_slice(mixed[k], i, "particle" if k in ("x", "v") else "frame")
+            assert np.array_equal(bits(part), bits(s[k]))
+
+
+def test_every_tile_configuration_gives_the_same_bits():
+    rods = [benchmark_rod(n) for n in (12, 40, 64, 90)]
+    ref = None
+    for cfg in rp.tile_configs():
+        try:
+            b = rp.Batch(rods, strategy="tiled", world=CONTACT_WORLD, tile_config=cfg)
+            b.step(benchmark_params(), 12)
+        except rp.RopePhysicsError as e:
+            assert e.code == -4  # this shape cannot hold these rods
+            continue
+        s = b.get_state()
+        if ref is None:
+            ref = s
+        else:
+            for k in s:
+                assert np.array_equal(bits(s[k]), bits(ref[k])), (cfg, k)
+    assert ref is not None
+
+
+def test_plain_equals_tiled_and_runs_repeat_bitwise():
+    rods = [benchmark_rod(n) for n in (32, 128)] * 8
+    outs = []
+    for strategy in ("plain", "tiled", "tiled", "tiled"):
+        b = rp.Batch(rods, strategy=strategy)
+        b.step(benchmark_params(), 100)
+        outs.append(b.get_state())
+    for o in outs[1:]:
+        for k in o:
+            assert np.array_equal(bits(o[k]), bits(outs[0][k]))
+
+
+def test_full_size_uniform_batch_sampled_rods_bitwise():
+    """The 4096 x 64 floor workload at full size for 30 steps: a sample of rods
+    against the FP32 oracle, bit for bit; every rod finite and above the floor."""
+    from ropephysics_amd import scenes
+
+    w = scenes.uniform_floor_batch(steps=30)
+    b = rp.Batch(w.prototype, w.num_rods, strategy="tiled", world=w.world)
+    b.set_material_scales(w.youngs_scale, w.friction_scale)
+    R = w.num_rods
+    b.set_state(0, R, x=w.x0, v=np.zeros_like(w.x0), q=np.tile(w.prototype.q, (R, 1)),
+                omega=np.tile(w.prototype.omega, (R, 1)))
+    b.step(w.params, 30)
+    s = b.get_state()
+    assert np.all(np.isfinite(s["x"]))
+    assert s["x"][:, 2].min() > -w.prototype.material.radius
+    nP = w.prototype.num_particles
+    ow = oracle_world(w.world)
+    for r in (0, 1, 777, 2048, R - 1):
+        rod = w.prototype.copy()
+        rod.x = w.x0[r * nP:(r + 1) * nP].copy()
+        rod.xPrev = rod.x.copy()
+        o = oracle_twin(rod, 32)
+        o.step(oracle_params(w.params, compliance_scale=compliance_scale_f32(w.youngs_scale[r]),
+                             friction_scale=float(np.float32(w.friction_scale[r]))), ow, 30)
+        got = b.download(r)
+        assert_bitwise(compare_bitwise(got, oracle_state(o)))
+
+
+def test_invalid_inputs_are_rejected():
+    rod = benchmark_rod(8)
+    b = rp.Batch(rod, 2)
+    with pytest.raises(ValueError):
+        b.set_state(0, 1, x=np.zeros((3, 3)))
+    with pytest.raises(rp.RopePhysicsError):
+        b.get_state(1, 5)
+    with pytest.raises(rp.RopePhysicsError):
+        b.set_material_scales([1.0, -1.0], None)
+    with pytest.raises(ValueError):
+        b.set_material_scales([1.0], None)
+    bad = rod.copy()
+    bad.sP1 = bad.sP1.copy()
+    bad.sP1[3] = 999
+    with pytest.raises(rp.RopePhysicsError) as e:
+        rp.Batch([bad])
+    assert e.value.code == -1
+    with pytest.raises(rp.RopePhysicsError):
+        b.step(rp.SolverParams(substeps=0), 1)
