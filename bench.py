@@ -72,6 +72,54 @@ def time_steps(b: rp.Batch, params, steps: int):
     return t1 - t0, c
 
 
+def make_ragged_batch(w: scenes.Workload, strategy: str) -> rp.Batch:
+    b = rp.Batch(w.rods, strategy=strategy, world=w.world)
+    b.set_material_scales(w.youngs_scale, w.friction_scale)
+    if "paths" in w.info:
+        b.set_paths(0, w.info["paths"])
+    return b
+
+
+def reset_ragged(b: rp.Batch, w: scenes.Workload):
+    b.upload_all(w.rods)
+    b.set_clock()
+
+
+def ragged_comparison(steps: int, plain_steps: int, warmup: int) -> dict:
+    """The ragged routing workload (1024 cables of 48-160 segments with
+    self-collision, gripper paths on the device), TILED and PLAIN, each from
+    the workload's initial state over its first `steps` / `plain_steps`
+    steps."""
+    w = scenes.routing_rollouts()
+    log(f"ragged workload {w.name}: {w.num_rods} rods, {w.segments_total} segments")
+    out = {"workload": w.name, "rods": w.num_rods, "segments_total": w.segments_total,
+           "segments": "48-160 uniform", "self_collision": True, "substeps": w.params.substeps}
+    for strategy, k in (("tiled", steps), ("plain", plain_steps)):
+        b = make_ragged_batch(w, strategy)
+        reset_ragged(b, w)
+        b.step(w.params, min(warmup, 4))
+        b.synchronize()
+        reset_ragged(b, w)
+        b.set_timing(True)
+        secs, c = time_steps(b, w.params, k)
+        work = float(w.segments_total) * w.params.substeps * k
+        ent = {"value": work / secs, "steps": k, "seconds": secs,
+               "kernel_ms_avg": c["last_step_gpu_ms"] / max(1, c["last_step_launches"]),
+               "self_contact_overflow": c["self_contact_overflow"]}
+        if strategy == "tiled":
+            ent.update({"tile_config": c.get("tile_config_name"), "workgroups": c["workgroups"],
+                        "lanes_busy": c["lanes_busy"], "lanes_total": c["lanes_total"],
+                        "lds_bytes_per_workgroup": c["lds_bytes_per_workgroup"]})
+            x = b.download_positions()
+            if not np.all(np.isfinite(x)):
+                raise RuntimeError("non-finite state in the ragged run")
+        log(f"ragged {strategy}: {secs:.3f} s for {k} steps, {ent['value'] / 1e9:.3f} G seg-substeps/s")
+        out[strategy] = ent
+        b.destroy()
+    out["speedup_vs_plain"] = out["tiled"]["value"] / out["plain"]["value"]
+    return out
+
+
 def cpu_baseline(w: scenes.Workload, stride: int, threads: int):
     """The reference CPU path (its own sources, built into oracle/_ref) on every
     `stride`-th rod of the workload for all its steps, FP64, on `threads` host
@@ -127,6 +175,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--tile-config", type=int, default=0)
+    ap.add_argument("--ragged-steps", type=int, default=1000, help="steps of the ragged comparison (0: skip)")
+    ap.add_argument("--ragged-plain-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one, and this path has no cross-GPU "
@@ -210,6 +260,10 @@ def main():
         roofline["traffic"] = tr.get("hbm_bytes_per_launch")
         roofline["traffic_source"] = os.path.relpath(traffic_path, HERE)
 
+    ragged = None
+    if args.ragged_steps > 0:
+        ragged = ragged_comparison(args.ragged_steps, args.ragged_plain_steps, W)
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -224,6 +278,7 @@ def main():
         "tiled": tiled_info,
         "plain": plain,
         "speedup_vs_plain": (rate / plain["value"]) if plain else None,
+        "ragged": ragged,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "device": {"arch": info["arch"], "compute_units": info["compute_units"]},

@@ -194,6 +194,48 @@ int rp_batch_get_state(void* batch, int first_rod, int num_rods, float* x, float
  * (src/gpu/gpu_batch.cpp:140-148,477-484). */
 int rp_batch_counters(void* batch, rp_counters* out);
 
+/* A kinematically driven particle per rod (ragged extension of
+ * setKinematicVelocities, src/gpu/gpu_batch.cpp:418-432, with the gripper
+ * schedule of HarnessTask::gripperAt/gripperVelocity,
+ * src/apps/harness_routing.cpp:69-92, evaluated on the device in FP64):
+ * the gripper starts at `start`, blends to waypoint k over leg_seconds[k]
+ * with a smoothstep, and holds at the last waypoint; during the rod's step s
+ * the particle's kinematic velocity is (P(t1) - P(t0)) / control_period with
+ * t0 = floor(s / steps_per_control) * control_period and t1 = t0 +
+ * control_period, P at height `height`. The particle must be pinned. */
+typedef struct rp_path_desc {
+    int particle;            /* rod-local particle index; -1 clears the rod's path */
+    int num_waypoints;       /* 1..16 */
+    const double* waypoints; /* num_waypoints * 2 (x, y) */
+    const double* leg_seconds; /* num_waypoints */
+    double start[2];
+    double height;
+    double control_period;   /* seconds; the reference uses dt * steps_per_control */
+    int steps_per_control;
+} rp_path_desc;
+
+/* Install paths for rods [first_rod, first_rod + num_rods); paths[i] belongs
+ * to rod first_rod + i. */
+int rp_batch_set_paths(void* batch, int first_rod, int num_rods, const rp_path_desc* paths);
+
+/* Per-rod step clock (steps taken since it was last set), which paths read.
+ * clock == NULL resets to 0. */
+int rp_batch_set_clock(void* batch, int first_rod, int num_rods, const int* clock);
+int rp_batch_get_clock(void* batch, int first_rod, int num_rods, int* clock);
+
+/* A hold/slide observer per rod (HangOutcome, src/apps/cable_hanging.h:68-77,
+ * sampled as heldOnGpu does, src/validation/application_cases.cpp:55-65):
+ * whenever the rod's clock reaches a multiple of `every`, the observer takes
+ * max_rise = max(max_rise, z(particle) - z0) in FP64. Setting an observer
+ * resets its max_rise to 0. */
+typedef struct rp_observer_desc {
+    int particle;  /* rod-local particle index; -1 clears */
+    int every;     /* steps between samples, >= 1 */
+    double z0;     /* reference height */
+} rp_observer_desc;
+int rp_batch_set_observers(void* batch, int first_rod, int num_rods, const rp_observer_desc* observers);
+int rp_batch_get_observations(void* batch, int first_rod, int num_rods, double* max_rise);
+
 /* Record HIP events around every rp_batch_step on the batch's stream
  * (enable = 1) so rp_counters reports its device time. Off by default. */
 int rp_batch_set_timing(void* batch, int enable);

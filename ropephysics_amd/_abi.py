@@ -46,6 +46,16 @@ class RpStepParams(C.Structure):
                 ("linear_damping", C.c_float), ("angular_damping", C.c_float), ("contact_interval", C.c_int)]
 
 
+class RpPathDesc(C.Structure):
+    _fields_ = [("particle", C.c_int), ("num_waypoints", C.c_int), ("waypoints", C.POINTER(C.c_double)),
+                ("leg_seconds", C.POINTER(C.c_double)), ("start", C.c_double * 2), ("height", C.c_double),
+                ("control_period", C.c_double), ("steps_per_control", C.c_int)]
+
+
+class RpObserverDesc(C.Structure):
+    _fields_ = [("particle", C.c_int), ("every", C.c_int), ("z0", C.c_double)]
+
+
 class RpDeviceInfo(C.Structure):
     _fields_ = [("name", C.c_char * 256), ("arch", C.c_char * 64), ("compute_units", C.c_int),
                 ("wavefront_size", C.c_int), ("lds_per_workgroup", C.c_size_t), ("total_global_mem", C.c_size_t),
@@ -73,6 +83,11 @@ SIGNATURES = {
     "rp_batch_get_state": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _fp, _fp, _fp, _fp]),
     "rp_batch_counters": (C.c_int, [C.c_void_p, C.POINTER(RpCounters)]),
     "rp_batch_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "rp_batch_set_paths": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(RpPathDesc)]),
+    "rp_batch_set_clock": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip]),
+    "rp_batch_get_clock": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip]),
+    "rp_batch_set_observers": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(RpObserverDesc)]),
+    "rp_batch_get_observations": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "rp_tile_config_name": (C.c_char_p, [C.c_int]),
     "rp_tile_config_count": (C.c_int, []),
     "rp_last_error": (C.c_char_p, []),

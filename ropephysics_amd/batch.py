@@ -214,6 +214,51 @@ class Batch:
         sp = step_params(params)
         _abi.check(_abi.lib().rp_batch_step(self._h, C.byref(sp), int(steps)))
 
+    # ------------------------------------------------------------------ drivers and observers
+
+    def set_paths(self, first: int, paths: list):
+        """Gripper paths for rods first.. (one `GripperPath` or None per rod)."""
+        descs = (_abi.RpPathDesc * max(1, len(paths)))()
+        keep = []
+        for i, p in enumerate(paths):
+            if p is None:
+                descs[i].particle = -1
+                continue
+            wp = np.ascontiguousarray(np.asarray(p.waypoints, np.float64).reshape(-1, 2))
+            legs = np.ascontiguousarray(np.asarray(p.leg_seconds, np.float64))
+            keep += [wp, legs]
+            descs[i] = _abi.RpPathDesc(int(p.particle), len(legs), wp.ctypes.data_as(C.POINTER(C.c_double)),
+                                       legs.ctypes.data_as(C.POINTER(C.c_double)),
+                                       (C.c_double * 2)(*map(float, p.start)), float(p.height),
+                                       float(p.control_period), int(p.steps_per_control))
+        _abi.check(_abi.lib().rp_batch_set_paths(self._h, first, len(paths), descs))
+
+    def set_clock(self, first: int = 0, count: int | None = None, clock=None):
+        count = self.num_rods - first if count is None else count
+        c = None if clock is None else np.ascontiguousarray(clock, np.int32)
+        _abi.check(_abi.lib().rp_batch_set_clock(self._h, first, count, _abi.iptr(c)))
+
+    def get_clock(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        count = self.num_rods - first if count is None else count
+        out = np.empty(count, np.int32)
+        _abi.check(_abi.lib().rp_batch_get_clock(self._h, first, count, _abi.iptr(out)))
+        return out
+
+    def set_observers(self, first: int, observers: list):
+        """(particle, every, z0) per rod, or None."""
+        descs = (_abi.RpObserverDesc * max(1, len(observers)))()
+        for i, o in enumerate(observers):
+            descs[i] = _abi.RpObserverDesc(-1, 1, 0.0) if o is None else _abi.RpObserverDesc(int(o[0]), int(o[1]),
+                                                                                                   float(o[2]))
+        _abi.check(_abi.lib().rp_batch_set_observers(self._h, first, len(observers), descs))
+
+    def get_observations(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        count = self.num_rods - first if count is None else count
+        out = np.empty(count, np.float64)
+        _abi.check(_abi.lib().rp_batch_get_observations(self._h, first, count,
+                                                        out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
     def set_timing(self, enable: bool = True):
         """Time every step call with HIP events on the batch's stream."""
         _abi.check(_abi.lib().rp_batch_set_timing(self._h, int(enable)))

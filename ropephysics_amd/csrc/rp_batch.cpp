@@ -129,6 +129,16 @@ struct rp_batch_impl {
     bool evPending = false;
     int lastLaunches = 0;
     float lastMs = 0.0f;
+    PathRec* paths = nullptr;
+    ObsRec* obs = nullptr;
+    int* clock = nullptr;
+
+    // Device pointers are baked into captured graphs; drop them when one changes.
+    void drop_graphs() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
 
     ~rp_batch_impl() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
@@ -677,8 +687,10 @@ int rp_batch_create(const rp_batch_desc* desc, void** out_batch) {
     if ((rc = b->alloc(x, size_t(P) * 3)) || (rc = b->alloc(v, size_t(P) * 3)) || (rc = b->alloc(q, size_t(F) * 4)) ||
         (rc = b->alloc(om, size_t(F) * 3)) || (rc = b->alloc(force, size_t(P) * 3)) ||
         (rc = b->alloc(kin, size_t(P) * 3)) || (rc = b->alloc(torque, size_t(F) * 3)) || (rc = b->alloc(cs, R)) ||
-        (rc = b->alloc(fs, R)) || (rc = b->alloc(ovf, R)) || (rc = b->alloc(b->fixedStage, size_t(F) * 4)))
+        (rc = b->alloc(fs, R)) || (rc = b->alloc(ovf, R)) || (rc = b->alloc(b->fixedStage, size_t(F) * 4)) ||
+        (rc = b->alloc(b->clock, R)))
         return rc;
+    RP_HIP(hipMemset(b->clock, 0, size_t(R) * 4));
     RP_HIP(hipMemset(x, 0, size_t(P) * 3 * 4));
     RP_HIP(hipMemset(v, 0, size_t(P) * 3 * 4));
     RP_HIP(hipMemset(om, 0, size_t(F) * 3 * 4));
@@ -704,6 +716,9 @@ int rp_batch_create(const rp_batch_desc* desc, void** out_batch) {
     d.complianceScale = cs;
     d.frictionScale = fs;
     d.selfOverflow = ovf;
+    d.clock = b->clock;
+    d.paths = nullptr;
+    d.obs = nullptr;
     d.numRods = R;
     float *dInvMass, *dI4, *dII4, *dsPar, *dbPar, *dbRest;
     int *dsIdx, *dbIdx, *dSegEnd, *dCol;
@@ -883,6 +898,125 @@ int rp_batch_step(void* batch, const rp_step_params* params, int num_steps) {
     b.lastLaunches = b.strategy == RP_STRATEGY_PLAIN ? num_steps
                                                      : (num_steps + b.stepsPerLaunch - 1) / b.stepsPerLaunch;
     b.stepsTaken += num_steps;
+    return RP_OK;
+}
+
+int rp_batch_set_paths(void* batch, int first_rod, int num_rods, const rp_path_desc* paths) {
+    if (!batch) return fail(RP_ERR_INVALID, "null batch");
+    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
+    int rc = check_range(b, first_rod, num_rods);
+    if (rc) return rc;
+    if (!b.paths) {
+        if ((rc = b.alloc(b.paths, b.numRods))) return rc;
+        std::vector<PathRec> none(b.numRods);
+        for (PathRec& p : none) {
+            std::memset(&p, 0, sizeof(p));
+            p.particle = -1;
+        }
+        RP_HIP(hipMemcpy(b.paths, none.data(), none.size() * sizeof(PathRec), hipMemcpyHostToDevice));
+        b.drop_graphs();
+        b.d.paths = b.paths;
+    }
+    std::vector<PathRec> recs(num_rods);
+    for (int i = 0; i < num_rods; ++i) {
+        PathRec& p = recs[i];
+        std::memset(&p, 0, sizeof(p));
+        p.particle = -1;
+        if (!paths || paths[i].particle < 0) continue;
+        const rp_path_desc& s = paths[i];
+        const RodInfo& ri = b.rods[first_rod + i];
+        if (s.particle >= ri.nP)
+            return fail(RP_ERR_INVALID, "path " + std::to_string(i) + ": particle out of range");
+        if (s.num_waypoints < 1 || s.num_waypoints > kMaxWaypoints || !s.waypoints || !s.leg_seconds)
+            return fail(RP_ERR_INVALID, "path " + std::to_string(i) + ": 1.." + std::to_string(kMaxWaypoints) +
+                                            " waypoints with leg durations required");
+        if (!(s.control_period > 0.0) || s.steps_per_control < 1)
+            return fail(RP_ERR_INVALID, "path " + std::to_string(i) + ": control period and steps per control "
+                                            "must be positive");
+        p.particle = s.particle;
+        p.nWay = s.num_waypoints;
+        p.stepsPerControl = s.steps_per_control;
+        p.height = s.height;
+        p.period = s.control_period;
+        p.sx = s.start[0];
+        p.sy = s.start[1];
+        for (int k = 0; k < s.num_waypoints; ++k) {
+            if (!(s.leg_seconds[k] > 0.0))
+                return fail(RP_ERR_INVALID, "path " + std::to_string(i) + ": leg durations must be positive");
+            p.legs[k] = s.leg_seconds[k];
+            p.wx[k] = s.waypoints[2 * k];
+            p.wy[k] = s.waypoints[2 * k + 1];
+        }
+    }
+    RP_HIP(hipMemcpyAsync(b.paths + first_rod, recs.data(), recs.size() * sizeof(PathRec), hipMemcpyHostToDevice,
+                          b.stream));
+    RP_HIP(hipStreamSynchronize(b.stream));
+    return RP_OK;
+}
+
+int rp_batch_set_clock(void* batch, int first_rod, int num_rods, const int* clock) {
+    if (!batch) return fail(RP_ERR_INVALID, "null batch");
+    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
+    int rc = check_range(b, first_rod, num_rods);
+    if (rc) return rc;
+    std::vector<int> v(num_rods, 0);
+    if (clock)
+        for (int i = 0; i < num_rods; ++i) {
+            if (clock[i] < 0) return fail(RP_ERR_INVALID, "clock values must be non-negative");
+            v[i] = clock[i];
+        }
+    RP_HIP(hipMemcpyAsync(b.clock + first_rod, v.data(), v.size() * 4, hipMemcpyHostToDevice, b.stream));
+    RP_HIP(hipStreamSynchronize(b.stream));
+    return RP_OK;
+}
+
+int rp_batch_get_clock(void* batch, int first_rod, int num_rods, int* clock) {
+    if (!batch || !clock) return fail(RP_ERR_INVALID, "null argument");
+    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
+    int rc = check_range(b, first_rod, num_rods);
+    if (rc) return rc;
+    RP_HIP(hipMemcpyAsync(clock, b.clock + first_rod, size_t(num_rods) * 4, hipMemcpyDeviceToHost, b.stream));
+    RP_HIP(hipStreamSynchronize(b.stream));
+    return RP_OK;
+}
+
+int rp_batch_set_observers(void* batch, int first_rod, int num_rods, const rp_observer_desc* observers) {
+    if (!batch) return fail(RP_ERR_INVALID, "null batch");
+    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
+    int rc = check_range(b, first_rod, num_rods);
+    if (rc) return rc;
+    if (!b.obs) {
+        if ((rc = b.alloc(b.obs, b.numRods))) return rc;
+        std::vector<ObsRec> none(b.numRods, ObsRec{-1, 1, 0.0, 0.0});
+        RP_HIP(hipMemcpy(b.obs, none.data(), none.size() * sizeof(ObsRec), hipMemcpyHostToDevice));
+        b.drop_graphs();
+        b.d.obs = b.obs;
+    }
+    std::vector<ObsRec> recs(num_rods, ObsRec{-1, 1, 0.0, 0.0});
+    for (int i = 0; observers && i < num_rods; ++i) {
+        const rp_observer_desc& o = observers[i];
+        if (o.particle < 0) continue;
+        if (o.particle >= b.rods[first_rod + i].nP || o.every < 1)
+            return fail(RP_ERR_INVALID, "observer " + std::to_string(i) + ": bad particle or interval");
+        recs[i] = ObsRec{o.particle, o.every, o.z0, 0.0};
+    }
+    RP_HIP(hipMemcpyAsync(b.obs + first_rod, recs.data(), recs.size() * sizeof(ObsRec), hipMemcpyHostToDevice,
+                          b.stream));
+    RP_HIP(hipStreamSynchronize(b.stream));
+    return RP_OK;
+}
+
+int rp_batch_get_observations(void* batch, int first_rod, int num_rods, double* max_rise) {
+    if (!batch || !max_rise) return fail(RP_ERR_INVALID, "null argument");
+    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
+    int rc = check_range(b, first_rod, num_rods);
+    if (rc) return rc;
+    if (!b.obs) return fail(RP_ERR_STATE, "no observers installed");
+    std::vector<ObsRec> recs(num_rods);
+    RP_HIP(hipMemcpyAsync(recs.data(), b.obs + first_rod, recs.size() * sizeof(ObsRec), hipMemcpyDeviceToHost,
+                          b.stream));
+    RP_HIP(hipStreamSynchronize(b.stream));
+    for (int i = 0; i < num_rods; ++i) max_rise[i] = recs[i].maxRise;
     return RP_OK;
 }
 

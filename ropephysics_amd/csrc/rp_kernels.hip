@@ -129,9 +129,14 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
     const int* segEnd = d.segEnd + 2 * ri.sStart;
     const v3 g = mk(sa.gx, sa.gy, sa.gz);
     const float h = sa.h;
+    int clk = d.clock[r];
+    const int drivenP = d.paths ? d.paths[r].particle : -1;
+    const int obsP = d.obs ? d.obs[r].particle : -1;
+    double maxRise = obsP >= 0 ? d.obs[r].maxRise : 0.0;
     __syncthreads();
 
     for (int step = 0; step < sa.steps; ++step) {
+        if (drivenP >= 0 && drivenP % nt == tid) kv4[drivenP] = vf4(path_velocity(d.paths[r], clk), 0.0f);
         for (int sub = 0; sub < sa.substeps; ++sub) {
             const bool regen = sub % sa.contactInterval == 0;
             for (int k = tid; k < nS; k += nt) lamS[k] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -236,7 +241,14 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
             }
             __syncthreads();
         }
+        ++clk;
+        if (obsP >= 0 && obsP % nt == tid && clk % d.obs[r].every == 0) {
+            const double rise = double(x4[obsP].z) - d.obs[r].z0;
+            maxRise = maxRise < rise ? rise : maxRise;
+        }
     }
+    if (tid == 0) d.clock[r] = clk;
+    if (obsP >= 0 && obsP % nt == tid) d.obs[r].maxRise = maxRise;
 
     for (int i = tid; i < nP; i += nt) {
         st3(d.x, p0 + i, f4v(x4[i]));
@@ -422,9 +434,16 @@ __global__ __launch_bounds__(256) void k_tiled_step(DeviceArrays d, StepArgs sa,
     }
     const v3 g = mk(sa.gx, sa.gy, sa.gz);
     const float h = sa.h;
+    int clk = active ? d.clock[rod] : 0;
+    const int drivenP = (active && d.paths) ? d.paths[rod].particle : -1;
+    const bool driver = drivenP >= 0 && drivenP % G == l;
+    const int obsP = (active && d.obs) ? d.obs[rod].particle : -1;
+    const bool observer = obsP >= 0 && obsP % G == l;
+    double maxRise = observer ? d.obs[rod].maxRise : 0.0;
     __syncthreads();
 
     for (int step = 0; step < sa.steps; ++step) {
+        if (driver) kv4[drivenP] = vf4(path_velocity(d.paths[rod], clk), 0.0f);
         for (int sub = 0; sub < sa.substeps; ++sub) {
             const bool regen = sub % sa.contactInterval == 0;
 #pragma unroll
@@ -546,6 +565,11 @@ __global__ __launch_bounds__(256) void k_tiled_step(DeviceArrays d, StepArgs sa,
                 if (active && j < nF) om[t] = finish_frame(f4q(q4s[j]), qpv[t], iIF[t], h, sa.angDecay);
             }
         }
+        ++clk;
+        if (observer && clk % d.obs[rod].every == 0) {
+            const double rise = double(x4[obsP].z) - d.obs[rod].z0;
+            maxRise = maxRise < rise ? rise : maxRise;
+        }
     }
 
     __syncthreads();
@@ -563,6 +587,8 @@ __global__ __launch_bounds__(256) void k_tiled_step(DeviceArrays d, StepArgs sa,
         }
     }
     if (ri.self && l == 0) d.selfOverflow[rod] += sl.misc[1];
+    if (active && l == 0) d.clock[rod] = clk;
+    if (observer) d.obs[rod].maxRise = maxRise;
 }
 
 // Copy the orientation of zero-inertia frames from `src` (frames * 4, packed

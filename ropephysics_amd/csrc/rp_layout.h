@@ -62,6 +62,36 @@ struct Tile {
     int pad;
 };
 
+constexpr int kMaxWaypoints = 16;
+
+// A kinematically driven particle: the gripper path of one rod. Positions are
+// a smoothstep blend between consecutive waypoints over each leg's duration,
+// held at the last waypoint afterwards; the velocity applied during a step is
+// the average over its control period (reference HarnessTask::gripperAt and
+// gripperVelocity, src/apps/harness_routing.cpp:69-92). Evaluated in FP64.
+struct PathRec {
+    int particle;  // rod-local index, -1: no path
+    int nWay;
+    int stepsPerControl;
+    int pad;
+    double height;  // z of the gripper
+    double period;  // control period in seconds
+    double sx, sy;  // start
+    double legs[kMaxWaypoints];
+    double wx[kMaxWaypoints];
+    double wy[kMaxWaypoints];
+};
+
+// Largest rise of one particle's height above z0, sampled every `every` steps
+// of the rod's clock (reference HangOutcome, src/apps/cable_hanging.h:68-77,
+// observed as heldOnGpu does, src/validation/application_cases.cpp:55-65).
+struct ObsRec {
+    int particle;  // rod-local index, -1: no observer
+    int every;
+    double z0;
+    double maxRise;
+};
+
 struct DeviceArrays {
     // state
     float* x;
@@ -88,6 +118,9 @@ struct DeviceArrays {
     float* complianceScale;
     float* frictionScale;
     int* selfOverflow;
+    int* clock;            // steps taken, per rod
+    const PathRec* paths;  // per rod, or null
+    ObsRec* obs;           // per rod, or null
     int numRods;
 };
 

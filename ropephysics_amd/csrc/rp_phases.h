@@ -128,6 +128,38 @@ __device__ __forceinline__ void world_contacts_project(float* slots, int nPrim, 
     }
 }
 
+// ------------------------------------------------------------------ drivers and observers
+
+__device__ inline void path_point(const PathRec& p, double t, double& x, double& y) {
+    double fx = p.sx, fy = p.sy, t0 = 0.0;
+    for (int k = 0; k < p.nWay; ++k) {
+        const double t1 = t0 + p.legs[k];
+        if (t < t1) {
+            double u = (t - t0) / p.legs[k];
+            u = u * u * (3.0 - 2.0 * u);
+            x = fx + (p.wx[k] - fx) * u;
+            y = fy + (p.wy[k] - fy) * u;
+            return;
+        }
+        fx = p.wx[k];
+        fy = p.wy[k];
+        t0 = t1;
+    }
+    x = fx;
+    y = fy;
+}
+
+// Velocity of the driven particle during the rod's step `step`: constant over
+// each control tick, (P(t_{k+1}) - P(t_k)) / period, rounded to FP32.
+__device__ inline v3 path_velocity(const PathRec& p, int step) {
+    const int tick = step / p.stepsPerControl;
+    double x1, y1, x0, y0;
+    path_point(p, double(tick + 1) * p.period, x1, y1);
+    path_point(p, double(tick) * p.period, x0, y0);
+    const double inv = 1.0 / p.period;
+    return mk(float((x1 - x0) * inv), float((y1 - y0) * inv), float((p.height - p.height) * inv));
+}
+
 // ------------------------------------------------------------------ self-collision
 //
 // One rod's working set in LDS. Segments are visited in the rod's original

@@ -76,3 +76,67 @@ def uniform_floor_batch(num_rods: int = 4096, segments: int = 64, seed: int = 20
                     params, steps, num_rods * segments,
                     {"rods": num_rods, "segments": segments, "length_m": 0.5, "height_m": height, "seed": seed,
                      "dt": 1e-3, "substeps": 10, "steps": steps, "contacts": "floor plane, Coulomb friction"})
+
+
+def routing_rollouts(num_rods: int = 1024, seed: int = 20260917, steps: int | None = None) -> Workload:
+    """Ragged batch of routing rollouts: cables of 48-160 segments (uniform
+    integer), 1.2 m, connector pinned, the other end driven along a seeded
+    8-waypoint smoothstep (piecewise-cubic) path; floor, two capsule pegs
+    (r = 15 mm), one box clip; friction 0.5 x U[0.6, 1.4]; stiffness
+    0.4-4x log-uniform; self-collision on; dt = 1e-3, 10 substeps, 6 s."""
+    from .apps import RoutingTask
+
+    task = RoutingTask()
+    rng = np.random.default_rng(seed)
+    n = rng.integers(48, 161, num_rods)
+    youngs = 0.4 * 10.0 ** rng.uniform(0.0, 1.0, num_rods)
+    fric = rng.uniform(0.6, 1.4, num_rods)
+    wps = task.base_waypoints()[None, :, :] + rng.normal(0.0, 0.015, (num_rods, 8, 2))
+    cache = {}
+    rods = []
+    for k in n:
+        k = int(k)
+        if k not in cache:
+            cache[k] = task.build(k)
+        rods.append(cache[k])
+    paths = [task.path(int(n[r]), wps[r]) for r in range(num_rods)]
+    st = task.steps() if steps is None else steps
+    w = Workload("routing-1024-ragged" if num_rods == 1024 else f"routing-{num_rods}-ragged", None, rods, num_rods,
+                 None, task.world(), None, youngs, fric, task.params(), st, int(n.sum()),
+                 {"rods": num_rods, "segments": "48-160 uniform", "segments_total": int(n.sum()), "length_m": 1.2,
+                  "seed": seed, "dt": 1e-3, "substeps": 10, "steps": st, "self_collision": True,
+                  "contacts": "floor, 2 capsule pegs r=15 mm, box clip"})
+    w.info["paths"] = paths
+    w.info["task"] = task
+    return w
+
+
+def drape_sweep(seed: int = 20260917, steps: int | None = None, ratios: int = 60, frictions: int = 30) -> Workload:
+    """Hold/slide sweep: 3600 cables (60 leg ratios 1:1-4:1 x 30 friction
+    values 0.05-0.5 x stiffness E and 3E, E = 5e6 Pa), 120 segments, 1 m,
+    r = 3 mm, draped over a horizontal capsule bar (r = 10 mm); dt = 5e-4,
+    10 substeps, 8 s after release."""
+    from .apps import HangingCable
+
+    task = HangingCable()
+    ratio_vals = np.linspace(1.0, 4.0, ratios)
+    mu_vals = np.linspace(0.05, 0.5, frictions)
+    protos = [task.build(float(r)) for r in ratio_vals]
+    rods, ys, fs, meta = [], [], [], []
+    for stiff in (1.0, 3.0):
+        for i, r in enumerate(ratio_vals):
+            for mu in mu_vals:
+                rods.append(protos[i])
+                ys.append(stiff)
+                fs.append(mu)
+                meta.append((float(r), float(mu), stiff))
+    world = task.world(1.0)
+    st = task.steps() if steps is None else steps
+    w = Workload("drape-3600" if ratios * frictions == 1800 else f"drape-{len(rods)}", None, rods, len(rods), None,
+                 world, None, np.array(ys), np.array(fs), task.params(), st, len(rods) * task.segments,
+                 {"rods": len(rods), "segments": task.segments, "length_m": task.length, "dt": task.dt,
+                  "substeps": task.substeps, "steps": st, "bar_radius_m": task.bar_radius,
+                  "contacts": "capsule bar and floor"})
+    w.info["task"] = task
+    w.info["meta"] = meta
+    return w

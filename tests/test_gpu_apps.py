@@ -1,0 +1,133 @@
+"""GPU tests of the device-side drivers and observers, and of the two
+application tasks' verdicts (routing; hold/slide)."""
+import numpy as np
+import pytest
+
+import oracle
+import ropephysics_amd as rp
+from ropephysics_amd.apps import HangingCable, RoutingTask
+from helpers import bits, compare_bitwise, compliance_scale_f32, oracle_params, oracle_state, oracle_twin, oracle_world
+
+pytestmark = pytest.mark.gpu
+
+
+def _routing_small(num, seed, segs=(48, 61, 80)):
+    task = RoutingTask()
+    rng = np.random.default_rng(seed)
+    rods, paths = [], []
+    for k in range(num):
+        n = segs[k % len(segs)]
+        rods.append(task.build(n))
+        paths.append(task.path(n, task.base_waypoints() + rng.normal(0, 0.015, (8, 2))))
+    return task, rods, paths
+
+
+@pytest.mark.parametrize("strategy", ["plain", "tiled"])
+def test_gripper_path_driver_bitwise(strategy):
+    """The device evaluates each rod's gripper schedule in FP64 every step;
+    the oracle is driven with the same schedule evaluated on the host
+    (attemptOnCpu, application_cases.cpp:275-289). Bitwise equal throughout,
+    with the routing world (floor, pegs, box clip) and self-collision on."""
+    task, rods, paths = _routing_small(3, 7)
+    world = task.world()
+    params = task.params()
+    ys = np.array([0.5, 1.0, 3.0])
+    fs = np.array([0.7, 1.0, 1.3])
+    b = rp.Batch(rods, strategy=strategy, world=world)
+    b.set_material_scales(ys, fs)
+    b.set_paths(0, paths)
+    twins = [(oracle_twin(r, 32), oracle_params(params, compliance_scale=compliance_scale_f32(ys[i]),
+                                                friction_scale=float(np.float32(fs[i]))))
+             for i, r in enumerate(rods)]
+    ow = oracle_world(world)
+    steps = 400
+    for s in range(steps):
+        for i, (o, op) in enumerate(twins):
+            v = np.float32(np.array(paths[i].velocity(s)))
+            a = o.get()
+            a["kinVel"][paths[i].particle] = v
+            o.set({"kinVel": a["kinVel"]})
+            o.step(op, ow, 1)
+    b.step(params, steps)
+    b.synchronize()
+    for i, (o, _) in enumerate(twins):
+        d = compare_bitwise(b.download(i), oracle_state(o))
+        assert all(v == 0 for v in d.values()), (i, d)
+    assert list(b.get_clock()) == [steps] * 3
+    # The gripper really moved the cable end.
+    x = b.download(0)["x"]
+    assert np.linalg.norm(x[-1, :2] - np.array(paths[0].start)) > 0.05
+
+
+def test_clock_and_paths_resume_across_calls():
+    task, rods, paths = _routing_small(2, 3)
+    params = task.params()
+    a = rp.Batch(rods, strategy="tiled", world=task.world())
+    a.set_paths(0, paths)
+    a.step(params, 250)
+    b = rp.Batch(rods, strategy="tiled", world=task.world())
+    b.set_paths(0, paths)
+    for chunk in (1, 63, 64, 100, 22):
+        b.step(params, chunk)
+    sa, sb = a.get_state(), b.get_state()
+    for k in sa:
+        assert np.array_equal(bits(sa[k]), bits(sb[k]))
+    assert list(b.get_clock()) == [250, 250]
+
+
+@pytest.mark.parametrize("strategy", ["plain", "tiled"])
+def test_hang_observer_matches_host_sampling(strategy):
+    """The device observer's max rise equals the host-side HangOutcome fed the
+    FP32 oracle's short-end heights every 50 steps (bitwise, FP64)."""
+    task = HangingCable(segments=60, duration=0.3, dt=1e-3, substeps=8, iterations=2)
+    rods = [task.build(r) for r in (1.2, 3.5)]
+    mus = [0.5, 0.05]
+    world = task.world(1.0)
+    b = rp.Batch(rods, strategy=strategy, world=world)
+    b.set_material_scales(None, mus)
+    z0 = [float(r.x[0, 2]) for r in rods]
+    b.set_observers(0, [(0, 50, z0[i]) for i in range(2)])
+    steps = task.steps()
+    b.step(task.params(), steps)
+    got = b.get_observations()
+    ow = oracle_world(world)
+    for i, r in enumerate(rods):
+        o = oracle_twin(r, 32)
+        op = oracle_params(task.params(), friction_scale=float(np.float32(mus[i])))
+        rise = 0.0
+        for s in range(1, steps + 1):
+            o.step(op, ow, 1)
+            if s % 50 == 0:
+                rise = max(rise, float(o.get()["x"][0, 2]) - z0[i])
+        assert got[i] == rise
+    assert got[1] > got[0]  # the slippery, lopsided drape rises more
+
+
+def test_routing_verdicts_agree_with_fp64_oracle():
+    """Full 6 s routing rollouts for a few cables: the routed / not-routed
+    verdict from GPU positions equals the verdict from the FP64 oracle's."""
+    task, rods, paths = _routing_small(4, 11, segs=(48, 64))
+    # One deliberately bad path (straight to the end point): not routed.
+    paths[3] = task.path(64, np.tile(task.base_waypoints()[-1], (8, 1)))
+    world = task.world()
+    params = task.params()
+    b = rp.Batch(rods, strategy="tiled", world=world)
+    b.set_paths(0, paths)
+    steps = task.steps()
+    b.step(params, steps)
+    b.synchronize()
+    ow = oracle_world(world)
+    verdicts = []
+    for i, r in enumerate(rods):
+        o = oracle_twin(r, 64)
+        op = oracle_params(params, precision=64)
+        for s in range(steps):
+            a = o.get()
+            a["kinVel"][paths[i].particle] = paths[i].velocity(s)
+            o.set({"kinVel": a["kinVel"]})
+            o.step(op, ow, 1)
+        g = task.evaluate(b.download(i)["x"].astype(np.float64))
+        c = task.evaluate(o.get()["x"])
+        assert g["routed"] == c["routed"], (i, g, c)
+        verdicts.append(g["routed"])
+    assert not verdicts[3]
