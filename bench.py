@@ -107,7 +107,8 @@ def ragged_comparison(steps: int, plain_steps: int, warmup: int) -> dict:
                "kernel_ms_avg": c["last_step_gpu_ms"] / max(1, c["last_step_launches"]),
                "self_contact_overflow": c["self_contact_overflow"]}
         if strategy == "tiled":
-            ent.update({"tile_config": c.get("tile_config_name"), "workgroups": c["workgroups"],
+            ent.update({"tile_config": c.get("tile_config_name"), "autotune_trials_ms": b.tune_report(),
+                        "workgroups": c["workgroups"],
                         "lanes_busy": c["lanes_busy"], "lanes_total": c["lanes_total"],
                         "lds_bytes_per_workgroup": c["lds_bytes_per_workgroup"]})
             x = b.download_positions()
@@ -218,7 +219,7 @@ def main():
     tiled_info = {"tile_config": c.get("tile_config_name"), "workgroups": c["workgroups"],
                   "lanes_busy": c["lanes_busy"], "lanes_total": c["lanes_total"],
                   "lds_bytes_per_workgroup": c["lds_bytes_per_workgroup"], "launches": launches,
-                  "kernel_ms_avg": kernel_ms}
+                  "kernel_ms_avg": kernel_ms, "autotune_trials_ms": b.tune_report()}
     b.destroy()
 
     # ---- PLAIN, same workload

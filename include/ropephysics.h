@@ -240,6 +240,11 @@ int rp_batch_get_observations(void* batch, int first_rod, int num_rods, double* 
  * (enable = 1) so rp_counters reports its device time. Off by default. */
 int rp_batch_set_timing(void* batch, int enable);
 
+/* The autotuner's trials: for each tile configuration it timed, its id and
+ * the device time of an 8-step trial launch (best of two), in order. Returns
+ * the number of trials (which may exceed capacity), or a negative status. */
+int rp_batch_tune_report(void* batch, int* ids, float* ms, int capacity);
+
 /* Name of a tile configuration id (for logs and the autotuner's report);
  * NULL for an unknown id. */
 const char* rp_tile_config_name(int id);

@@ -83,6 +83,7 @@ SIGNATURES = {
     "rp_batch_get_state": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _fp, _fp, _fp, _fp]),
     "rp_batch_counters": (C.c_int, [C.c_void_p, C.POINTER(RpCounters)]),
     "rp_batch_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "rp_batch_tune_report": (C.c_int, [C.c_void_p, _ip, _fp, C.c_int]),
     "rp_batch_set_paths": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(RpPathDesc)]),
     "rp_batch_set_clock": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip]),
     "rp_batch_get_clock": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip]),

@@ -142,6 +142,8 @@ class Batch:
     # ------------------------------------------------------------------ state and loads
 
     def _range_sizes(self, first, count):
+        if first < 0 or count < 0 or first + count > self.num_rods:
+            raise ValueError(f"rod range [{first}, {first + count}) outside a batch of {self.num_rods}")
         p0, p1 = self.particle_start[first], self.particle_start[first + count]
         f0, f1 = self.frame_start[first], self.frame_start[first + count]
         return int(p1 - p0), int(f1 - f0)
@@ -258,6 +260,15 @@ class Batch:
         _abi.check(_abi.lib().rp_batch_get_observations(self._h, first, count,
                                                         out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
+
+    def tune_report(self) -> list:
+        """(tile configuration name, trial ms) for each shape the autotuner timed."""
+        ids = np.zeros(64, np.int32)
+        ms = np.zeros(64, np.float32)
+        n = _abi.lib().rp_batch_tune_report(self._h, _abi.iptr(ids), _abi.fptr(ms), 64)
+        if n < 0:
+            _abi.check(n)
+        return [(_abi.lib().rp_tile_config_name(int(i)).decode(), float(m)) for i, m in zip(ids[:n], ms[:n])]
 
     def set_timing(self, enable: bool = True):
         """Time every step call with HIP events on the batch's stream."""

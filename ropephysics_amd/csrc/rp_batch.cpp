@@ -124,6 +124,7 @@ struct rp_batch_impl {
     long long stepsTaken = 0;
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::vector<int> colStart;  // host copy of the colour-start table
+    std::vector<std::pair<int, float>> tuneLog;  // (config, ms for the trial) of the last autotune
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool evPending = false;
@@ -346,25 +347,33 @@ int launch_steps(rp_batch_impl& b, const rp_step_params& p, int steps, int strat
     return RP_OK;
 }
 
-size_t state_floats(const rp_batch_impl& b) {
-    const size_t P = size_t(b.pStart[b.numRods]), F = size_t(b.fStart[b.numRods]);
-    return P * 6 + F * 7;
+// Device state a trial run changes: positions, velocities, orientations,
+// angular velocities, clocks, overflow counters and observers.
+struct StateBackup {
+    std::vector<std::pair<void*, size_t>> fields;
+    void* store = nullptr;
+    size_t bytes = 0;
+};
+
+size_t backup_bytes(const rp_batch_impl& b) {
+    const size_t P = size_t(b.pStart[b.numRods]), F = size_t(b.fStart[b.numRods]), R = size_t(b.numRods);
+    return (P * 6 + F * 7) * sizeof(float) + 2 * R * sizeof(int) + (b.obs ? R * sizeof(ObsRec) : 0);
 }
 
 int backup_state(rp_batch_impl& b, bool restore) {
-    const size_t P = size_t(b.pStart[b.numRods]), F = size_t(b.fStart[b.numRods]);
-    float* s = b.stateBackup;
-    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
-    struct F_ {
-        float* dev;
-        size_t n;
-    } fields[] = {{b.d.x, P * 3}, {b.d.v, P * 3}, {b.d.q, F * 4}, {b.d.omega, F * 3}};
-    for (const F_& f : fields) {
+    const size_t P = size_t(b.pStart[b.numRods]), F = size_t(b.fStart[b.numRods]), R = size_t(b.numRods);
+    char* s = reinterpret_cast<char*>(b.stateBackup);
+    std::pair<void*, size_t> fields[] = {{b.d.x, P * 12},          {b.d.v, P * 12},
+                                         {b.d.q, F * 16},          {b.d.omega, F * 12},
+                                         {b.clock, R * 4},         {b.d.selfOverflow, R * 4},
+                                         {b.obs, b.obs ? R * sizeof(ObsRec) : 0}};
+    for (const auto& f : fields) {
+        if (!f.first || !f.second) continue;
         if (restore)
-            RP_HIP(hipMemcpyAsync(f.dev, s, f.n * sizeof(float), k, b.stream));
+            RP_HIP(hipMemcpyAsync(f.first, s, f.second, hipMemcpyDeviceToDevice, b.stream));
         else
-            RP_HIP(hipMemcpyAsync(s, f.dev, f.n * sizeof(float), k, b.stream));
-        s += f.n;
+            RP_HIP(hipMemcpyAsync(s, f.first, f.second, hipMemcpyDeviceToDevice, b.stream));
+        s += f.second;
     }
     RP_HIP(hipStreamSynchronize(b.stream));
     return RP_OK;
@@ -376,7 +385,7 @@ int backup_state(rp_batch_impl& b, bool restore) {
 // same whatever lanes and workgroup it lands in.
 int autotune(rp_batch_impl& b, const rp_step_params& p) {
     if (!b.stateBackup) {
-        int rc = b.alloc(b.stateBackup, state_floats(b));
+        int rc = b.alloc(b.stateBackup, (backup_bytes(b) + sizeof(float) * 2 + 0) / sizeof(float));
         if (rc) return rc;
     }
     int rc = backup_state(b, false);
@@ -386,20 +395,25 @@ int autotune(rp_batch_impl& b, const rp_step_params& p) {
     hipEvent_t e0, e1;
     RP_HIP(hipEventCreate(&e0));
     RP_HIP(hipEventCreate(&e1));
-    const int trialSteps = 2;
+    const int trialSteps = 8;
     for (int id = 1; id <= kNumConfigs; ++id) {
         auto t = std::make_unique<PackedTiles>();
         if (pack_tiles(b, id, b.colStart, *t) != RP_OK) continue;
         if (size_t(t->maxLds) > b.ldsLimit) continue;
         if (upload_tiles(b, *t) != RP_OK) continue;
-        // Warm-up launch (code object load), then a timed one.
+        // A warm-up launch (code object load), then the best of two timed ones.
         if (launch_steps(b, p, 1, RP_STRATEGY_TILED, t.get()) != RP_OK) continue;
-        RP_HIP(hipEventRecord(e0, b.stream));
-        if (launch_steps(b, p, trialSteps, RP_STRATEGY_TILED, t.get()) != RP_OK) continue;
-        RP_HIP(hipEventRecord(e1, b.stream));
-        RP_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        RP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        float ms = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            RP_HIP(hipEventRecord(e0, b.stream));
+            if (launch_steps(b, p, trialSteps, RP_STRATEGY_TILED, t.get()) != RP_OK) break;
+            RP_HIP(hipEventRecord(e1, b.stream));
+            RP_HIP(hipEventSynchronize(e1));
+            float m = 0;
+            RP_HIP(hipEventElapsedTime(&m, e0, e1));
+            ms = std::min(ms, m);
+        }
+        b.tuneLog.push_back(std::make_pair(id, ms));
         if (ms < best) {
             best = ms;
             bestTiles = std::move(t);
@@ -1018,6 +1032,17 @@ int rp_batch_get_observations(void* batch, int first_rod, int num_rods, double* 
     RP_HIP(hipStreamSynchronize(b.stream));
     for (int i = 0; i < num_rods; ++i) max_rise[i] = recs[i].maxRise;
     return RP_OK;
+}
+
+int rp_batch_tune_report(void* batch, int* ids, float* ms, int capacity) {
+    if (!batch) return fail(RP_ERR_INVALID, "null batch");
+    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
+    const int n = int(b.tuneLog.size());
+    for (int i = 0; i < n && i < capacity; ++i) {
+        if (ids) ids[i] = b.tuneLog[i].first;
+        if (ms) ms[i] = b.tuneLog[i].second;
+    }
+    return n;
 }
 
 int rp_batch_set_timing(void* batch, int enable) {

@@ -464,19 +464,8 @@ __global__ __launch_bounds__(256) void k_tiled_step(DeviceArrays d, StepArgs sa,
             }
             if (anySelf) {
                 if (regen) {
-                    if (ri.self && l == 0) self_build_hash(sl, x4, segEnd, nS, ri.table, ri.radius);
-                    __syncthreads();
-                    if (ri.self)
-                        for (int j = l; j < nS; j += G)
-                            self_query(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
-                                       ri.selfFriction * fscale, j, false);
-                    __syncthreads();
-                    if (ri.self && l == 0) self_allocate(sl, nS, ri.selfCap);
-                    __syncthreads();
-                    if (ri.self)
-                        for (int j = l; j < nS; j += G)
-                            self_query(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
-                                       ri.selfFriction * fscale, j, true);
+                    self_regen_group(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
+                                     ri.selfFriction * fscale, ri.self != 0, l, G);
                 } else if (ri.self && l == 0) {
                     self_reset(sl);
                 }

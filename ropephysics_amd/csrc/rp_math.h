@@ -257,7 +257,9 @@ RP_HD void seg_seg(v3 p1, v3 q1, v3 p2, v3 q2, float& s, float& t) {
 RP_HD unsigned cell_hash(int ix, int iy, int iz, int table) {
     const unsigned h = (static_cast<unsigned>(ix) * 73856093u) ^ (static_cast<unsigned>(iy) * 19349663u) ^
                        (static_cast<unsigned>(iz) * 83492791u);
-    return h % static_cast<unsigned>(table);
+    // A power-of-two table reduces to a mask; the result is the same.
+    const unsigned t = static_cast<unsigned>(table);
+    return (t & (t - 1u)) == 0u ? (h & (t - 1u)) : h % t;
 }
 RP_HD int cell_coord(float v, float cell) { return static_cast<int>(floorf(v / cell)); }
 

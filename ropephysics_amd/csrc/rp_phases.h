@@ -172,11 +172,14 @@ struct SelfLds {
     int* segStart;   // nS
     int* cellStart;  // table + 1
     float* pool;     // cap * kSelfFloats
-    int* misc;       // [0] live, [1] overflow, [2] cell size (float bits)
+    int* misc;       // [0] live, [1] overflow, [2] cell size, [3] longest segment (float bits)
+    int* scan;       // kScanLanes, scratch of the group-parallel scans
 };
 
+constexpr int kScanLanes = 256;
+
 __host__ __device__ inline int self_lds_bytes(int nS, int table, int cap) {
-    return nS * 16 + 4 * nS * 4 + (table + 1) * 4 + cap * kSelfFloats * 4 + 16;
+    return nS * 16 + 4 * nS * 4 + (table + 1) * 4 + cap * kSelfFloats * 4 + 16 + kScanLanes * 4;
 }
 
 __device__ inline SelfLds self_carve(char* base, int nS, int table, int cap) {
@@ -189,6 +192,7 @@ __device__ inline SelfLds self_carve(char* base, int nS, int table, int cap) {
     s.segCount = s.sorted + nS;
     s.segStart = s.segCount + nS;
     s.cellStart = s.segStart + nS;
+    s.scan = s.cellStart + table + 1;
     return s;
 }
 
@@ -270,6 +274,94 @@ __device__ inline void self_query(const SelfLds& s, const float4* x4, const int*
                 }
     }
     if (!fill) s.segCount[j] = count;
+}
+
+// ------------------------------------------------------------------ group-parallel broadphase
+//
+// The same hash, candidate order and pool layout as self_build_hash /
+// self_allocate, built by the G lanes of a rod's lane group: the longest
+// segment by an atomic max on the (non-negative) float bits, bucket counts by
+// atomic adds, prefix sums by a chunked scan, and each bucket's items placed
+// in ascending segment order by counting the earlier segments in the same
+// bucket. Every step is exact integer work or an order-free max, so the
+// tables equal the serial build's. Every lane of the workgroup calls these
+// (they contain barriers); `on` is false for lanes whose rod has no
+// self-collision.
+
+// Exclusive prefix sum of in[0, n) into out[0, n) by lanes l < G; returns the
+// total. Fixed barrier count whatever G, n or `on`.
+__device__ inline int group_scan(const int* in, int* out, int n, int* scratch, int l, int G, bool on) {
+    const int chunk = (n + G - 1) / max(G, 1);
+    const int lo = on ? min(n, l * chunk) : 0;
+    const int hi = on ? min(n, lo + chunk) : 0;
+    int sum = 0;
+    for (int i = lo; i < hi; ++i) sum += in[i];
+    if (on) scratch[l] = sum;
+    __syncthreads();
+    for (int off = 1; off < kScanLanes; off <<= 1) {
+        const int v = (on && l >= off) ? scratch[l - off] : 0;
+        __syncthreads();
+        if (on) scratch[l] += v;
+        __syncthreads();
+    }
+    const int total = on ? scratch[G - 1] : 0;
+    int run = on ? scratch[l] - sum : 0;
+    for (int i = lo; i < hi; ++i) {
+        const int v = in[i];
+        out[i] = run;
+        run += v;
+    }
+    __syncthreads();
+    return total;
+}
+
+__device__ inline void self_regen_group(const SelfLds& s, const float4* x4, const int* segEnd, int nS, int table,
+                                        int gap, int cap, float radius, float friction, bool on, int l, int G) {
+    if (on) {
+        for (int t = l; t <= table; t += G) s.cellStart[t] = 0;
+        if (l == 0) s.misc[3] = 0;
+    }
+    __syncthreads();
+    if (on)
+        for (int j = l; j < nS; j += G) {
+            const v3 a = f4v(x4[segEnd[2 * j]]), b = f4v(x4[segEnd[2 * j + 1]]);
+            s.cen[j] = vf4(scale(add(a, b), 0.5f), 0.0f);
+            atomicMax(&s.misc[3], __float_as_int(norm(sub(b, a))));
+        }
+    __syncthreads();
+    if (on) {
+        const float cell = __int_as_float(s.misc[3]) + 2.0f * radius;
+        if (l == 0) s.misc[2] = __float_as_int(cell);
+        for (int j = l; j < nS; j += G) {
+            const v3 c = f4v(s.cen[j]);
+            const int h = int(cell_hash(cell_coord(c.x, cell), cell_coord(c.y, cell), cell_coord(c.z, cell), table));
+            s.cellOf[j] = h;
+            atomicAdd(&s.cellStart[h], 1);
+        }
+    }
+    __syncthreads();
+    const int items = group_scan(s.cellStart, s.cellStart, on ? table : 0, s.scan, l, G, on);
+    if (on) {
+        if (l == 0) s.cellStart[table] = items;
+        for (int j = l; j < nS; j += G) {
+            const int h = s.cellOf[j];
+            int rank = 0;
+            for (int i = 0; i < j; ++i) rank += s.cellOf[i] == h ? 1 : 0;
+            s.sorted[s.cellStart[h] + rank] = j;
+        }
+    }
+    __syncthreads();
+    if (on)
+        for (int j = l; j < nS; j += G) self_query(s, x4, segEnd, nS, table, gap, cap, radius, friction, j, false);
+    __syncthreads();
+    const int total = group_scan(s.segCount, s.segStart, on ? nS : 0, s.scan, l, G, on);
+    if (on && l == 0) {
+        s.misc[0] = total < cap ? total : cap;
+        if (total > cap) s.misc[1] += total - cap;
+    }
+    __syncthreads();
+    if (on)
+        for (int j = l; j < nS; j += G) self_query(s, x4, segEnd, nS, table, gap, cap, radius, friction, j, true);
 }
 
 // Counts -> pool offsets; contacts past the capacity are tallied as overflow.

@@ -167,8 +167,10 @@ def helix_rope():
     return rp.make_rod_from_centerline(pts, rp.RodMaterial(2e6, 0.35, 1000.0, 5e-3), (0, 0, 1))
 
 
+# The coil holds more contacts than the reference's default pool
+# (max(16, segments / 2)); a larger pool keeps every contact, as the CPU does.
 SELF_WORLD = rp.CollisionWorld(primitives=[rp.Primitive.plane((0, 0, 0), (0, 0, 1), 0.6)], self_collision=True,
-                               self_friction=0.3, hash_table_size=256)
+                               self_friction=0.3, hash_table_size=256, self_capacity=512)
 SELF_PARAMS = rp.SolverParams(dt=1e-3, substeps=16, iterations=4, gravity=(0, 0, -9.81), linear_damping=1.0,
                               angular_damping=1.0)
 
@@ -280,8 +282,12 @@ def test_invalid_inputs_are_rejected():
     b = rp.Batch(rod, 2)
     with pytest.raises(ValueError):
         b.set_state(0, 1, x=np.zeros((3, 3)))
-    with pytest.raises(rp.RopePhysicsError):
+    with pytest.raises(ValueError):
         b.get_state(1, 5)
+    from ropephysics_amd import _abi
+    out = np.zeros(1000, np.float32)
+    assert _abi.lib().rp_batch_get_state(b._h, 1, 5, _abi.fptr(out), None, None, None) == _abi.RP_ERR_INVALID
+    assert b"outside" in _abi.lib().rp_last_error()
     with pytest.raises(rp.RopePhysicsError):
         b.set_material_scales([1.0, -1.0], None)
     with pytest.raises(ValueError):
