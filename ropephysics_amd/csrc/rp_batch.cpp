@@ -29,7 +29,7 @@ namespace rp {
 int plain_lds_bytes(int nP, int nF, int nS, int nB, int nPrim, int self, int table, int cap);
 int tiled_lds_bytes(int nP, int nF, int nS, int nPrim, int self, int table, int cap);
 hipError_t launch_plain(const DeviceArrays& d, const StepArgs& sa, size_t lds, hipStream_t s);
-hipError_t launch_tiled(int tp, int tc, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
+hipError_t launch_tiled(int tp, int tc, bool self, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
                         const TileSlot* slots, int numTiles, int threads, size_t lds, hipStream_t s);
 hipError_t launch_fixed_frames(float* q, const float* src, const float* invInertia4, int f0, int count,
                                hipStream_t s);
@@ -126,6 +126,7 @@ struct rp_batch_impl {
     std::vector<int> colStart;  // host copy of the colour-start table
     std::vector<std::pair<int, float>> tuneLog;  // (config, ms for the trial) of the last autotune
     bool timing = false;
+    bool anySelf = false;  // some rod collides with itself: run the SELF kernel variant
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool evPending = false;
     int lastLaunches = 0;
@@ -339,7 +340,7 @@ int launch_steps(rp_batch_impl& b, const rp_step_params& p, int steps, int strat
     while (left > 0) {
         const int n = std::min(left, b.stepsPerLaunch);
         const StepArgs a = step_args(p, n);
-        RP_HIP(launch_tiled(cfg.tp, cfg.tc, b.d, a, static_cast<const Tile*>(t->dTiles.p),
+        RP_HIP(launch_tiled(cfg.tp, cfg.tc, b.anySelf, b.d, a, static_cast<const Tile*>(t->dTiles.p),
                             static_cast<const TileSlot*>(t->dSlots.p), int(t->tiles.size()), 64 * cfg.waves,
                             size_t(t->maxLds), b.stream));
         left -= n;
@@ -670,6 +671,7 @@ int rp_batch_create(const rp_batch_desc* desc, void** out_batch) {
     RP_HIP(hipGetDeviceProperties(&prop, 0));
     b->ldsLimit = std::min<size_t>(prop.sharedMemPerBlock, 65536);
     b->plainLds = maxPlainLds;
+    for (int r = 0; r < R; ++r) b->anySelf = b->anySelf || b->rods[r].self != 0;
     bool allTileable = true;
     for (int r = 0; r < R; ++r) {
         if (b->tileable[r] && size_t(rod_tiled_lds(b->rods[r])) > b->ldsLimit) b->tileable[r] = 0;

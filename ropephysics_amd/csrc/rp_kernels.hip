@@ -296,7 +296,9 @@ __host__ __device__ inline TiledLayout tiled_layout(int nP, int nF, int nS, int 
 // TP: trips over particles and frames; TC: trips per colour per family.
 // A lane group of G lanes covers up to TP*G particles and frames and TC*G
 // constraints of each colour; the host sizes G per rod to satisfy that.
-template <int TP, int TC>
+// SELF: compiled with rod self-collision; batches without it run the variant
+// without, whose smaller register footprint keeps two waves per SIMD.
+template <int TP, int TC, bool SELF>
 __global__ __launch_bounds__(256) void k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles,
                                                     const TileSlot* slots) {
     extern __shared__ float4 smem4[];
@@ -343,10 +345,10 @@ __global__ __launch_bounds__(256) void k_tiled_step(DeviceArrays d, StepArgs sa,
     float* prims = reinterpret_cast<float*>(base + L.prims);
     float4* xp4 = reinterpret_cast<float4*>(base + L.xp);
     SelfLds sl;
-    if (ri.self) sl = self_carve(base + L.self, nS, ri.table, ri.selfCap);
+    if (SELF && ri.self) sl = self_carve(base + L.self, nS, ri.table, ri.selfCap);
     // Any lane group in the workgroup may run self-collision; the barriers
     // around it must be reached by every lane of the workgroup.
-    const bool anySelf = __syncthreads_or(ri.self) != 0;
+    const bool anySelf = SELF && __syncthreads_or(ri.self) != 0;
     const bool anyPrim = __syncthreads_or(nPrim > 0) != 0;
 
     const int p0 = ri.pStart, f0 = ri.fStart;
@@ -428,7 +430,7 @@ __global__ __launch_bounds__(256) void k_tiled_step(DeviceArrays d, StepArgs sa,
     const float cscale = active ? d.complianceScale[rod] : 1.0f;
     const float fscale = active ? d.frictionScale[rod] : 1.0f;
     const int* segEnd = d.segEnd + 2 * ri.sStart;
-    if (ri.self && l == 0) {
+    if (SELF && ri.self && l == 0) {
         sl.misc[0] = 0;
         sl.misc[1] = 0;
     }
@@ -479,7 +481,7 @@ __global__ __launch_bounds__(256) void k_tiled_step(DeviceArrays d, StepArgs sa,
                     v3 x = f4v(x4[i]);
                     predict_particle(x, vel[t], xpv[t], wP[t], f4v(f4[i]), f4v(kv4[i]), h, g);
                     x4[i] = vf4(x, wP[t]);
-                    if (ri.self) xp4[i] = vf4(xpv[t], wP[t]);
+                    if (SELF && ri.self) xp4[i] = vf4(xpv[t], wP[t]);
                 }
                 const int j = l + G * t;
                 if (active && j < nF) {
@@ -575,7 +577,7 @@ __global__ __launch_bounds__(256) void k_tiled_step(DeviceArrays d, StepArgs sa,
             st3(d.omega, f0 + j, om[t]);
         }
     }
-    if (ri.self && l == 0) d.selfOverflow[rod] += sl.misc[1];
+    if (SELF && ri.self && l == 0) d.selfOverflow[rod] += sl.misc[1];
     if (active && l == 0) d.clock[rod] = clk;
     if (observer) d.obs[rod].maxRise = maxRise;
 }
@@ -610,19 +612,23 @@ hipError_t launch_plain(const DeviceArrays& d, const StepArgs& sa, size_t lds, h
 }
 
 template <int TP, int TC>
-static hipError_t launch_tiled_t(const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
+static hipError_t launch_tiled_t(bool self, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
                                  const TileSlot* slots, int numTiles, int threads, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((k_tiled_step<TP, TC>), dim3(numTiles), dim3(threads), lds, s, d, sa, tiles, slots);
+    if (self)
+        hipLaunchKernelGGL((k_tiled_step<TP, TC, true>), dim3(numTiles), dim3(threads), lds, s, d, sa, tiles, slots);
+    else
+        hipLaunchKernelGGL((k_tiled_step<TP, TC, false>), dim3(numTiles), dim3(threads), lds, s, d, sa, tiles,
+                           slots);
     return hipGetLastError();
 }
 
 // Trip shapes compiled into the library; (TP, TC) pairs.
-hipError_t launch_tiled(int tp, int tc, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
+hipError_t launch_tiled(int tp, int tc, bool self, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
                         const TileSlot* slots, int numTiles, int threads, size_t lds, hipStream_t s) {
-    if (tp == 2 && tc == 1) return launch_tiled_t<2, 1>(d, sa, tiles, slots, numTiles, threads, lds, s);
-    if (tp == 3 && tc == 1) return launch_tiled_t<3, 1>(d, sa, tiles, slots, numTiles, threads, lds, s);
-    if (tp == 3 && tc == 2) return launch_tiled_t<3, 2>(d, sa, tiles, slots, numTiles, threads, lds, s);
-    if (tp == 5 && tc == 2) return launch_tiled_t<5, 2>(d, sa, tiles, slots, numTiles, threads, lds, s);
+    if (tp == 2 && tc == 1) return launch_tiled_t<2, 1>(self, d, sa, tiles, slots, numTiles, threads, lds, s);
+    if (tp == 3 && tc == 1) return launch_tiled_t<3, 1>(self, d, sa, tiles, slots, numTiles, threads, lds, s);
+    if (tp == 3 && tc == 2) return launch_tiled_t<3, 2>(self, d, sa, tiles, slots, numTiles, threads, lds, s);
+    if (tp == 5 && tc == 2) return launch_tiled_t<5, 2>(self, d, sa, tiles, slots, numTiles, threads, lds, s);
     return hipErrorInvalidValue;
 }
 
