@@ -178,11 +178,26 @@ def main():
     ap.add_argument("--tile-config", type=int, default=0)
     ap.add_argument("--ragged-steps", type=int, default=1000, help="steps of the ragged comparison (0: skip)")
     ap.add_argument("--ragged-plain-steps", type=int, default=200)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a known-traffic kernel for "
+                         "scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one, and this path has no cross-GPU "
               f"exchange (--gpus {args.gpus} requested).", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration_mib:
+        import torch
+
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calibration_mib} MiB read + {args.copy_calibration_mib} MiB written")
+        del src, dst
 
     w = scenes.uniform_floor_batch(num_rods=args.rods)
     K = args.steps if args.steps is not None else w.steps
