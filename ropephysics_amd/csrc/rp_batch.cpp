@@ -29,6 +29,7 @@ namespace rp {
 int plain_lds_bytes(int nP, int nF, int nS, int nB, int nPrim, int self, int table, int cap);
 int tiled_lds_bytes(int nP, int nF, int nS, int nPrim, int self, int table, int cap);
 hipError_t launch_plain(const DeviceArrays& d, const StepArgs& sa, size_t lds, hipStream_t s);
+hipError_t allow_dynamic_lds(int bytes, int* granted);
 hipError_t launch_tiled(int tp, int tc, bool self, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
                         const TileSlot* slots, int numTiles, int threads, size_t lds, hipStream_t s);
 hipError_t launch_fixed_frames(float* q, const float* src, const float* invInertia4, int f0, int count,
@@ -63,7 +64,7 @@ struct TileConfig {
 const TileConfig kConfigs[] = {
     {2, 1, 1, "tp2-tc1-w1"}, {3, 1, 1, "tp3-tc1-w1"}, {3, 2, 1, "tp3-tc2-w1"}, {5, 2, 1, "tp5-tc2-w1"},
     {2, 1, 2, "tp2-tc1-w2"}, {3, 1, 2, "tp3-tc1-w2"}, {3, 2, 2, "tp3-tc2-w2"}, {5, 2, 2, "tp5-tc2-w2"},
-    {2, 1, 4, "tp2-tc1-w4"}, {3, 1, 4, "tp3-tc1-w4"},
+    {2, 1, 4, "tp2-tc1-w4"}, {3, 1, 4, "tp3-tc1-w4"}, {1, 1, 2, "tp1-tc1-w2"}, {1, 1, 4, "tp1-tc1-w4"},
 };
 constexpr int kNumConfigs = int(sizeof(kConfigs) / sizeof(kConfigs[0]));
 
@@ -108,7 +109,8 @@ struct rp_batch_impl {
     int strategy = RP_STRATEGY_PLAIN;
     int requestedConfig = 0;
     int plainLds = 0;
-    size_t ldsLimit = 65536;
+    size_t ldsLimit = 65536;  // dynamic LDS a workgroup may take
+    size_t cuLds = 65536;     // LDS of a compute unit
     hipStream_t stream = nullptr;
 
     // device
@@ -230,13 +232,19 @@ int rod_tiled_lds(const RodInfo& r) {
 
 namespace {
 
-// Pack rods into workgroups of `waves` wavefronts: best fit over open
-// workgroups, rods taken widest first; a workgroup closes when neither lanes
-// nor LDS are left for the narrowest remaining rod.
-int pack_tiles(rp_batch_impl& b, int configId, const std::vector<int>& colStart, PackedTiles& out) {
+// Dynamic LDS budget of one workgroup when `perCU` workgroups are to share a
+// compute unit's LDS: a 512-byte allocation granule is left for the static
+// part.
+int lds_budget(const rp_batch_impl& b, int perCU) {
+    return std::min(int(b.ldsLimit), (int(b.cuLds / size_t(perCU)) & ~511) - 512);
+}
+
+// Pack rods into workgroups of `waves` wavefronts and at most `ldsCap` bytes
+// of LDS: best fit over open workgroups, rods taken widest first; a workgroup
+// closes when neither lanes nor LDS are left for the narrowest remaining rod.
+int pack_tiles(rp_batch_impl& b, int configId, int ldsCap, const std::vector<int>& colStart, PackedTiles& out) {
     const TileConfig& cfg = kConfigs[configId - 1];
     const int laneCap = 64 * cfg.waves;
-    const int ldsCap = int(b.ldsLimit);
     std::vector<int> lanes(b.numRods), lds(b.numRods), order(b.numRods);
     for (int r = 0; r < b.numRods; ++r) {
         if (!b.tileable[r]) return RP_ERR_CAPACITY;
@@ -384,7 +392,11 @@ int backup_state(rp_batch_impl& b, bool restore) {
 // steps, keep the fastest, and restore the state so the tuning leaves no trace.
 // Results do not depend on the configuration: each rod's arithmetic is the
 // same whatever lanes and workgroup it lands in.
-int autotune(rp_batch_impl& b, const rp_step_params& p) {
+// Time each tile shape (or only `onlyConfig`) at LDS budgets for 1 to 4
+// workgroups per compute unit: the budget decides how many workgroups there
+// are and how many are resident at once, so a batch whose workgroups spill
+// just past a multiple of the resident count pays a whole extra round.
+int autotune(rp_batch_impl& b, const rp_step_params& p, int onlyConfig) {
     if (!b.stateBackup) {
         int rc = b.alloc(b.stateBackup, (backup_bytes(b) + sizeof(float) * 2 + 0) / sizeof(float));
         if (rc) return rc;
@@ -397,10 +409,14 @@ int autotune(rp_batch_impl& b, const rp_step_params& p) {
     RP_HIP(hipEventCreate(&e0));
     RP_HIP(hipEventCreate(&e1));
     const int trialSteps = 8;
-    for (int id = 1; id <= kNumConfigs; ++id) {
+    b.tuneLog.clear();
+    for (int id = onlyConfig > 0 ? onlyConfig : 1; id <= (onlyConfig > 0 ? onlyConfig : kNumConfigs); ++id)
+      for (int perCU = 1, lastTiles = -1; perCU <= 4; ++perCU) {
         auto t = std::make_unique<PackedTiles>();
-        if (pack_tiles(b, id, b.colStart, *t) != RP_OK) continue;
-        if (size_t(t->maxLds) > b.ldsLimit) continue;
+        if (pack_tiles(b, id, lds_budget(b, perCU), b.colStart, *t) != RP_OK) continue;
+        // A budget that does not change the packing needs no second trial.
+        if (int(t->tiles.size()) == lastTiles) continue;
+        lastTiles = int(t->tiles.size());
         if (upload_tiles(b, *t) != RP_OK) continue;
         // A warm-up launch (code object load), then the best of two timed ones.
         if (launch_steps(b, p, 1, RP_STRATEGY_TILED, t.get()) != RP_OK) continue;
@@ -419,7 +435,7 @@ int autotune(rp_batch_impl& b, const rp_step_params& p) {
             best = ms;
             bestTiles = std::move(t);
         }
-    }
+      }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     rc = backup_state(b, true);
@@ -436,16 +452,13 @@ namespace {
 int resolve_tiles(rp_batch_impl& b, const rp_step_params& p) {
     if (b.tiles) return RP_OK;
     if (b.requestedConfig > 0) {
-        auto t = std::make_unique<PackedTiles>();
-        const int rc = pack_tiles(b, b.requestedConfig, b.colStart, *t);
+        PackedTiles probe;
+        const int rc = pack_tiles(b, b.requestedConfig, lds_budget(b, 1), b.colStart, probe);
         if (rc) return fail(rc, std::string("tile configuration ") + kConfigs[b.requestedConfig - 1].name +
                                     " does not fit this batch");
-        const int rc2 = upload_tiles(b, *t);
-        if (rc2) return rc2;
-        b.tiles = std::move(t);
-        return RP_OK;
+        return autotune(b, p, b.requestedConfig);
     }
-    return autotune(b, p);
+    return autotune(b, p, 0);
 }
 
 int check_range(const rp_batch_impl& b, int first, int count) {
@@ -485,13 +498,17 @@ int rp_device_query(rp_device_info* out) {
     out->lds_per_workgroup = prop.sharedMemPerBlock;
     out->total_global_mem = prop.totalGlobalMem;
     out->clock_khz = prop.clockRate;
-    // Longest clamped chain (n + 1 particles, n + 1 frames, n and n bend
-    // constraints) the widest configuration takes.
+    // Longest clamped chain (n + 1 particles, n + 1 frames, n stretch and n
+    // bend constraints, two colours of about n / 2 + 1) some configuration
+    // takes.
     int best = 0;
     for (int n2 = 4; n2 <= 4096; ++n2) {
-        const int lanes = std::max((n2 + 1 + 4) / 5, (n2 / 2 + 1));
         const int lds = tiled_lds_bytes(n2 + 1, n2 + 1, n2, 0, 0, 1, 0);
-        if (lanes <= 128 && size_t(lds) <= std::min<size_t>(prop.sharedMemPerBlock, 65536)) best = n2;
+        if (size_t(lds) > prop.sharedMemPerBlock) break;
+        for (const TileConfig& c : kConfigs) {
+            const int lanes = std::max((n2 + 1 + c.tp - 1) / c.tp, (n2 / 2 + 1 + c.tc - 1) / c.tc);
+            if (lanes <= 64 * c.waves) best = n2;
+        }
     }
     out->max_tiled_segments = best;
     return RP_OK;
@@ -669,7 +686,16 @@ int rp_batch_create(const rp_batch_desc* desc, void** out_batch) {
 
     hipDeviceProp_t prop;
     RP_HIP(hipGetDeviceProperties(&prop, 0));
-    b->ldsLimit = std::min<size_t>(prop.sharedMemPerBlock, 65536);
+    // Workgroups may use the whole LDS of a compute unit; where the runtime
+    // refuses the larger dynamic allocation, stay within 64 KiB.
+    b->cuLds = prop.sharedMemPerBlock;
+    int granted = 0;
+    if (allow_dynamic_lds(int(prop.sharedMemPerBlock), &granted) == hipSuccess) {
+        b->ldsLimit = size_t(granted);
+    } else {
+        (void)hipGetLastError();
+        b->cuLds = b->ldsLimit = 65536 - 1024;
+    }
     b->plainLds = maxPlainLds;
     for (int r = 0; r < R; ++r) b->anySelf = b->anySelf || b->rods[r].self != 0;
     bool allTileable = true;
@@ -681,9 +707,9 @@ int rp_batch_create(const rp_batch_desc* desc, void** out_batch) {
     if (strategy == RP_STRATEGY_AUTO) strategy = allTileable ? RP_STRATEGY_TILED : RP_STRATEGY_PLAIN;
     if (strategy == RP_STRATEGY_TILED && !allTileable)
         return fail(RP_ERR_CAPACITY, "a rod does not fit the TILED strategy (more than 2 colours or LDS budget)");
-    if (strategy == RP_STRATEGY_PLAIN && size_t(maxPlainLds) > prop.sharedMemPerBlock)
+    if (strategy == RP_STRATEGY_PLAIN && size_t(maxPlainLds) > b->ldsLimit)
         return fail(RP_ERR_CAPACITY, "a rod needs " + std::to_string(maxPlainLds) + " bytes of LDS; the device has " +
-                                         std::to_string(prop.sharedMemPerBlock) + " per workgroup");
+                                         std::to_string(b->ldsLimit) + " per workgroup");
     b->strategy = strategy;
     if (const char* e = std::getenv("RP_STEPS_PER_LAUNCH")) b->stepsPerLaunch = std::max(1, std::atoi(e));
 

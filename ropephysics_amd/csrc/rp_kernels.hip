@@ -298,9 +298,12 @@ __host__ __device__ inline TiledLayout tiled_layout(int nP, int nF, int nS, int 
 // constraints of each colour; the host sizes G per rod to satisfy that.
 // SELF: compiled with rod self-collision; batches without it run the variant
 // without, whose smaller register footprint keeps two waves per SIMD.
+//
+// Shapes with TP <= 2 are held to 256 registers, so two waves fit on a SIMD
+// (the self-collision variant spills a few bytes to do so).
 template <int TP, int TC, bool SELF>
-__global__ __launch_bounds__(256) void k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles,
-                                                    const TileSlot* slots) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2 : 1))) void k_tiled_step(
+    DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slots) {
     extern __shared__ float4 smem4[];
     char* smem = reinterpret_cast<char*>(smem4);
     const Tile tile = tiles[blockIdx.x];
@@ -466,7 +469,7 @@ __global__ __launch_bounds__(256) void k_tiled_step(DeviceArrays d, StepArgs sa,
             }
             if (anySelf) {
                 if (regen) {
-                    self_regen_group(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
+                    self_regen_group<TP>(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
                                      ri.selfFriction * fscale, ri.self != 0, l, G);
                 } else if (ri.self && l == 0) {
                     self_reset(sl);
@@ -622,9 +625,43 @@ static hipError_t launch_tiled_t(bool self, const DeviceArrays& d, const StepArg
     return hipGetLastError();
 }
 
+// Raise a kernel's dynamic LDS limit to `bytes` less its static LDS (the
+// wave-vote barriers keep a little); lowers `*granted` to what it got.
+static hipError_t allow_lds(const void* kernel, int bytes, int* granted) {
+    hipFuncAttributes a;
+    hipError_t e = hipFuncGetAttributes(&a, kernel);
+    if (e != hipSuccess) return e;
+    const int dyn = bytes - int(a.sharedSizeBytes);
+    e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    if (e == hipSuccess && dyn < *granted) *granted = dyn;
+    return e;
+}
+
+template <int TP, int TC>
+static hipError_t allow_tiled_lds(int bytes, int* granted) {
+    const hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_tiled_step<TP, TC, true>), bytes, granted);
+    if (e != hipSuccess) return e;
+    return allow_lds(reinterpret_cast<const void*>(&k_tiled_step<TP, TC, false>), bytes, granted);
+}
+
+// Let every stepping kernel take up to `bytes` of LDS per workgroup, static
+// and dynamic together; `*granted` receives the dynamic bytes every one of
+// them may use.
+hipError_t allow_dynamic_lds(int bytes, int* granted) {
+    *granted = bytes;
+    hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_plain_step), bytes, granted);
+    if (e == hipSuccess) e = allow_tiled_lds<1, 1>(bytes, granted);
+    if (e == hipSuccess) e = allow_tiled_lds<2, 1>(bytes, granted);
+    if (e == hipSuccess) e = allow_tiled_lds<3, 1>(bytes, granted);
+    if (e == hipSuccess) e = allow_tiled_lds<3, 2>(bytes, granted);
+    if (e == hipSuccess) e = allow_tiled_lds<5, 2>(bytes, granted);
+    return e;
+}
+
 // Trip shapes compiled into the library; (TP, TC) pairs.
 hipError_t launch_tiled(int tp, int tc, bool self, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
                         const TileSlot* slots, int numTiles, int threads, size_t lds, hipStream_t s) {
+    if (tp == 1 && tc == 1) return launch_tiled_t<1, 1>(self, d, sa, tiles, slots, numTiles, threads, lds, s);
     if (tp == 2 && tc == 1) return launch_tiled_t<2, 1>(self, d, sa, tiles, slots, numTiles, threads, lds, s);
     if (tp == 3 && tc == 1) return launch_tiled_t<3, 1>(self, d, sa, tiles, slots, numTiles, threads, lds, s);
     if (tp == 3 && tc == 2) return launch_tiled_t<3, 2>(self, d, sa, tiles, slots, numTiles, threads, lds, s);

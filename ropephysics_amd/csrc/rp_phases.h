@@ -165,7 +165,7 @@ __device__ inline v3 path_velocity(const PathRec& p, int step) {
 // One rod's working set in LDS. Segments are visited in the rod's original
 // stretch-constraint order, as the CPU broadphase does.
 struct SelfLds {
-    float4* cen;     // nS centres
+    float4* cen;     // nS centres and lengths (after a lane-group build: in bucket order)
     int* cellOf;     // nS
     int* sorted;     // nS
     int* segCount;   // nS
@@ -221,6 +221,43 @@ __device__ inline void self_build_hash(const SelfLds& s, const float4* x4, const
     s.cellStart[0] = 0;
 }
 
+// Narrow phase of the candidate pair (segment j = particles j0, j1 at a0, a1;
+// segment k = particles k0, k1): a contact when the closest points are nearer
+// than 2r. Each contact takes the next slot from slotBase; with `fill`, a slot
+// below the capacity receives the contact record.
+__device__ __forceinline__ void self_pair(const SelfLds& s, const float4* x4, int j0, int j1, v3 a0, v3 a1, int k0,
+                                          int k1, float radius, float friction, bool fill, int slotBase, int cap,
+                                          int& count) {
+    const v3 b0 = f4v(x4[k0]), b1 = f4v(x4[k1]);
+    float u, v;
+    seg_seg(a0, a1, b0, b1, u, v);
+    const v3 pa = add(a0, scale(sub(a1, a0), u));
+    const v3 pb = add(b0, scale(sub(b1, b0), v));
+    const v3 delta = sub(pa, pb);
+    const float dist = norm(delta);
+    const float sep = dist - 2.0f * radius;
+    if (sep >= 0.0f) return;
+    const int slot = slotBase + count++;
+    if (!fill || slot >= cap) return;
+    const v3 n = dist > 1e-12f ? divs(delta, dist) : mk(0.0f, 0.0f, 1.0f);
+    const int idx[4] = {j0, j1, k0, k1};
+    const float wt[4] = {1.0f - u, u, -(1.0f - v), -v};
+    float dotSum = 0.0f;
+    for (int m = 0; m < 4; ++m) dotSum += wt[m] * dot(f4v(x4[idx[m]]), n);
+    float* c = s.pool + slot * kSelfFloats;
+    for (int m = 0; m < 4; ++m) {
+        c[m] = __int_as_float(idx[m]);
+        c[4 + m] = wt[m];
+    }
+    c[8] = n.x;
+    c[9] = n.y;
+    c[10] = n.z;
+    c[11] = dotSum - sep;  // offset
+    c[12] = friction;
+    c[13] = 0.0f;  // lamN
+    c[14] = 0.0f;  // applied
+}
+
 // Contacts of segment j with later segments in the CPU's candidate order
 // (27 cells dz, dy, dx; bucket items ascending; a bucket reached twice is
 // visited twice). Counts them, or with `fill` writes them at j's pool offset.
@@ -233,6 +270,7 @@ __device__ inline void self_query(const SelfLds& s, const float4* x4, const int*
         const int cx = cell_coord(cj.x, cell), cy = cell_coord(cj.y, cell), cz = cell_coord(cj.z, cell);
         const int j0 = segEnd[2 * j], j1 = segEnd[2 * j + 1];
         const v3 a0 = f4v(x4[j0]), a1 = f4v(x4[j1]);
+        const int slotBase = fill ? s.segStart[j] : 0;
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
@@ -241,35 +279,8 @@ __device__ inline void self_query(const SelfLds& s, const float4* x4, const int*
                     for (int e = s.cellStart[h]; e < e1; ++e) {
                         const int k = s.sorted[e];
                         if (k - j < gap) continue;
-                        const int k0 = segEnd[2 * k], k1 = segEnd[2 * k + 1];
-                        const v3 b0 = f4v(x4[k0]), b1 = f4v(x4[k1]);
-                        float u, v;
-                        seg_seg(a0, a1, b0, b1, u, v);
-                        const v3 pa = add(a0, scale(sub(a1, a0), u));
-                        const v3 pb = add(b0, scale(sub(b1, b0), v));
-                        const v3 delta = sub(pa, pb);
-                        const float dist = norm(delta);
-                        const float sep = dist - 2.0f * radius;
-                        if (sep >= 0.0f) continue;
-                        const int slot = s.segStart[j] + count++;
-                        if (!fill || slot >= cap) continue;
-                        const v3 n = dist > 1e-12f ? divs(delta, dist) : mk(0.0f, 0.0f, 1.0f);
-                        const int idx[4] = {j0, j1, k0, k1};
-                        const float wt[4] = {1.0f - u, u, -(1.0f - v), -v};
-                        float dotSum = 0.0f;
-                        for (int m = 0; m < 4; ++m) dotSum += wt[m] * dot(f4v(x4[idx[m]]), n);
-                        float* c = s.pool + slot * kSelfFloats;
-                        for (int m = 0; m < 4; ++m) {
-                            c[m] = __int_as_float(idx[m]);
-                            c[4 + m] = wt[m];
-                        }
-                        c[8] = n.x;
-                        c[9] = n.y;
-                        c[10] = n.z;
-                        c[11] = dotSum - sep;  // offset
-                        c[12] = friction;
-                        c[13] = 0.0f;  // lamN
-                        c[14] = 0.0f;  // applied
+                        self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * k], segEnd[2 * k + 1], radius, friction, fill,
+                                  slotBase, cap, count);
                     }
                 }
     }
@@ -315,44 +326,156 @@ __device__ inline int group_scan(const int* in, int* out, int n, int* scratch, i
     return total;
 }
 
+__device__ __forceinline__ float abs_max3(v3 a) { return fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z))); }
+__device__ __forceinline__ int count_eq4(int4 c, int h) {
+    return (c.x == h ? 1 : 0) + (c.y == h ? 1 : 0) + (c.z == h ? 1 : 0) + (c.w == h ? 1 : 0);
+}
+
+// self_query for the lane group: segment j's centre and length (cj4, held by
+// the lane) and candidates' centres stored in bucket order (s.cen[e] belongs
+// to s.sorted[e]), so a candidate costs one LDS round trip; the three buckets
+// of a (dz, dy) row are fetched together. Same candidates in the same order as
+// self_query, with one addition: a candidate whose centre is farther from j's
+// than the two half-lengths plus 2r (with a relative margin of 1e-3 and an
+// absolute one of 1e-6 times the coordinate scale, far above the FP32
+// rounding of centres and lengths) is dropped before the closest-point test.
+// Every point of a segment lies within half its length of its centre, so such
+// a pair's computed separation is positive and it would not be a contact: the
+// contact list is unchanged. Returns the number of contacts.
+template <bool kFill>
+__device__ inline int self_query_rows(const SelfLds& s, const float4* x4, const int* segEnd, int table, int gap,
+                                      int cap, float radius, float friction, float cell, int j, float4 cj4,
+                                      int slotBase) {
+    const v3 cj = f4v(cj4);
+    const float reachJ = 0.5f * cj4.w + 2.0f * radius;
+    const float scaleJ = abs_max3(cj);
+    const int cx = cell_coord(cj.x, cell), cy = cell_coord(cj.y, cell), cz = cell_coord(cj.z, cell);
+    const int j0 = segEnd[2 * j], j1 = segEnd[2 * j + 1];
+    const v3 a0 = f4v(x4[j0]), a1 = f4v(x4[j1]);
+    const unsigned tbl = static_cast<unsigned>(table);
+    // Usually every lane group of the wave has a power-of-two table: then the
+    // modulo path is not issued at all.
+    const bool pow2 = __all((tbl & (tbl - 1u)) == 0u) != 0;
+    int count = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy) {
+            // cell_hash's XOR of the three products, the x term varying fastest.
+            const unsigned hzy =
+                (static_cast<unsigned>(cz + dz) * 83492791u) ^ (static_cast<unsigned>(cy + dy) * 19349663u);
+            int lo[3], hi[3];
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const unsigned hh = (static_cast<unsigned>(cx + dx - 1) * 73856093u) ^ hzy;
+                const int h = static_cast<int>(pow2 ? (hh & (tbl - 1u)) : hh % tbl);
+                lo[dx] = s.cellStart[h];
+                hi[dx] = s.cellStart[h + 1];
+            }
+            // The row's candidates, buckets dx = -1, 0, 1 concatenated, four
+            // at a time: their loads go out together (indices past the end
+            // repeat the last candidate, so every load is in range and none
+            // waits on a test), the culls run side by side without branches,
+            // and survivors reach the narrow phase in candidate order.
+            const int n0 = hi[0] - lo[0], n01 = n0 + hi[1] - lo[1], total = n01 + hi[2] - lo[2];
+            for (int base = 0; base < total; base += 4) {
+                int kq[4];
+                float4 cq[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = min(base + u, total - 1);
+                    const int e = q < n0 ? lo[0] + q : (q < n01 ? lo[1] + (q - n0) : lo[2] + (q - n01));
+                    kq[u] = s.sorted[e];
+                    cq[u] = s.cen[e];
+                }
+                unsigned pass = 0u;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const v3 ck = f4v(cq[u]);
+                    const float lim =
+                        1.001f * (reachJ + 0.5f * cq[u].w) + 1e-6f * (1.0f + fmaxf(scaleJ, abs_max3(ck)));
+                    const bool keep = (base + u < total) & (kq[u] - j >= gap) & !(norm2(sub(ck, cj)) > lim * lim);
+                    pass |= keep ? (1u << u) : 0u;
+                }
+                while (pass != 0u) {
+                    const int u = __builtin_ctz(pass);
+                    pass &= pass - 1u;
+                    const int k = u == 0 ? kq[0] : u == 1 ? kq[1] : u == 2 ? kq[2] : kq[3];
+                    self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * k], segEnd[2 * k + 1], radius, friction, kFill,
+                              slotBase, cap, count);
+                }
+            }
+        }
+    return count;
+}
+
+// Broadphase and contact generation by the G lanes of a rod's lane group; lane
+// l owns segments j = l + G t, t < TP (the host guarantees nS <= TP G), and
+// keeps their centres in registers. s.cen ends up in bucket order.
+template <int TP>
 __device__ inline void self_regen_group(const SelfLds& s, const float4* x4, const int* segEnd, int nS, int table,
                                         int gap, int cap, float radius, float friction, bool on, int l, int G) {
+    float4 cj[TP];
     if (on) {
         for (int t = l; t <= table; t += G) s.cellStart[t] = 0;
         if (l == 0) s.misc[3] = 0;
     }
     __syncthreads();
-    if (on)
-        for (int j = l; j < nS; j += G) {
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+        const int j = l + G * t;
+        cj[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (on && j < nS) {
             const v3 a = f4v(x4[segEnd[2 * j]]), b = f4v(x4[segEnd[2 * j + 1]]);
-            s.cen[j] = vf4(scale(add(a, b), 0.5f), 0.0f);
-            atomicMax(&s.misc[3], __float_as_int(norm(sub(b, a))));
+            const float len = norm(sub(b, a));
+            cj[t] = vf4(scale(add(a, b), 0.5f), len);
+            atomicMax(&s.misc[3], __float_as_int(len));
         }
+    }
     __syncthreads();
-    if (on) {
-        const float cell = __int_as_float(s.misc[3]) + 2.0f * radius;
-        if (l == 0) s.misc[2] = __float_as_int(cell);
-        for (int j = l; j < nS; j += G) {
-            const v3 c = f4v(s.cen[j]);
+    const float cell = on ? __int_as_float(s.misc[3]) + 2.0f * radius : 0.0f;
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+        const int j = l + G * t;
+        if (on && j < nS) {
+            const v3 c = f4v(cj[t]);
             const int h = int(cell_hash(cell_coord(c.x, cell), cell_coord(c.y, cell), cell_coord(c.z, cell), table));
             s.cellOf[j] = h;
             atomicAdd(&s.cellStart[h], 1);
         }
     }
+    if (on && l == 0) s.misc[2] = __float_as_int(cell);
     __syncthreads();
     const int items = group_scan(s.cellStart, s.cellStart, on ? table : 0, s.scan, l, G, on);
-    if (on) {
-        if (l == 0) s.cellStart[table] = items;
-        for (int j = l; j < nS; j += G) {
+    if (on && l == 0) s.cellStart[table] = items;
+    // Stable placement: j goes after the earlier segments of its bucket.
+    // cellOf is 16-byte aligned, so four buckets per LDS read.
+    const int4* cellOf4 = reinterpret_cast<const int4*>(s.cellOf);
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+        const int j = l + G * t;
+        if (on && j < nS) {
             const int h = s.cellOf[j];
-            int rank = 0;
-            for (int i = 0; i < j; ++i) rank += s.cellOf[i] == h ? 1 : 0;
-            s.sorted[s.cellStart[h] + rank] = j;
+            int rank = 0, i = 0;
+            for (; i + 16 <= j; i += 16) {
+                const int4 c0 = cellOf4[(i >> 2) + 0], c1 = cellOf4[(i >> 2) + 1];
+                const int4 c2 = cellOf4[(i >> 2) + 2], c3 = cellOf4[(i >> 2) + 3];
+                rank += count_eq4(c0, h) + count_eq4(c1, h) + count_eq4(c2, h) + count_eq4(c3, h);
+            }
+            for (; i + 4 <= j; i += 4) rank += count_eq4(cellOf4[i >> 2], h);
+            for (; i < j; ++i) rank += s.cellOf[i] == h ? 1 : 0;
+            const int e = s.cellStart[h] + rank;
+            s.sorted[e] = j;
+            s.cen[e] = cj[t];
         }
     }
     __syncthreads();
-    if (on)
-        for (int j = l; j < nS; j += G) self_query(s, x4, segEnd, nS, table, gap, cap, radius, friction, j, false);
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+        const int j = l + G * t;
+        if (on && j < nS)
+            s.segCount[j] = nS > gap ? self_query_rows<false>(s, x4, segEnd, table, gap, cap, radius, friction,
+                                                              cell, j, cj[t], 0)
+                                     : 0;
+    }
     __syncthreads();
     const int total = group_scan(s.segCount, s.segStart, on ? nS : 0, s.scan, l, G, on);
     if (on && l == 0) {
@@ -360,8 +483,12 @@ __device__ inline void self_regen_group(const SelfLds& s, const float4* x4, cons
         if (total > cap) s.misc[1] += total - cap;
     }
     __syncthreads();
-    if (on)
-        for (int j = l; j < nS; j += G) self_query(s, x4, segEnd, nS, table, gap, cap, radius, friction, j, true);
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+        const int j = l + G * t;
+        if (on && j < nS && s.segCount[j] > 0)
+            self_query_rows<true>(s, x4, segEnd, table, gap, cap, radius, friction, cell, j, cj[t], s.segStart[j]);
+    }
 }
 
 // Counts -> pool offsets; contacts past the capacity are tallied as overflow.

@@ -202,6 +202,40 @@ def test_self_collision_overflow_is_counted():
         assert tb.self_contact_overflow() > 0
 
 
+def pinned_coil(turns=5, per_turn=16, pitch=0.008, radius=0.025):
+    """A helix whose pitch (8 mm) is below the rope's diameter (10 mm), both
+    ends pinned: the turns keep pressing on each other."""
+    t = np.arange(per_turn * turns + 1) / per_turn * 2 * np.pi
+    pts = np.stack([radius * np.cos(t), radius * np.sin(t), 0.03 + pitch * t / (2 * np.pi)], axis=-1)
+    rod = rp.make_rod_from_centerline(pts, rp.RodMaterial(2e6, 0.35, 1000.0, 5e-3), (0, 0, 1))
+    rod.pin_particle(0)
+    rod.pin_particle(rod.num_segments)
+    return rod
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
+@pytest.mark.parametrize("table", [16, 64, 256])
+def test_self_collision_pinned_coil_hash_tables(strategy, table):
+    """Dense, changing self-contact sets. Small tables make neighbour cells
+    share buckets, so a bucket is visited more than once and its contacts
+    appear more than once in the list, as on the CPU. The pool is sized so that nothing overflows (the CPU list is
+    unbounded). The batch also holds a free helix and a rod without contacts
+    of its own."""
+    world = rp.CollisionWorld(primitives=[rp.Primitive.plane((0, 0, 0), (0, 0, 1), 0.6)], self_collision=True,
+                              self_friction=0.3, hash_table_size=table, self_capacity=700)
+    rods = [pinned_coil(), helix_rope(), benchmark_rod(24)]
+    diffs, _, b = run_pair(rods, world, SELF_PARAMS, 30, strategy, checkpoints=[1, 3, 30])
+    assert b.self_contact_overflow() == 0
+    assert_bitwise(diffs)
+    o = oracle_twin(rods[0], 32)
+    op, ow = oracle_params(SELF_PARAMS), oracle_world(world)
+    seen = 0
+    for _ in range(3):
+        o.step(op, ow, 1)
+        seen = max(seen, o.contact_counts()[1])
+    assert seen >= 10  # the coil really is in contact
+
+
 @pytest.mark.parametrize("strategy", STRATEGIES)
 def test_ragged_batch_and_alone_equals_mixed(strategy):
     sizes = [7, 20, 33, 64, 97, 130, 160, 48]
