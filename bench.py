@@ -1,13 +1,18 @@
 #!/usr/bin/env python3
 """Throughput of the batched XPBD rod stepper on one MI355X.
 
-Workload (default): the uniform batch of 4096 rods x 64 segments over a floor
-plane with per-rod stiffness and friction, dt = 1e-3, 10 substeps, 1000 steps
-(synthetic inputs from a fixed seed). A "step" is one step of every rod in the
-batch. The TILED stepper is timed over --steps steps after --warmup untimed
-ones; the PLAIN one-rod-per-workgroup launch (the reference's launch shape) is
-timed on the same workload beside it, and the reference CPU path on a bounded
-sample of it.
+Headline workload: the ragged batch of 1024 cable-routing rollouts (cables of
+48-160 segments, 1.2 m, a gripper path per cable evaluated on the device,
+floor, two capsule pegs and a box clip, self-collision on, dt = 1e-3, 10
+substeps, the whole 6 s = 6000 steps; synthetic inputs from a fixed seed). A
+"step" is one step of every rod in the batch. The ragged-packed TILED stepper
+is timed over --steps steps after --warmup untimed ones (the autotuner runs
+during the warm-up); the PLAIN one-rod-per-workgroup launch (the reference's
+launch shape) is timed over the same steps of the same workload beside it; the
+reference CPU path runs a bounded sample of it.
+
+Secondary: the uniform batch of 4096 rods x 64 segments over a floor plane with
+per-rod stiffness and friction, 1000 steps, TILED and PLAIN.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -29,12 +34,18 @@ from ropephysics_amd import scenes  # noqa: E402
 
 METRIC = "segment-substeps/s, ragged-batched vs plain on 1 GPU; p50/p99 solve latency"
 
-# Algorithmic FP32 operations per segment-substep (adds, multiplies, divides
-# and square roots, each counted once) of the contact-free step with one sweep,
-# counted by running the oracle with an operation-counting scalar over a
-# clamped 64-segment rod for 10 steps of 10 substeps
-# (tests/test_oracle_pins.py::test_flop_count_frozen checks this value).
+# Algorithmic FP32 operations (adds, multiplies, divides and square roots, each
+# counted once) per segment-substep, counted by running the oracle with an
+# operation-counting scalar (tests/test_oracle_pins.py::test_flop_counts_frozen
+# re-derives both):
+#  - contact-free step, one sweep: a clamped 64-segment rod, 10 steps of 10
+#    substeps;
+#  - cable routing: every 64th cable of the routing workload from its initial
+#    pose, 10 steps of 10 substeps, with the routing world (floor, two pegs,
+#    clip: contact generation and projection) and self-collision (spatial hash,
+#    closest points of every candidate pair).
 FLOPS_PER_SEGMENT_SUBSTEP = 888.39
+FLOPS_PER_SEGMENT_SUBSTEP_ROUTING = 1483.08
 
 # Bytes the TILED kernel moves through LDS per segment-substep with one sweep
 # and no contacts: predict 112 (x, force, kinematic velocity, orientation,
@@ -45,21 +56,14 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector/matrix peak
 PEAK_HBM_GBS = 8000.0
 PEAK_LDS_GBS = 128.0 * 256 * 2.4  # 128 B/clk/CU x 256 CUs x 2.4 GHz
 
+TRAFFIC = {
+    "routing": os.path.join(HERE, "profiles", "k_tiled_step_routing-1024-ragged_traffic.json"),
+    "uniform": os.path.join(HERE, "profiles", "k_tiled_step_uniform-4096x64-floor_traffic.json"),
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def make_batch(w: scenes.Workload, strategy: str) -> rp.Batch:
-    b = rp.Batch(w.prototype, w.num_rods, strategy=strategy, world=w.world)
-    b.set_material_scales(w.youngs_scale, w.friction_scale)
-    return b
-
-
-def reset(b: rp.Batch, w: scenes.Workload):
-    R = w.num_rods
-    p = w.prototype
-    b.set_state(0, R, x=w.x0, v=np.zeros_like(w.x0), q=np.tile(p.q, (R, 1)), omega=np.tile(p.omega, (R, 1)))
 
 
 def time_steps(b: rp.Batch, params, steps: int):
@@ -68,15 +72,38 @@ def time_steps(b: rp.Batch, params, steps: int):
     b.step(params, steps)
     b.synchronize()
     t1 = time.perf_counter()
-    c = b.counters()
-    return t1 - t0, c
+    return t1 - t0, b.counters()
 
 
-def make_ragged_batch(w: scenes.Workload, strategy: str) -> rp.Batch:
-    b = rp.Batch(w.rods, strategy=strategy, world=w.world)
+def roofline(flops_per_unit: float, kernel_rate: float, kernel_ms: float, units_per_launch: float, traffic_key):
+    achieved = flops_per_unit * kernel_rate / 1e12
+    out = {
+        "bound": "mfma",
+        "bound_detail": "FP32 vector-ALU bound; the path has no dense contraction and issues no MFMA, so the "
+                        "FP32 peak is the compute roofline (unpacked FP32 tops out at half of it)",
+        "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+        "traffic": None, "kernel": "k_tiled_step", "flops_per_segment_substep": flops_per_unit,
+        "segment_substeps_per_launch": units_per_launch, "kernel_ms_avg": kernel_ms,
+        "lds": {"bytes_per_segment_substep_contact_free": LDS_BYTES_PER_SEGMENT_SUBSTEP,
+                "achieved_GBs_contact_free_share": LDS_BYTES_PER_SEGMENT_SUBSTEP * kernel_rate / 1e9,
+                "peak_GBs": PEAK_LDS_GBS},
+    }
+    path = TRAFFIC.get(traffic_key)
+    if path and os.path.exists(path):
+        with open(path) as f:
+            tr = json.load(f)
+        out["traffic"] = tr.get("hbm_bytes_per_launch")
+        out["traffic_source"] = os.path.relpath(path, HERE)
+        out["traffic_launch_steps"] = tr.get("launch", {}).get("steps_per_launch")
+    return out
+
+
+# ------------------------------------------------------------------ ragged routing (headline)
+
+def make_ragged_batch(w: scenes.Workload, strategy: str, tile_config: int = 0) -> rp.Batch:
+    b = rp.Batch(w.rods, strategy=strategy, world=w.world, tile_config=tile_config)
     b.set_material_scales(w.youngs_scale, w.friction_scale)
-    if "paths" in w.info:
-        b.set_paths(0, w.info["paths"])
+    b.set_paths(0, w.info["paths"])
     return b
 
 
@@ -85,99 +112,142 @@ def reset_ragged(b: rp.Batch, w: scenes.Workload):
     b.set_clock()
 
 
-def ragged_comparison(steps: int, plain_steps: int, warmup: int) -> dict:
-    """The ragged routing workload (1024 cables of 48-160 segments with
-    self-collision, gripper paths on the device), TILED and PLAIN, each from
-    the workload's initial state over its first `steps` / `plain_steps`
-    steps."""
-    w = scenes.routing_rollouts()
-    log(f"ragged workload {w.name}: {w.num_rods} rods, {w.segments_total} segments")
-    out = {"workload": w.name, "rods": w.num_rods, "segments_total": w.segments_total,
-           "segments": "48-160 uniform", "self_collision": True, "substeps": w.params.substeps}
+def routing_leg(w: scenes.Workload, steps: int, plain_steps: int, warmup: int, tile_config: int,
+                no_plain: bool) -> dict:
+    out = {}
     for strategy, k in (("tiled", steps), ("plain", plain_steps)):
-        b = make_ragged_batch(w, strategy)
+        if strategy == "plain" and (no_plain or k <= 0):
+            continue
+        b = make_ragged_batch(w, strategy, tile_config if strategy == "tiled" else 0)
         reset_ragged(b, w)
-        b.step(w.params, min(warmup, 4))
+        b.step(w.params, warmup if strategy == "tiled" else min(warmup, 4))
         b.synchronize()
         reset_ragged(b, w)
         b.set_timing(True)
         secs, c = time_steps(b, w.params, k)
         work = float(w.segments_total) * w.params.substeps * k
-        ent = {"value": work / secs, "steps": k, "seconds": secs,
-               "kernel_ms_avg": c["last_step_gpu_ms"] / max(1, c["last_step_launches"]),
+        launches = max(1, c["last_step_launches"])
+        ent = {"value": work / secs, "steps": k, "seconds": secs, "ms_per_step": secs * 1e3 / k,
+               "device_ms": c["last_step_gpu_ms"], "launches": launches,
+               "kernel_ms_avg": c["last_step_gpu_ms"] / launches,
                "self_contact_overflow": c["self_contact_overflow"]}
         if strategy == "tiled":
             ent.update({"tile_config": c.get("tile_config_name"), "autotune_trials_ms": b.tune_report(),
-                        "workgroups": c["workgroups"],
-                        "lanes_busy": c["lanes_busy"], "lanes_total": c["lanes_total"],
-                        "lds_bytes_per_workgroup": c["lds_bytes_per_workgroup"]})
+                        "workgroups": c["workgroups"], "lanes_busy": c["lanes_busy"],
+                        "lanes_total": c["lanes_total"], "lds_bytes_per_workgroup": c["lds_bytes_per_workgroup"]})
             x = b.download_positions()
             if not np.all(np.isfinite(x)):
-                raise RuntimeError("non-finite state in the ragged run")
-        log(f"ragged {strategy}: {secs:.3f} s for {k} steps, {ent['value'] / 1e9:.3f} G seg-substeps/s")
+                raise RuntimeError("non-finite state after the timed routing run")
+            task = w.info["task"]
+            routed = [task.evaluate(b.download(i)["x"].astype(np.float64))["routed"] for i in range(w.num_rods)] \
+                if k == w.steps else None
+            ent["routed"] = None if routed is None else int(sum(routed))
+        log(f"routing {strategy} {ent.get('tile_config', '')}: {secs:.3f} s for {k} steps, "
+            f"{ent['value'] / 1e9:.3f} G seg-substeps/s, device {c['last_step_gpu_ms']:.1f} ms over "
+            f"{launches} launches")
         out[strategy] = ent
         b.destroy()
-    out["speedup_vs_plain"] = out["tiled"]["value"] / out["plain"]["value"]
     return out
 
 
-def cpu_baseline(w: scenes.Workload, stride: int, threads: int):
+def cpu_baseline_routing(w: scenes.Workload, stride: int, steps: int, threads: int) -> dict:
     """The reference CPU path (its own sources, built into oracle/_ref) on every
-    `stride`-th rod of the workload for all its steps, FP64, on `threads` host
-    threads with a static partition over rods."""
+    `stride`-th cable of the routing workload over its first `steps` steps, FP64,
+    the gripper schedule applied before each step, on `threads` host threads
+    with a static partition over rods."""
     import oracle  # the CPU checker/baseline; never part of the product path
 
     kind = "ref64" if oracle.available("ref64") else "oracle"
     sel = np.arange(0, w.num_rods, stride)
-    nP = w.prototype.num_particles
-    arrays = w.prototype.arrays()
-    rods = []
+    paths = w.info["paths"]
+    rods, worlds = [], []
+    p = w.params
     for r in sel:
-        o = oracle.Rod.straight(2, 1.0, w.prototype.material.as_tuple(), precision=64, kind=kind)
-        a = dict(arrays)
-        a["x"] = w.x0[r * nP:(r + 1) * nP]
-        a["xPrev"] = a["x"]
-        a["sComp"] = arrays["sComp"] / float(w.youngs_scale[r])
-        a["bComp"] = arrays["bComp"] / float(w.youngs_scale[r])
+        rod = w.rods[r]
+        o = oracle.Rod.straight(2, 1.0, rod.material.as_tuple(), precision=64, kind=kind)
+        a = rod.arrays()
+        a["sComp"] = a["sComp"] / float(w.youngs_scale[r])
+        a["bComp"] = a["bComp"] / float(w.youngs_scale[r])
         o.set(a)
         rods.append(o)
-    # Per-rod stiffness and friction, as the reference CPU applies them: the
-    # rod's material and the world's friction. The scales are folded into
-    # per-rod parameters here; the work per step is the same.
-    worlds = []
-    for r in sel:
-        pr = w.world.primitives[0]
-        worlds.append(oracle.World(prims=[oracle.make_prim(pr.type, pr.a, pr.b, pr.normal, pr.half_extents,
-                                                           pr.rotation, pr.radius,
-                                                           pr.friction * float(w.friction_scale[r]))]))
-    p = w.params
+        prims = [oracle.make_prim(pr.type, pr.a, pr.b, pr.normal, pr.half_extents, pr.rotation, pr.radius,
+                                  pr.friction * float(w.friction_scale[r])) for pr in w.world.primitives]
+        worlds.append(oracle.World(prims=prims, self_collision=True,
+                                   self_friction=w.world.self_friction * float(w.friction_scale[r]),
+                                   self_skip=w.world.self_skip, hash_table_size=w.world.hash_table_size))
+    kv = np.array([[paths[r].velocity(s) for s in range(steps)] for r in sel])
     params = oracle.Params(dt=p.dt, substeps=p.substeps, iterations=p.iterations, gravity=p.gravity,
                            coloured=False)
     t0 = time.perf_counter()
-    oracle.step_many(rods, params, worlds, w.steps, threads)
+    oracle.step_many_driven(rods, params, worlds, steps, threads, [paths[r].particle for r in sel], kv)
     dt = time.perf_counter() - t0
-    work = len(sel) * w.prototype.num_segments * p.substeps * w.steps
+    segs = sum(w.rods[r].num_segments for r in sel)
+    work = segs * p.substeps * steps
     return {"value": work / dt, "unit": "segment-substeps/s", "cores": threads,
             "kind": "reference" if kind == "ref64" else "port",
-            "sample": f"{len(sel)} of {w.num_rods} rods (every {stride}th), all {w.steps} steps, FP64, "
-                      f"sequential sweep order, {threads} threads static partition; {dt:.1f} s",
+            "sample": f"{len(sel)} of {w.num_rods} cables (every {stride}th, {segs} segments), first {steps} of "
+                      f"{w.steps} steps, gripper schedule applied per step, FP64, sequential sweep order, "
+                      f"{threads} threads static partition over rods; {dt:.1f} s",
             "seconds": dt}
+
+
+# ------------------------------------------------------------------ uniform batch (secondary)
+
+def uniform_leg(steps: int, warmup: int, no_plain: bool) -> dict:
+    w = scenes.uniform_floor_batch(steps=steps)
+    R, p = w.num_rods, w.prototype
+
+    def make(strategy):
+        b = rp.Batch(p, R, strategy=strategy, world=w.world)
+        b.set_material_scales(w.youngs_scale, w.friction_scale)
+        return b
+
+    def reset(b):
+        b.set_state(0, R, x=w.x0, v=np.zeros_like(w.x0), q=np.tile(p.q, (R, 1)), omega=np.tile(p.omega, (R, 1)))
+
+    out = {"config": {"workload": w.name, **w.info}}
+    for strategy in ("tiled", "plain"):
+        if strategy == "plain" and no_plain:
+            continue
+        b = make(strategy)
+        reset(b)
+        b.step(w.params, warmup if strategy == "tiled" else min(warmup, 4))
+        b.synchronize()
+        reset(b)
+        b.set_timing(True)
+        secs, c = time_steps(b, w.params, steps)
+        work = float(w.segments_total) * w.params.substeps * steps
+        launches = max(1, c["last_step_launches"])
+        kernel_ms = c["last_step_gpu_ms"] / launches
+        ent = {"value": work / secs, "steps": steps, "seconds": secs, "kernel_ms_avg": kernel_ms,
+               "launches": launches}
+        if strategy == "tiled":
+            ent.update({"tile_config": c.get("tile_config_name"), "workgroups": c["workgroups"],
+                        "lds_bytes_per_workgroup": c["lds_bytes_per_workgroup"]})
+            ent["roofline"] = roofline(FLOPS_PER_SEGMENT_SUBSTEP, (work / launches) / (kernel_ms * 1e-3),
+                                       kernel_ms, work / launches, "uniform")
+        log(f"uniform {strategy}: {secs:.3f} s for {steps} steps, {ent['value'] / 1e9:.3f} G seg-substeps/s")
+        out[strategy] = ent
+        b.destroy()
+    if "plain" in out:
+        out["speedup_vs_plain"] = out["tiled"]["value"] / out["plain"]["value"]
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's 1000)")
+    ap.add_argument("--steps", type=int, default=None, help="timed routing steps (default: the whole 6000)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--plain-steps", type=int, default=None, help="PLAIN timed steps (default: --steps)")
-    ap.add_argument("--rods", type=int, default=4096)
+    ap.add_argument("--plain-steps", type=int, default=None, help="PLAIN timed routing steps (default: --steps)")
+    ap.add_argument("--rods", type=int, default=1024, help="routing cables")
+    ap.add_argument("--tile-config", type=int, default=0)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--uniform-steps", type=int, default=1000, help="steps of the uniform leg (0: skip)")
     ap.add_argument("--cpu-stride", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=3000, help="routing steps of the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--tile-config", type=int, default=0)
-    ap.add_argument("--ragged-steps", type=int, default=1000, help="steps of the ragged comparison (0: skip)")
-    ap.add_argument("--ragged-plain-steps", type=int, default=200)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-traffic kernel for "
                          "scaling hardware byte counters)")
@@ -199,104 +269,43 @@ def main():
         log(f"copy calibration: {args.copy_calibration_mib} MiB read + {args.copy_calibration_mib} MiB written")
         del src, dst
 
-    w = scenes.uniform_floor_batch(num_rods=args.rods)
-    K = args.steps if args.steps is not None else w.steps
-    W = args.warmup
-    log(f"workload {w.name}: {w.num_rods} rods, {w.segments_total} segments, K={K}, W={W}")
     info = rp.device_info()
     log("device", info)
+    w = scenes.routing_rollouts(num_rods=args.rods)
+    K = args.steps if args.steps is not None else w.steps
+    Kp = args.plain_steps if args.plain_steps is not None else K
+    W = args.warmup
+    log(f"workload {w.name}: {w.num_rods} cables, {w.segments_total} segments, K={K} (PLAIN {Kp}), W={W}")
 
-    # ---- TILED
-    b = make_batch(w, "tiled")
-    if args.tile_config:
-        b.destroy()
-        b = rp.Batch(w.prototype, w.num_rods, strategy="tiled", world=w.world, tile_config=args.tile_config)
-        b.set_material_scales(w.youngs_scale, w.friction_scale)
-    reset(b, w)
-    b.step(w.params, W)
-    b.synchronize()
-    reset(b, w)
-    b.set_timing(True)
-    secs, c = time_steps(b, w.params, K)
-    x = b.download_positions()
-    if not np.all(np.isfinite(x)):
-        raise RuntimeError("non-finite state after the timed run")
+    leg = routing_leg(w, K, Kp, W, args.tile_config, args.no_plain)
+    t = leg["tiled"]
     work = float(w.segments_total) * w.params.substeps * K
-    rate = work / secs
-    gpu_ms = c["last_step_gpu_ms"]
-    launches = max(1, c["last_step_launches"])
-    kernel_ms = gpu_ms / launches
-    work_per_launch = work / launches
-    kernel_rate = work_per_launch / (kernel_ms * 1e-3)
-    log(f"TILED {c.get('tile_config_name')}: {secs:.3f} s wall, {gpu_ms:.2f} ms device over {launches} launches, "
-        f"{rate / 1e9:.3f} G seg-substeps/s; lanes {c['lanes_busy']}/{c['lanes_total']}, "
-        f"{c['workgroups']} workgroups, LDS {c['lds_bytes_per_workgroup']} B")
-    tiled_info = {"tile_config": c.get("tile_config_name"), "workgroups": c["workgroups"],
-                  "lanes_busy": c["lanes_busy"], "lanes_total": c["lanes_total"],
-                  "lds_bytes_per_workgroup": c["lds_bytes_per_workgroup"], "launches": launches,
-                  "kernel_ms_avg": kernel_ms, "autotune_trials_ms": b.tune_report()}
-    b.destroy()
+    kernel_rate = (work / t["launches"]) / (t["kernel_ms_avg"] * 1e-3)
+    roof = roofline(FLOPS_PER_SEGMENT_SUBSTEP_ROUTING, kernel_rate, t["kernel_ms_avg"], work / t["launches"],
+                    "routing" if not args.tile_config else None)
 
-    # ---- PLAIN, same workload
-    plain = None
-    if not args.no_plain:
-        Kp = args.plain_steps if args.plain_steps is not None else K
-        pb = make_batch(w, "plain")
-        reset(pb, w)
-        pb.step(w.params, min(W, 4))
-        pb.synchronize()
-        reset(pb, w)
-        pb.set_timing(True)
-        psecs, pc = time_steps(pb, w.params, Kp)
-        pwork = float(w.segments_total) * w.params.substeps * Kp
-        prate = pwork / psecs
-        plain = {"value": prate, "steps": Kp, "seconds": psecs, "ms_per_step": psecs * 1e3 / Kp,
-                 "kernel_ms_avg": pc["last_step_gpu_ms"] / max(1, pc["last_step_launches"])}
-        log(f"PLAIN: {psecs:.3f} s for {Kp} steps, {prate / 1e9:.3f} G seg-substeps/s")
-        pb.destroy()
-
-    # ---- roofline of the dominant kernel (k_tiled_step)
-    achieved_tflops = FLOPS_PER_SEGMENT_SUBSTEP * kernel_rate / 1e12
-    roofline = {
-        "bound": "mfma",
-        "bound_detail": "FP32 vector-ALU bound; the path has no dense contraction and issues no MFMA, so "
-                        "the FP32 peak is the compute roofline (unpacked FP32 tops out at half of it)",
-        "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-        "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": None,
-        "kernel": "k_tiled_step", "flops_per_segment_substep": FLOPS_PER_SEGMENT_SUBSTEP,
-        "segment_substeps_per_launch": work_per_launch, "kernel_ms_avg": kernel_ms,
-        "lds": {"bytes_per_segment_substep": LDS_BYTES_PER_SEGMENT_SUBSTEP,
-                "achieved_GBs": LDS_BYTES_PER_SEGMENT_SUBSTEP * kernel_rate / 1e9, "peak_GBs": PEAK_LDS_GBS,
-                "frac": LDS_BYTES_PER_SEGMENT_SUBSTEP * kernel_rate / 1e9 / PEAK_LDS_GBS},
-    }
-    traffic_path = os.path.join(HERE, "profiles", "k_tiled_step_uniform-4096x64-floor_traffic.json")
-    if os.path.exists(traffic_path) and not args.tile_config:
-        with open(traffic_path) as f:
-            tr = json.load(f)
-        roofline["traffic"] = tr.get("hbm_bytes_per_launch")
-        roofline["traffic_source"] = os.path.relpath(traffic_path, HERE)
-
-    ragged = None
-    if args.ragged_steps > 0:
-        ragged = ragged_comparison(args.ragged_steps, args.ragged_plain_steps, W)
+    uniform = uniform_leg(args.uniform_steps, W, args.no_plain) if args.uniform_steps > 0 else None
 
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(w, args.cpu_stride, threads)
+        cpu = cpu_baseline_routing(w, args.cpu_stride, min(args.cpu_steps, w.steps), threads)
         log(f"CPU baseline: {cpu['value'] / 1e6:.2f} M seg-substeps/s ({cpu['sample']})")
 
+    plain = leg.get("plain")
     out = {
-        "metric": METRIC, "value": rate, "unit": "segment-substeps/s", "n_gpus": 1, "steps": K, "warmup": W,
-        "ms_per_step": secs * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "metric": METRIC, "value": t["value"], "unit": "segment-substeps/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": t["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded)",
-        "config": {"workload": w.name, **w.info, "strategy": "tiled", "parallelism": "single GPU"},
-        "tiled": tiled_info,
+        "config": {"workload": w.name, **{k: v for k, v in w.info.items() if k not in ("paths", "task")},
+                   "strategy": "tiled (ragged-packed persistent launches, hipGraph-captured)",
+                   "parallelism": "single GPU"},
+        "tiled": t,
         "plain": plain,
-        "speedup_vs_plain": (rate / plain["value"]) if plain else None,
-        "ragged": ragged,
-        "roofline": roofline,
+        "speedup_vs_plain": (t["value"] / plain["value"]) if plain else None,
+        "roofline": roof,
         "cpu_baseline": cpu,
+        "uniform": uniform,
         "device": {"arch": info["arch"], "compute_units": info["compute_units"]},
     }
     print(json.dumps(out))

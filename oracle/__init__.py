@@ -316,6 +316,27 @@ def step_many(rods: list, params: Params, worlds: list | None, steps: int, threa
     L.orc_step_many(handles, len(rods), C.byref(pc), ptrs, int(steps), int(threads))
 
 
+def step_many_driven(rods: list, params: Params, worlds: list | None, steps: int, threads: int,
+                     driven: list, kin_vel: np.ndarray) -> None:
+    """step_many with a per-step kinematic velocity for one particle of each
+    rod: driven[r] (-1: none) and kin_vel[r, s] (shape rods x steps x 3)."""
+    if not rods:
+        return
+    L = rods[0].L
+    handles = (C.c_void_p * len(rods))(*[r.h.value for r in rods])
+    pc = params.c()
+    if worlds is not None:
+        wcs = [w.c() if w is not None else None for w in worlds]
+        ptrs = (C.POINTER(WorldC) * len(rods))(*[C.pointer(w) if w is not None else None for w in wcs])
+    else:
+        wcs, ptrs = None, None
+    kv = np.ascontiguousarray(kin_vel, dtype=np.float64)
+    assert kv.shape == (len(rods), steps, 3)
+    dr = (C.c_int * len(rods))(*[int(d) for d in driven])
+    L.orc_step_many_driven(handles, len(rods), C.byref(pc), ptrs, int(steps), int(threads), dr,
+                           kv.ctypes.data_as(C.POINTER(C.c_double)))
+
+
 # ------------------------------------------------------------------ pinned pieces
 
 def chol_solve3(A, b, precision=64, kind="oracle"):

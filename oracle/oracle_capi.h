@@ -80,6 +80,11 @@ void orc_contact_counts(void* rod, int* worldContacts, int* selfContacts);
 /* Step many rods on `threads` std::threads, static partition over rods. */
 int orc_step_many(void** rods, int count, const orc_params* p, const orc_world* const* worlds, int steps,
                   int threads);
+/* As orc_step_many, and before step s of rod r the kinematic velocity of its
+ * particle driven[r] (when >= 0) is set to kin_vel[(r * steps + s) * 3 ..+3]:
+ * a gripper schedule evaluated by the caller. */
+int orc_step_many_driven(void** rods, int count, const orc_params* p, const orc_world* const* worlds, int steps,
+                         int threads, const int* driven, const double* kin_vel);
 
 /* Pieces of the arithmetic, for pinning against textbook routines. */
 int orc_chol_solve3(int precision, const double A[9], const double b[3], double y[3]);

@@ -296,6 +296,28 @@ int orc_step_many(void** rods, int count, const orc_params* p, const orc_world* 
     return 0;
 }
 
+int orc_step_many_driven(void** rods, int count, const orc_params* p, const orc_world* const* worlds, int steps,
+                         int threads, const int* driven, const double* kin_vel) {
+    if (threads < 1) threads = 1;
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) {
+        const int begin = int((long long)count * t / threads);
+        const int end = int((long long)count * (t + 1) / threads);
+        pool.emplace_back([=]() {
+            for (int r = begin; r < end; ++r) {
+                for (int s = 0; s < steps; ++s) {
+                    if (driven[r] >= 0)
+                        H(rods[r])->rod.state.kinematicVelocity[driven[r]] =
+                            v3(kin_vel + (size_t(r) * steps + s) * 3);
+                    orc_step(rods[r], p, worlds ? worlds[r] : nullptr, 1);
+                }
+            }
+        });
+    }
+    for (std::thread& th : pool) th.join();
+    return 0;
+}
+
 int orc_chol_solve3(int precision, const double A[9], const double b[3], double y[3]) {
     if (precision != kPrecision) return -1;
     Mat3 M;

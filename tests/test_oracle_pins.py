@@ -18,7 +18,7 @@ import pytest
 
 import oracle
 import ropephysics_amd as rp
-from helpers import REFERENCE_MATERIAL, benchmark_rod, oracle_params, oracle_twin
+from helpers import REFERENCE_MATERIAL, benchmark_rod, oracle_params, oracle_twin, oracle_world
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 MAT = REFERENCE_MATERIAL.as_tuple()
@@ -268,9 +268,12 @@ def test_rest_state_is_a_fixed_point():
     assert np.max(np.abs(r.get()["x"] - x0)) < 1e-13
 
 
-def test_flop_count_frozen():
-    """The per-segment-substep FP32 operation count bench.py uses."""
+def test_flop_counts_frozen():
+    """The per-segment-substep FP32 operation counts bench.py uses: the
+    contact-free step, and the routing cables with their world and
+    self-collision from the initial pose."""
     import bench
+    from ropephysics_amd import scenes
 
     rod = rp.make_straight_rod(64, 0.5, REFERENCE_MATERIAL, (0, 0, 0.1), (1, 0, 0))
     rp.clamp_root_exact(rod)
@@ -278,3 +281,12 @@ def test_flop_count_frozen():
     c = o.count_ops(oracle_params(rp.SolverParams(dt=1e-3, substeps=10), precision=64), None, 10)
     flops = (c["add"] + c["mul"] + c["div"] + c["sqrt"]) / (64 * 10 * 10)
     assert flops == pytest.approx(bench.FLOPS_PER_SEGMENT_SUBSTEP, abs=0.01)
+
+    w = scenes.routing_rollouts()
+    ow = oracle_world(w.world)
+    total = segs = 0
+    for r in range(0, w.num_rods, 64):
+        c = oracle_twin(w.rods[r], 64).count_ops(oracle_params(w.params, precision=64), ow, 10)
+        total += c["add"] + c["mul"] + c["div"] + c["sqrt"]
+        segs += w.rods[r].num_segments
+    assert total / (segs * 10 * 10) == pytest.approx(bench.FLOPS_PER_SEGMENT_SUBSTEP_ROUTING, abs=0.01)
