@@ -142,6 +142,7 @@ def routing_leg(w: scenes.Workload, steps: int, plain_steps: int, warmup: int, t
             routed = [task.evaluate(b.download(i)["x"].astype(np.float64))["routed"] for i in range(w.num_rods)] \
                 if k == w.steps else None
             ent["routed"] = None if routed is None else int(sum(routed))
+            out["_verdicts"] = routed
         log(f"routing {strategy} {ent.get('tile_config', '')}: {secs:.3f} s for {k} steps, "
             f"{ent['value'] / 1e9:.3f} G seg-substeps/s, device {c['last_step_gpu_ms']:.1f} ms over "
             f"{launches} launches")
@@ -183,7 +184,11 @@ def cpu_baseline_routing(w: scenes.Workload, stride: int, steps: int, threads: i
     dt = time.perf_counter() - t0
     segs = sum(w.rods[r].num_segments for r in sel)
     work = segs * p.substeps * steps
-    return {"value": work / dt, "unit": "segment-substeps/s", "cores": threads,
+    verdicts = None
+    if steps == w.steps:
+        task = w.info["task"]
+        verdicts = {int(r): task.evaluate(o.get()["x"])["routed"] for r, o in zip(sel, rods)}
+    return {"value": work / dt, "unit": "segment-substeps/s", "cores": threads, "_verdicts": verdicts,
             "kind": "reference" if kind == "ref64" else "port",
             "sample": f"{len(sel)} of {w.num_rods} cables (every {stride}th, {segs} segments), first {steps} of "
                       f"{w.steps} steps, gripper schedule applied per step, FP64, sequential sweep order, "
@@ -245,7 +250,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--uniform-steps", type=int, default=1000, help="steps of the uniform leg (0: skip)")
     ap.add_argument("--cpu-stride", type=int, default=16)
-    ap.add_argument("--cpu-steps", type=int, default=3000, help="routing steps of the CPU sample")
+    ap.add_argument("--cpu-steps", type=int, default=6000, help="routing steps of the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -278,6 +283,7 @@ def main():
     log(f"workload {w.name}: {w.num_rods} cables, {w.segments_total} segments, K={K} (PLAIN {Kp}), W={W}")
 
     leg = routing_leg(w, K, Kp, W, args.tile_config, args.no_plain)
+    leg.setdefault("_verdicts", None)
     t = leg["tiled"]
     work = float(w.segments_total) * w.params.substeps * K
     kernel_rate = (work / t["launches"]) / (t["kernel_ms_avg"] * 1e-3)
@@ -291,8 +297,16 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline_routing(w, args.cpu_stride, min(args.cpu_steps, w.steps), threads)
         log(f"CPU baseline: {cpu['value'] / 1e6:.2f} M seg-substeps/s ({cpu['sample']})")
+        cv, gv = cpu.pop("_verdicts"), leg.get("_verdicts")
+        if cv is not None and gv is not None:
+            agree = sum(1 for r, v in cv.items() if gv[r] == v)
+            t["verdicts_vs_reference_cpu"] = {"sampled_cables": len(cv), "agree": agree,
+                                              "routed_gpu": sum(1 for r in cv if gv[r]),
+                                              "routed_cpu": sum(1 for v in cv.values() if v)}
+            log(f"routed verdicts, GPU FP32 vs reference CPU FP64 on {len(cv)} cables: {agree} agree")
 
     plain = leg.get("plain")
+    leg.pop("_verdicts", None)
     out = {
         "metric": METRIC, "value": t["value"], "unit": "segment-substeps/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": t["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,

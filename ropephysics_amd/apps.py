@@ -95,7 +95,7 @@ class RoutingTask:
     peg_radius: float = 0.015
     peg_height: float = 0.06
     clip: tuple = (0.85, 0.0, 0.0)
-    clip_half: tuple = (0.012, 0.04, 0.006)   # box half extents
+    clip_half: tuple = (0.012, 0.06, 0.006)   # box half extents
     clip_bottom: float = 0.010                 # bar underside above the board
     board_friction: float = 0.5
     cable_length: float = 1.2
@@ -140,9 +140,20 @@ class RoutingTask:
         return int(round((sum(self.leg_seconds) + self.hold_seconds) / self.params().dt))
 
     def base_waypoints(self) -> np.ndarray:
+        """Under peg A, over peg B, then past the clip to an end point that
+        leaves the cable nearly taut, on the -y side so that the stretch from
+        peg B crosses the clip's line 2-3 cm from its centre, well inside the
+        6 cm half-width: a routed cable is routed by a clear margin."""
         a, b, c = self.peg_a, self.peg_b, self.clip
-        return np.array([(0.10, -0.30), (a[0], a[1] - 0.08), (0.33, -0.03), (0.40, 0.0), (0.48, 0.05),
-                         (b[0], b[1] + 0.08), (c[0] - 0.10, 0.0), (c[0] + 0.18, 0.0)], dtype=np.float64)
+        return np.array([(0.10, -0.30), (a[0], a[1] - 0.06), (0.33, -0.03), (0.40, 0.0), (0.48, 0.05),
+                         (b[0], b[1] + 0.06), (c[0] - 0.10, 0.0), (c[0] + 0.28, -0.03)], dtype=np.float64)
+
+    def wrong_side_waypoints(self) -> np.ndarray:
+        """The base path with its second waypoint above peg A instead of
+        below: the cable ends on the wrong side of A, by centimetres."""
+        w = self.base_waypoints()
+        w[1] = (self.peg_a[0], self.peg_a[1] + 0.06)
+        return w
 
     def path(self, segments: int, waypoints: np.ndarray) -> GripperPath:
         mat = self.material()

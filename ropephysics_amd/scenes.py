@@ -83,7 +83,10 @@ def routing_rollouts(num_rods: int = 1024, seed: int = 20260917, steps: int | No
     integer), 1.2 m, connector pinned, the other end driven along a seeded
     8-waypoint smoothstep (piecewise-cubic) path; floor, two capsule pegs
     (r = 15 mm), one box clip; friction 0.5 x U[0.6, 1.4]; stiffness
-    0.4-4x log-uniform; self-collision on; dt = 1e-3, 10 substeps, 6 s."""
+    0.4-4x log-uniform; self-collision on; dt = 1e-3, 10 substeps, 6 s.
+    Paths: the task's base waypoints, or for a quarter of the cables (drawn
+    from the seed) its wrong-side waypoints (over peg A), each waypoint
+    jittered by N(0, 15 mm)."""
     from .apps import RoutingTask
 
     task = RoutingTask()
@@ -91,7 +94,9 @@ def routing_rollouts(num_rods: int = 1024, seed: int = 20260917, steps: int | No
     n = rng.integers(48, 161, num_rods)
     youngs = 0.4 * 10.0 ** rng.uniform(0.0, 1.0, num_rods)
     fric = rng.uniform(0.6, 1.4, num_rods)
-    wps = task.base_waypoints()[None, :, :] + rng.normal(0.0, 0.015, (num_rods, 8, 2))
+    wrong = rng.uniform(0.0, 1.0, num_rods) < 0.25
+    base = np.where(wrong[:, None, None], task.wrong_side_waypoints()[None], task.base_waypoints()[None])
+    wps = base + rng.normal(0.0, 0.015, (num_rods, 8, 2))
     cache = {}
     rods = []
     for k in n:

@@ -104,30 +104,42 @@ def test_hang_observer_matches_host_sampling(strategy):
 
 
 def test_routing_verdicts_agree_with_fp64_oracle():
-    """Full 6 s routing rollouts for a few cables: the routed / not-routed
-    verdict from GPU positions equals the verdict from the FP64 oracle's."""
-    task, rods, paths = _routing_small(4, 11, segs=(48, 64))
-    # One deliberately bad path (straight to the end point): not routed.
-    paths[3] = task.path(64, np.tile(task.base_waypoints()[-1], (8, 1)))
-    world = task.world()
-    params = task.params()
-    b = rp.Batch(rods, strategy="tiled", world=world)
-    b.set_paths(0, paths)
-    steps = task.steps()
-    b.step(params, steps)
+    """Full 6 s routing rollouts of ten cables of the routing workload (its
+    seeded sizes, paths, stiffness and friction): the routed / not-routed
+    verdict from the GPU's FP32 positions equals the verdict from the FP64
+    oracle's, and the sample holds both outcomes. One more cable is sent
+    straight to the end point (not routed)."""
+    from ropephysics_amd import scenes
+
+    w = scenes.routing_rollouts()
+    task, paths = w.info["task"], w.info["paths"]
+    sel = list(range(3, w.num_rods, 103))
+    rods = [w.rods[r] for r in sel] + [w.rods[sel[0]]]
+    cable_paths = [paths[r] for r in sel] + [task.path(w.rods[sel[0]].num_segments,
+                                                       np.tile(task.base_waypoints()[-1], (8, 1)))]
+    ys = np.array([w.youngs_scale[r] for r in sel] + [1.0])
+    fs = np.array([w.friction_scale[r] for r in sel] + [1.0])
+    b = rp.Batch(rods, strategy="tiled", world=w.world)
+    b.set_material_scales(ys, fs)
+    b.set_paths(0, cable_paths)
+    b.step(w.params, w.steps)
     b.synchronize()
-    ow = oracle_world(world)
-    verdicts = []
-    for i, r in enumerate(rods):
-        o = oracle_twin(r, 64)
-        op = oracle_params(params, precision=64)
-        for s in range(steps):
-            a = o.get()
-            a["kinVel"][paths[i].particle] = paths[i].velocity(s)
-            o.set({"kinVel": a["kinVel"]})
-            o.step(op, ow, 1)
-        g = task.evaluate(b.download(i)["x"].astype(np.float64))
-        c = task.evaluate(o.get()["x"])
-        assert g["routed"] == c["routed"], (i, g, c)
-        verdicts.append(g["routed"])
-    assert not verdicts[3]
+    twins = [oracle_twin(r, 64) for r in rods]
+    worlds = []
+    for i in range(len(rods)):
+        ow = oracle_world(w.world)
+        for pr in ow.prims:
+            pr.friction = pr.friction * fs[i]
+        ow.self_friction = ow.self_friction * fs[i]
+        worlds.append(ow)
+        a = twins[i].get()
+        twins[i].set({"sComp": a["sComp"] / ys[i], "bComp": a["bComp"] / ys[i]})
+    kv = np.array([[p.velocity(s) for s in range(w.steps)] for p in cable_paths])
+    oracle.step_many_driven(twins, oracle_params(w.params, precision=64), worlds, w.steps, 8,
+                            [p.particle for p in cable_paths], kv)
+    gpu = [task.evaluate(b.download(i)["x"].astype(np.float64)) for i in range(len(rods))]
+    cpu = [task.evaluate(o.get()["x"]) for o in twins]
+    assert [g["routed"] for g in gpu] == [c["routed"] for c in cpu], (gpu, cpu)
+    verdicts = [g["routed"] for g in gpu]
+    assert any(verdicts[:-1]) and not all(verdicts[:-1])
+    assert not verdicts[-1]
