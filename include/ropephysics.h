@@ -255,6 +255,98 @@ int rp_tile_config_count(void);
 /* Thread-local message for the last failed call on this thread. */
 const char* rp_last_error(void);
 
+/* ------------------------------------------------------------------ serving
+ *
+ * A continuous-batching server. No counterpart in the reference: its callers
+ * create one Batch per workload. Requests (each a set of rods with its own
+ * collision world, described exactly as for rp_batch_create, plus an initial
+ * state and a horizon in steps) are admitted while the device memory reserved
+ * for them (an HBM budget fixed at creation) has room, stay resident between
+ * calls, and advance together: each rp_server_step is one TILED launch of
+ * steps_per_launch steps over the rods of every resident unfinished request,
+ * every rod stopping at its own request's horizon. One step definition
+ * (rp_step_params) serves every request. A request's result is bitwise the
+ * one it gets alone: the per-rod arithmetic does not depend on its neighbours.
+ *
+ * Use: rp_server_sync; read and release what rp_server_finished lists; submit
+ * what has arrived; rp_server_step; repeat. Not thread-safe. */
+
+typedef struct rp_server_desc {
+    long long hbm_budget_bytes;  /* device memory for resident requests' state, constants and records */
+    int max_rods;                /* rod records (0: one per 16 segments of the budget) */
+    rp_step_params params;       /* the step every request takes */
+    int tile_config;             /* 0: tp2-tc1-w4 */
+    int steps_per_launch;        /* steps per rp_server_step (0: 64) */
+} rp_server_desc;
+
+/* A request's initial state and loads, packed rod after rod like the
+ * request's CSR; only x and q are required (the rest default to zero). The
+ * scales are per rod and may be NULL (1). */
+typedef struct rp_request_state {
+    const float* x;
+    const float* v;
+    const float* q;
+    const float* omega;
+    const float* force;
+    const float* kin_vel;
+    const float* torque;
+    const float* youngs_scale;
+    const float* friction_scale;
+} rp_request_state;
+
+typedef struct rp_request_info {
+    int num_rods, num_particles, num_frames;
+    int steps;       /* horizon */
+    int steps_done;  /* steps issued so far (all of them once finished) */
+    long long device_bytes;
+} rp_request_info;
+
+typedef struct rp_server_stats {
+    int resident_requests;  /* admitted and not released */
+    int running_requests;   /* of those, not yet at their horizon */
+    int running_rods;
+    long long bytes_in_use;  /* device bytes held by resident requests */
+    long long device_bytes;  /* allocated at creation */
+    long long capacity_particles, used_particles;
+    long long capacity_rods, used_rods;
+    long long launches;
+    int workgroups;  /* of the last packing */
+    int tile_config;
+    int steps_per_launch;
+} rp_server_stats;
+
+int rp_server_create(const rp_server_desc* desc, void** out_server);
+void rp_server_destroy(void* server);
+
+/* Admit a request of `steps` steps. RP_ERR_CAPACITY: no room now (release
+ * finished requests and retry) or never (larger than the budget, or a rod no
+ * tile shape takes; rp_last_error says which). */
+int rp_server_submit(void* server, const rp_batch_desc* request, const rp_request_state* state, int steps,
+                     int* out_id);
+
+/* Launch one chunk of steps for every running request (asynchronous); sets
+ * *num_finished to the number of requests that reach their horizon in it. */
+int rp_server_step(void* server, int* num_finished);
+
+/* Wait for every launched chunk. */
+int rp_server_sync(void* server);
+
+/* Ids of finished, unreleased requests (up to capacity); returns how many
+ * there are. */
+int rp_server_finished(void* server, int* ids, int capacity);
+
+int rp_server_request_info(void* server, int id, rp_request_info* out);
+
+/* Copy a finished request's state out (any pointer may be NULL); waits for
+ * its last chunk. Sizes as its CSR: x, v particles x 3; q frames x 4 (w, x,
+ * y, z); omega frames x 3. */
+int rp_server_result(void* server, int id, float* x, float* q, float* v, float* omega);
+
+/* Free a request's device ranges (finished or not). */
+int rp_server_release(void* server, int id);
+
+int rp_server_get_stats(void* server, rp_server_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

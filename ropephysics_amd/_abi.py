@@ -70,6 +70,29 @@ class RpCounters(C.Structure):
                 ("last_step_launches", C.c_int)]
 
 
+class RpServerDesc(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_longlong), ("max_rods", C.c_int), ("params", RpStepParams),
+                ("tile_config", C.c_int), ("steps_per_launch", C.c_int)]
+
+
+class RpRequestState(C.Structure):
+    _fields_ = [(n, _fp) for n in ("x", "v", "q", "omega", "force", "kin_vel", "torque", "youngs_scale",
+                                   "friction_scale")]
+
+
+class RpRequestInfo(C.Structure):
+    _fields_ = [("num_rods", C.c_int), ("num_particles", C.c_int), ("num_frames", C.c_int), ("steps", C.c_int),
+                ("steps_done", C.c_int), ("device_bytes", C.c_longlong)]
+
+
+class RpServerStats(C.Structure):
+    _fields_ = [("resident_requests", C.c_int), ("running_requests", C.c_int), ("running_rods", C.c_int),
+                ("bytes_in_use", C.c_longlong), ("device_bytes", C.c_longlong),
+                ("capacity_particles", C.c_longlong), ("used_particles", C.c_longlong),
+                ("capacity_rods", C.c_longlong), ("used_rods", C.c_longlong), ("launches", C.c_longlong),
+                ("workgroups", C.c_int), ("tile_config", C.c_int), ("steps_per_launch", C.c_int)]
+
+
 # Every symbol include/ropephysics.h declares, with its signature.
 SIGNATURES = {
     "rp_device_query": (C.c_int, [C.POINTER(RpDeviceInfo)]),
@@ -92,6 +115,16 @@ SIGNATURES = {
     "rp_tile_config_name": (C.c_char_p, [C.c_int]),
     "rp_tile_config_count": (C.c_int, []),
     "rp_last_error": (C.c_char_p, []),
+    "rp_server_create": (C.c_int, [C.POINTER(RpServerDesc), C.POINTER(C.c_void_p)]),
+    "rp_server_destroy": (None, [C.c_void_p]),
+    "rp_server_submit": (C.c_int, [C.c_void_p, C.POINTER(RpBatchDesc), C.POINTER(RpRequestState), C.c_int, _ip]),
+    "rp_server_step": (C.c_int, [C.c_void_p, _ip]),
+    "rp_server_sync": (C.c_int, [C.c_void_p]),
+    "rp_server_finished": (C.c_int, [C.c_void_p, _ip, C.c_int]),
+    "rp_server_request_info": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(RpRequestInfo)]),
+    "rp_server_result": (C.c_int, [C.c_void_p, C.c_int, _fp, _fp, _fp, _fp]),
+    "rp_server_release": (C.c_int, [C.c_void_p, C.c_int]),
+    "rp_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(RpServerStats)]),
 }
 
 _LIB = None

@@ -44,6 +44,68 @@ def _world_desc(w: CollisionWorld):
     return d, prims
 
 
+def describe(rods: list, world=None, world_of_rod=None):
+    """An ``rp_batch_desc`` for ``rods`` (ragged CSR over the list) and their
+    world(s): one ``CollisionWorld`` for every rod, or a list chosen per rod by
+    ``world_of_rod``. Returns the descriptor, the arrays it points into (keep
+    them alive while it is used) and the four CSR start arrays."""
+    R = len(rods)
+    if R == 0:
+        raise ValueError("a batch needs at least one rod")
+    nP = np.array([r.num_particles for r in rods])
+    nF = np.array([r.num_frames for r in rods])
+    nS = np.array([r.num_segments for r in rods])
+    nB = np.array([len(r.bA) for r in rods])
+    starts = tuple(_i32(np.concatenate([[0], np.cumsum(n)])) for n in (nP, nF, nS, nB))
+    same = len({id(r) for r in rods}) == 1
+    proto = rods[0]
+
+    def cat(name):
+        if same:
+            a = getattr(proto, name)
+            return np.tile(a, (R, 1)) if a.ndim == 2 else np.tile(a, R)
+        return np.concatenate([getattr(r, name) for r in rods])
+
+    keep = [starts]
+    arr = {
+        "inv_mass": _f32(cat("invMass")), "inv_inertia": _f32(cat("invInertia")), "inertia": _f32(cat("inertia")),
+        "stretch_p0": _i32(cat("sP0")), "stretch_p1": _i32(cat("sP1")), "stretch_seg": _i32(cat("sSeg")),
+        "stretch_rest": _f32(cat("sRest")), "stretch_compliance": _f32(cat("sComp")),
+        "bend_a": _i32(cat("bA")), "bend_b": _i32(cat("bB")), "bend_rest": _f32(cat("bRest")),
+        "bend_compliance": _f32(cat("bComp")), "bend_rest_darboux": _f32(cat("bRestD")),
+        "radius": _f32([r.material.radius for r in rods]),
+    }
+    keep.append(arr)
+    if world is None:
+        worlds = []
+    elif isinstance(world, CollisionWorld):
+        worlds = [world]
+    else:
+        worlds = list(world)
+    wdescs = (_abi.RpWorldDesc * max(1, len(worlds)))()
+    for i, w in enumerate(worlds):
+        d, prims = _world_desc(w)
+        keep.append(prims)
+        wdescs[i] = d
+    keep.append(wdescs)
+    wor = None
+    if world_of_rod is not None:
+        wor = _i32(world_of_rod)
+        keep.append(wor)
+    desc = _abi.RpBatchDesc()
+    desc.num_rods = R
+    desc.particle_start = _abi.iptr(starts[0])
+    desc.frame_start = _abi.iptr(starts[1])
+    desc.stretch_start = _abi.iptr(starts[2])
+    desc.bend_start = _abi.iptr(starts[3])
+    for k, a in arr.items():
+        setattr(desc, k, _abi.fptr(a) if a.dtype == np.float32 else _abi.iptr(a))
+    desc.num_worlds = len(worlds)
+    desc.worlds = C.cast(wdescs, C.POINTER(_abi.RpWorldDesc))
+    desc.world_of_rod = _abi.iptr(wor)
+    return desc, keep, starts
+
+
 def step_params(p: SolverParams) -> _abi.RpStepParams:
     g = (C.c_float * 3)(*map(float, p.gravity))
     return _abi.RpStepParams(float(p.dt), int(p.substeps), int(p.iterations), g, float(p.linear_damping),
@@ -68,61 +130,9 @@ class Batch:
         R = len(self.rods)
         if R == 0:
             raise ValueError("a batch needs at least one rod")
-        nP = np.array([r.num_particles for r in self.rods])
-        nF = np.array([r.num_frames for r in self.rods])
-        nS = np.array([r.num_segments for r in self.rods])
-        nB = np.array([len(r.bA) for r in self.rods])
-        self.particle_start = _i32(np.concatenate([[0], np.cumsum(nP)]))
-        self.frame_start = _i32(np.concatenate([[0], np.cumsum(nF)]))
-        self.stretch_start = _i32(np.concatenate([[0], np.cumsum(nS)]))
-        self.bend_start = _i32(np.concatenate([[0], np.cumsum(nB)]))
-        uniq = {id(r): r for r in self.rods}
-        same = len(uniq) == 1
-        proto = self.rods[0]
-
-        def cat(name, comps=None):
-            if same:
-                a = getattr(proto, name)
-                return np.tile(a, (R, 1)) if a.ndim == 2 else np.tile(a, R)
-            return np.concatenate([getattr(r, name) for r in self.rods])
-
-        self._keep = []
-        arr = {
-            "inv_mass": _f32(cat("invMass")), "inv_inertia": _f32(cat("invInertia")), "inertia": _f32(cat("inertia")),
-            "stretch_p0": _i32(cat("sP0")), "stretch_p1": _i32(cat("sP1")), "stretch_seg": _i32(cat("sSeg")),
-            "stretch_rest": _f32(cat("sRest")), "stretch_compliance": _f32(cat("sComp")),
-            "bend_a": _i32(cat("bA")), "bend_b": _i32(cat("bB")), "bend_rest": _f32(cat("bRest")),
-            "bend_compliance": _f32(cat("bComp")), "bend_rest_darboux": _f32(cat("bRestD")),
-            "radius": _f32([r.material.radius for r in self.rods]),
-        }
-        self._keep.append(arr)
-        if world is None:
-            worlds = []
-        elif isinstance(world, CollisionWorld):
-            worlds = [world]
-        else:
-            worlds = list(world)
-        wdescs = (_abi.RpWorldDesc * max(1, len(worlds)))()
-        for i, w in enumerate(worlds):
-            d, prims = _world_desc(w)
-            self._keep.append(prims)
-            wdescs[i] = d
-        self._keep.append(wdescs)
-        wor = None
-        if world_of_rod is not None:
-            wor = _i32(world_of_rod)
-            self._keep.append(wor)
-        desc = _abi.RpBatchDesc()
-        desc.num_rods = R
-        desc.particle_start = _abi.iptr(self.particle_start)
-        desc.frame_start = _abi.iptr(self.frame_start)
-        desc.stretch_start = _abi.iptr(self.stretch_start)
-        desc.bend_start = _abi.iptr(self.bend_start)
-        for k, a in arr.items():
-            setattr(desc, k, _abi.fptr(a) if a.dtype == np.float32 else _abi.iptr(a))
-        desc.num_worlds = len(worlds)
-        desc.worlds = C.cast(wdescs, C.POINTER(_abi.RpWorldDesc))
-        desc.world_of_rod = _abi.iptr(wor)
+        desc, keep, starts = describe(self.rods, world, world_of_rod)
+        self._keep = keep
+        self.particle_start, self.frame_start, self.stretch_start, self.bend_start = starts
         desc.strategy = _abi.STRATEGY[strategy]
         desc.tile_config = int(tile_config)
         h = C.c_void_p()

@@ -22,51 +22,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/ropephysics.h"
-#include "rp_layout.h"
-
-namespace rp {
-int plain_lds_bytes(int nP, int nF, int nS, int nB, int nPrim, int self, int table, int cap);
-int tiled_lds_bytes(int nP, int nF, int nS, int nPrim, int self, int table, int cap);
-hipError_t launch_plain(const DeviceArrays& d, const StepArgs& sa, size_t lds, hipStream_t s);
-hipError_t allow_dynamic_lds(int bytes, int* granted);
-hipError_t launch_tiled(int tp, int tc, bool self, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
-                        const TileSlot* slots, int numTiles, int threads, size_t lds, hipStream_t s);
-hipError_t launch_fixed_frames(float* q, const float* src, const float* invInertia4, int f0, int count,
-                               hipStream_t s);
-}  // namespace rp
+#include "rp_host.h"
 
 namespace {
 
 using namespace rp;
-
-thread_local std::string g_error;
-
-int fail(int code, const std::string& msg) {
-    g_error = msg;
-    return code;
-}
-
-#define RP_HIP(call)                                                                                     \
-    do {                                                                                                 \
-        const hipError_t e_ = (call);                                                                    \
-        if (e_ != hipSuccess) return fail(RP_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-// ------------------------------------------------------------------ tile configurations
-
-struct TileConfig {
-    int tp, tc, waves;
-    const char* name;
-};
-
-// id = index + 1. Every (tp, tc) here is instantiated in rp_kernels.hip.
-const TileConfig kConfigs[] = {
-    {2, 1, 1, "tp2-tc1-w1"}, {3, 1, 1, "tp3-tc1-w1"}, {3, 2, 1, "tp3-tc2-w1"}, {5, 2, 1, "tp5-tc2-w1"},
-    {2, 1, 2, "tp2-tc1-w2"}, {3, 1, 2, "tp3-tc1-w2"}, {3, 2, 2, "tp3-tc2-w2"}, {5, 2, 2, "tp5-tc2-w2"},
-    {2, 1, 4, "tp2-tc1-w4"}, {3, 1, 4, "tp3-tc1-w4"}, {1, 1, 2, "tp1-tc1-w2"}, {1, 1, 4, "tp1-tc1-w4"},
-};
-constexpr int kNumConfigs = int(sizeof(kConfigs) / sizeof(kConfigs[0]));
+using namespace rp::host;
 
 // ------------------------------------------------------------------ device buffers
 
@@ -81,12 +42,7 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
 };
 
-struct PackedTiles {
-    int config = 0;  // id
-    std::vector<Tile> tiles;
-    std::vector<TileSlot> slots;
-    int maxLds = 0;
-    int lanesBusy = 0;
+struct PackedTiles : Packing {
     DevBuf dTiles, dSlots;
 };
 
@@ -175,134 +131,21 @@ struct rp_batch_impl {
 
 namespace {
 
-// Greedy colouring, lowest free colour in constraint index order: the
-// reference's colorConstraints (src/core/coloring.cpp:11-57). `res` lists the
-// state ids each constraint writes.
-int greedy_colours(int count, const std::vector<std::array<int, 3>>& res, int width, int numIds,
-                   std::vector<int>& order, std::vector<int>& start) {
-    std::vector<std::vector<char>> used(numIds);
-    std::vector<int> colOf(count);
-    int nCol = 0;
-    for (int k = 0; k < count; ++k) {
-        int c = 0;
-        for (;; ++c) {
-            bool clash = false;
-            for (int m = 0; m < width; ++m) {
-                const std::vector<char>& u = used[res[k][m]];
-                if (c < int(u.size()) && u[c]) {
-                    clash = true;
-                    break;
-                }
-            }
-            if (!clash) break;
-        }
-        colOf[k] = c;
-        nCol = std::max(nCol, c + 1);
-        for (int m = 0; m < width; ++m) {
-            std::vector<char>& u = used[res[k][m]];
-            if (int(u.size()) <= c) u.resize(c + 1, 0);
-            u[c] = 1;
-        }
-    }
-    start.assign(nCol + 1, 0);
-    for (int k = 0; k < count; ++k) ++start[colOf[k] + 1];
-    for (int c = 0; c < nCol; ++c) start[c + 1] += start[c];
-    order.resize(count);
-    std::vector<int> cur(start.begin(), start.end() - 1);
-    for (int k = 0; k < count; ++k) order[cur[colOf[k]]++] = k;
-    return nCol;
-}
-
-// Lanes a rod needs under (tp, tc): every particle and frame within tp trips,
-// every colour within tc trips.
-int group_lanes(const RodInfo& r, const std::vector<int>& colStart, int tp, int tc) {
-    int g = std::max((r.nP + tp - 1) / tp, (r.nF + tp - 1) / tp);
-    const int* cs = colStart.data() + r.colOff;
-    for (int c = 0; c < r.nSCol; ++c) g = std::max(g, (cs[c + 1] - cs[c] + tc - 1) / tc);
-    const int* cb = cs + r.nSCol + 1;
-    for (int c = 0; c < r.nBCol; ++c) g = std::max(g, (cb[c + 1] - cb[c] + tc - 1) / tc);
-    return std::max(g, 1);
-}
-
-int rod_tiled_lds(const RodInfo& r) {
-    return tiled_lds_bytes(r.nP, r.nF, r.nS, r.nPrim, r.self, r.table, r.selfCap);
-}
-
 }  // namespace
 
 namespace {
 
-// Dynamic LDS budget of one workgroup when `perCU` workgroups are to share a
-// compute unit's LDS: a 512-byte allocation granule is left for the static
-// part.
-int lds_budget(const rp_batch_impl& b, int perCU) {
-    return std::min(int(b.ldsLimit), (int(b.cuLds / size_t(perCU)) & ~511) - 512);
-}
+int lds_budget(const rp_batch_impl& b, int perCU) { return lds_budget(LdsLimits{b.ldsLimit, b.cuLds}, perCU); }
 
-// Pack rods into workgroups of `waves` wavefronts and at most `ldsCap` bytes
-// of LDS: best fit over open workgroups, rods taken widest first; a workgroup
-// closes when neither lanes nor LDS are left for the narrowest remaining rod.
+// All rods of the batch packed at one LDS budget; rods that cannot run TILED
+// make it fail.
 int pack_tiles(rp_batch_impl& b, int configId, int ldsCap, const std::vector<int>& colStart, PackedTiles& out) {
-    const TileConfig& cfg = kConfigs[configId - 1];
-    const int laneCap = 64 * cfg.waves;
-    std::vector<int> lanes(b.numRods), lds(b.numRods), order(b.numRods);
+    std::vector<int> ids(b.numRods);
     for (int r = 0; r < b.numRods; ++r) {
         if (!b.tileable[r]) return RP_ERR_CAPACITY;
-        lanes[r] = group_lanes(b.rods[r], colStart, cfg.tp, cfg.tc);
-        lds[r] = rod_tiled_lds(b.rods[r]);
-        if (lanes[r] > laneCap || lds[r] > ldsCap) return RP_ERR_CAPACITY;
-        order[r] = r;
+        ids[r] = r;
     }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return lanes[a] > lanes[c]; });
-    struct Open {
-        int lanesUsed, ldsUsed;
-        std::vector<int> rods;
-    };
-    std::vector<Open> bins;
-    // Open bins keyed by free lanes, for best fit.
-    std::multimap<int, int> byFree;
-    for (int r : order) {
-        int chosen = -1;
-        for (auto it = byFree.lower_bound(lanes[r]); it != byFree.end(); ++it) {
-            Open& o = bins[it->second];
-            if (o.ldsUsed + lds[r] <= ldsCap) {
-                chosen = it->second;
-                byFree.erase(it);
-                break;
-            }
-        }
-        if (chosen < 0) {
-            bins.push_back(Open{0, 0, {}});
-            chosen = int(bins.size()) - 1;
-        }
-        Open& o = bins[chosen];
-        o.rods.push_back(r);
-        o.lanesUsed += lanes[r];
-        o.ldsUsed += lds[r];
-        if (laneCap - o.lanesUsed > 0) byFree.emplace(laneCap - o.lanesUsed, chosen);
-    }
-    out.config = configId;
-    out.tiles.clear();
-    out.slots.clear();
-    out.maxLds = 0;
-    out.lanesBusy = 0;
-    for (const Open& o : bins) {
-        Tile t;
-        t.firstSlot = int(out.slots.size());
-        t.numSlots = int(o.rods.size());
-        int lane = 0, off = 0;
-        for (int r : o.rods) {
-            out.slots.push_back(TileSlot{r, lane, lanes[r], off});
-            lane += lanes[r];
-            off += lds[r];
-        }
-        t.ldsBytes = off;
-        t.pad = 0;
-        out.maxLds = std::max(out.maxLds, off);
-        out.lanesBusy += lane;
-        out.tiles.push_back(t);
-    }
-    return RP_OK;
+    return pack_rods(b.rods, ids, colStart, configId, ldsCap, out);
 }
 
 int upload_tiles(rp_batch_impl& b, PackedTiles& t) {
@@ -321,29 +164,13 @@ int upload_tiles(rp_batch_impl& b, PackedTiles& t) {
     return rc;
 }
 
-StepArgs step_args(const rp_step_params& p, int steps) {
-    StepArgs a;
-    a.h = p.dt / float(p.substeps);
-    a.invH2 = 1.0f / (a.h * a.h);
-    a.gx = p.gravity[0];
-    a.gy = p.gravity[1];
-    a.gz = p.gravity[2];
-    a.linDecay = std::exp(-p.linear_damping * a.h);
-    a.angDecay = std::exp(-p.angular_damping * a.h);
-    a.substeps = p.substeps;
-    a.iterations = p.iterations;
-    a.contactInterval = p.contact_interval > 0 ? p.contact_interval : 1;
-    a.steps = steps;
-    return a;
-}
-
 int launch_steps(rp_batch_impl& b, const rp_step_params& p, int steps, int strategy, PackedTiles* t) {
     if (strategy == RP_STRATEGY_PLAIN) {
         const StepArgs a = step_args(p, 1);
         for (int s = 0; s < steps; ++s) RP_HIP(launch_plain(b.d, a, size_t(b.plainLds), b.stream));
         return RP_OK;
     }
-    const TileConfig& cfg = kConfigs[t->config - 1];
+    const TileConfig& cfg = tile_config(t->config);
     int left = steps;
     while (left > 0) {
         const int n = std::min(left, b.stepsPerLaunch);
@@ -410,7 +237,7 @@ int autotune(rp_batch_impl& b, const rp_step_params& p, int onlyConfig) {
     RP_HIP(hipEventCreate(&e1));
     const int trialSteps = 8;
     b.tuneLog.clear();
-    for (int id = onlyConfig > 0 ? onlyConfig : 1; id <= (onlyConfig > 0 ? onlyConfig : kNumConfigs); ++id)
+    for (int id = onlyConfig > 0 ? onlyConfig : 1; id <= (onlyConfig > 0 ? onlyConfig : num_tile_configs()); ++id)
       for (int perCU = 1, lastTiles = -1; perCU <= 4; ++perCU) {
         auto t = std::make_unique<PackedTiles>();
         if (pack_tiles(b, id, lds_budget(b, perCU), b.colStart, *t) != RP_OK) continue;
@@ -454,7 +281,7 @@ int resolve_tiles(rp_batch_impl& b, const rp_step_params& p) {
     if (b.requestedConfig > 0) {
         PackedTiles probe;
         const int rc = pack_tiles(b, b.requestedConfig, lds_budget(b, 1), b.colStart, probe);
-        if (rc) return fail(rc, std::string("tile configuration ") + kConfigs[b.requestedConfig - 1].name +
+        if (rc) return fail(rc, std::string("tile configuration ") + tile_config(b.requestedConfig).name +
                                     " does not fit this batch");
         return autotune(b, p, b.requestedConfig);
     }
@@ -472,13 +299,13 @@ int check_range(const rp_batch_impl& b, int first, int count) {
 
 extern "C" {
 
-const char* rp_last_error(void) { return g_error.c_str(); }
+const char* rp_last_error(void) { return last_error(); }
 
-int rp_tile_config_count(void) { return kNumConfigs; }
+int rp_tile_config_count(void) { return num_tile_configs(); }
 
 const char* rp_tile_config_name(int id) {
-    if (id < 1 || id > kNumConfigs) return nullptr;
-    return kConfigs[id - 1].name;
+    if (id < 1 || id > num_tile_configs()) return nullptr;
+    return tile_config(id).name;
 }
 
 int rp_device_query(rp_device_info* out) {
@@ -505,7 +332,8 @@ int rp_device_query(rp_device_info* out) {
     for (int n2 = 4; n2 <= 4096; ++n2) {
         const int lds = tiled_lds_bytes(n2 + 1, n2 + 1, n2, 0, 0, 1, 0);
         if (size_t(lds) > prop.sharedMemPerBlock) break;
-        for (const TileConfig& c : kConfigs) {
+        for (int id = 1; id <= num_tile_configs(); ++id) {
+            const TileConfig& c = tile_config(id);
             const int lanes = std::max((n2 + 1 + c.tp - 1) / c.tp, (n2 / 2 + 1 + c.tc - 1) / c.tc);
             if (lanes <= 64 * c.waves) best = n2;
         }
@@ -517,187 +345,34 @@ int rp_device_query(rp_device_info* out) {
 int rp_batch_create(const rp_batch_desc* desc, void** out_batch) {
     if (!desc || !out_batch) return fail(RP_ERR_INVALID, "null argument");
     *out_batch = nullptr;
-    const int R = desc->num_rods;
-    if (R <= 0) return fail(RP_ERR_INVALID, "num_rods must be positive");
-    if (!desc->particle_start || !desc->frame_start || !desc->stretch_start || !desc->bend_start)
-        return fail(RP_ERR_INVALID, "missing CSR start arrays");
     if (desc->strategy < RP_STRATEGY_PLAIN || desc->strategy > RP_STRATEGY_AUTO)
         return fail(RP_ERR_INVALID, "unknown strategy");
-    if (desc->tile_config < 0 || desc->tile_config > kNumConfigs)
+    if (desc->tile_config < 0 || desc->tile_config > num_tile_configs())
         return fail(RP_ERR_INVALID, "unknown tile configuration");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(RP_ERR_DEVICE, "no usable HIP device");
 
+    Records rec;
+    int rc = build_records(desc, rec);
+    if (rc) return rc;
+    const int R = rec.R, P = rec.P, F = rec.F;
     auto b = std::make_unique<rp_batch_impl>();
     b->numRods = R;
     b->requestedConfig = desc->tile_config;
-    b->pStart.assign(desc->particle_start, desc->particle_start + R + 1);
-    b->fStart.assign(desc->frame_start, desc->frame_start + R + 1);
-    b->sStart.assign(desc->stretch_start, desc->stretch_start + R + 1);
-    b->bStart.assign(desc->bend_start, desc->bend_start + R + 1);
-    for (const std::vector<int>* v : {&b->pStart, &b->fStart, &b->sStart, &b->bStart}) {
-        if ((*v)[0] != 0) return fail(RP_ERR_INVALID, "CSR start arrays must begin at 0");
-        for (int r = 0; r < R; ++r)
-            if ((*v)[r + 1] < (*v)[r]) return fail(RP_ERR_INVALID, "CSR start arrays must be non-decreasing");
-    }
-    const int P = b->pStart[R], F = b->fStart[R], S = b->sStart[R], B = b->bStart[R];
-    if (P > 0 && (!desc->inv_mass)) return fail(RP_ERR_INVALID, "missing inv_mass");
-    if (F > 0 && (!desc->inv_inertia || !desc->inertia)) return fail(RP_ERR_INVALID, "missing inertia");
-    if (S > 0 && (!desc->stretch_p0 || !desc->stretch_p1 || !desc->stretch_seg || !desc->stretch_rest ||
-                  !desc->stretch_compliance))
-        return fail(RP_ERR_INVALID, "missing stretch constraint arrays");
-    if (B > 0 && (!desc->bend_a || !desc->bend_b || !desc->bend_rest || !desc->bend_compliance ||
-                  !desc->bend_rest_darboux))
-        return fail(RP_ERR_INVALID, "missing bend constraint arrays");
-    if (!desc->radius) return fail(RP_ERR_INVALID, "missing radius");
-    if (desc->num_worlds < 0 || (desc->num_worlds > 0 && !desc->worlds))
-        return fail(RP_ERR_INVALID, "bad world table");
-    for (int w = 0; w < desc->num_worlds; ++w) {
-        const rp_world_desc& wd = desc->worlds[w];
-        if (wd.num_prims < 0 || wd.num_prims > kMaxPrims || (wd.num_prims > 0 && !wd.prims))
-            return fail(RP_ERR_INVALID, "world " + std::to_string(w) + ": 0.." + std::to_string(kMaxPrims) +
-                                            " primitives supported");
-        for (int k = 0; k < wd.num_prims; ++k)
-            if (wd.prims[k].type < RP_PLANE || wd.prims[k].type > RP_BOX)
-                return fail(RP_ERR_INVALID, "world " + std::to_string(w) + ": unknown primitive type");
-        if (wd.self_collision && wd.hash_table_size < 1)
-            return fail(RP_ERR_INVALID, "world " + std::to_string(w) + ": hash_table_size must be positive");
-    }
+    b->pStart = rec.pStart;
+    b->fStart = rec.fStart;
+    b->sStart = rec.sStart;
+    b->bStart = rec.bStart;
+    b->rods = rec.rods;
+    b->colStart = rec.colStart;
+    b->tileable = rec.colourFit;
+    b->anySelf = rec.anySelf;
+    b->plainLds = rec.maxPlainLds;
 
-    // ---- per-rod colouring, constraint reordering and rod info
-    std::vector<int4> sIdx(S), bIdx(B);
-    std::vector<float4> sPar(S), bPar(B), bRest4(B);
-    std::vector<int> segEnd(2 * size_t(S));
-    std::vector<int>& colStart = b->colStart;
-    colStart.clear();
-    b->rods.resize(R);
-    b->tileable.assign(R, 1);
-    std::vector<Prim> prims;
-    std::vector<int> primOffOfWorld(std::max(desc->num_worlds, 1), 0);
-    for (int w = 0; w < desc->num_worlds; ++w) {
-        primOffOfWorld[w] = int(prims.size());
-        for (int k = 0; k < desc->worlds[w].num_prims; ++k) {
-            const rp_primitive& s = desc->worlds[w].prims[k];
-            Prim p;
-            p.type = s.type;
-            for (int c = 0; c < 3; ++c) {
-                p.a[c] = s.a[c];
-                p.b[c] = s.b[c];
-                p.normal[c] = s.normal[c];
-                p.half[c] = s.half_extents[c];
-            }
-            for (int c = 0; c < 4; ++c) p.rot[c] = s.rotation[c];
-            p.radius = s.radius;
-            p.friction = s.friction;
-            prims.push_back(p);
-        }
-    }
-    std::vector<int> order, start;
-    std::vector<std::array<int, 3>> res;
-    int maxPlainLds = 0;
-    for (int r = 0; r < R; ++r) {
-        RodInfo& ri = b->rods[r];
-        std::memset(&ri, 0, sizeof(ri));
-        ri.pStart = b->pStart[r];
-        ri.nP = b->pStart[r + 1] - b->pStart[r];
-        ri.fStart = b->fStart[r];
-        ri.nF = b->fStart[r + 1] - b->fStart[r];
-        ri.sStart = b->sStart[r];
-        ri.nS = b->sStart[r + 1] - b->sStart[r];
-        ri.bStart = b->bStart[r];
-        ri.nB = b->bStart[r + 1] - b->bStart[r];
-        if (ri.nP < 1) return fail(RP_ERR_INVALID, "rod " + std::to_string(r) + " has no particles");
-        ri.radius = desc->radius[r];
-        const int w = desc->num_worlds == 0 ? -1 : (desc->world_of_rod ? desc->world_of_rod[r] : 0);
-        if (w >= desc->num_worlds) return fail(RP_ERR_INVALID, "rod " + std::to_string(r) + ": world out of range");
-        ri.world = w;
-        // Stretch: validate, colour, reorder.
-        res.resize(ri.nS);
-        for (int k = 0; k < ri.nS; ++k) {
-            const int g = ri.sStart + k;
-            const int a = desc->stretch_p0[g], c = desc->stretch_p1[g], f = desc->stretch_seg[g];
-            if (a < 0 || a >= ri.nP || c < 0 || c >= ri.nP || f < 0 || f >= ri.nF)
-                return fail(RP_ERR_INVALID, "rod " + std::to_string(r) + ": stretch constraint " +
-                                                std::to_string(k) + " index out of range");
-            res[k] = {a, c, ri.nP + f};
-            segEnd[2 * size_t(g)] = a;
-            segEnd[2 * size_t(g) + 1] = c;
-        }
-        ri.colOff = int(colStart.size());
-        ri.nSCol = greedy_colours(ri.nS, res, 3, ri.nP + ri.nF, order, start);
-        for (int k = 0; k < ri.nS; ++k) {
-            const int src = ri.sStart + order[k], dst = ri.sStart + k;
-            sIdx[dst] = make_int4(desc->stretch_p0[src], desc->stretch_p1[src], desc->stretch_seg[src], order[k]);
-            sPar[dst] = make_float4(desc->stretch_rest[src], desc->stretch_compliance[3 * size_t(src)],
-                                    desc->stretch_compliance[3 * size_t(src) + 1],
-                                    desc->stretch_compliance[3 * size_t(src) + 2]);
-        }
-        colStart.insert(colStart.end(), start.begin(), start.end());
-        // Bend.
-        res.resize(ri.nB);
-        for (int k = 0; k < ri.nB; ++k) {
-            const int g = ri.bStart + k;
-            const int a = desc->bend_a[g], c = desc->bend_b[g];
-            if (a < 0 || a >= ri.nF || c < 0 || c >= ri.nF)
-                return fail(RP_ERR_INVALID, "rod " + std::to_string(r) + ": bend constraint " + std::to_string(k) +
-                                                " index out of range");
-            res[k] = {ri.nP + a, ri.nP + c, ri.nP + c};
-        }
-        ri.nBCol = greedy_colours(ri.nB, res, 2, ri.nP + ri.nF, order, start);
-        for (int k = 0; k < ri.nB; ++k) {
-            const int src = ri.bStart + order[k], dst = ri.bStart + k;
-            bIdx[dst] = make_int4(desc->bend_a[src], desc->bend_b[src], order[k], 0);
-            bPar[dst] = make_float4(desc->bend_rest[src], desc->bend_compliance[3 * size_t(src)],
-                                    desc->bend_compliance[3 * size_t(src) + 1],
-                                    desc->bend_compliance[3 * size_t(src) + 2]);
-            bRest4[dst] = make_float4(desc->bend_rest_darboux[3 * size_t(src)],
-                                      desc->bend_rest_darboux[3 * size_t(src) + 1],
-                                      desc->bend_rest_darboux[3 * size_t(src) + 2], 0.0f);
-        }
-        colStart.insert(colStart.end(), start.begin(), start.end());
-        if (ri.nSCol > kMaxColours || ri.nBCol > kMaxColours)
-            return fail(RP_ERR_CAPACITY, "rod " + std::to_string(r) + " needs more than " +
-                                             std::to_string(kMaxColours) + " colours");
-        if (ri.nSCol > kTiledColours || ri.nBCol > kTiledColours) b->tileable[r] = 0;
-        // Contacts.
-        if (w >= 0) {
-            const rp_world_desc& wd = desc->worlds[w];
-            ri.nPrim = wd.num_prims;
-            ri.primOff = primOffOfWorld[w];
-            // contactMarginRadii * radius, in FP32 as the CPU path forms it.
-            ri.margin = wd.contact_margin_radii * ri.radius;
-            ri.self = wd.self_collision ? 1 : 0;
-            ri.selfFriction = wd.self_friction;
-            ri.table = wd.self_collision ? wd.hash_table_size : 1;
-            if (ri.self) {
-                // selfCollisionIndexGap (src/core/collision.cpp:67-74), in FP32.
-                float lMin = ri.nS > 0 ? desc->stretch_rest[ri.sStart] : 1.0f;
-                for (int k = 0; k < ri.nS; ++k) lMin = std::min(lMin, desc->stretch_rest[ri.sStart + k]);
-                const int geometric = int(std::floor(2.0f * ri.radius / lMin)) + 2;
-                ri.selfGap = std::max(wd.self_skip, geometric);
-                ri.selfCap = wd.self_capacity > 0 ? wd.self_capacity : std::max(16, ri.nS / 2);
-            }
-        } else {
-            ri.table = 1;
-        }
-        maxPlainLds =
-            std::max(maxPlainLds, plain_lds_bytes(ri.nP, ri.nF, ri.nS, ri.nB, ri.nPrim, ri.self, ri.table, ri.selfCap));
-    }
-
-    hipDeviceProp_t prop;
-    RP_HIP(hipGetDeviceProperties(&prop, 0));
-    // Workgroups may use the whole LDS of a compute unit; where the runtime
-    // refuses the larger dynamic allocation, stay within 64 KiB.
-    b->cuLds = prop.sharedMemPerBlock;
-    int granted = 0;
-    if (allow_dynamic_lds(int(prop.sharedMemPerBlock), &granted) == hipSuccess) {
-        b->ldsLimit = size_t(granted);
-    } else {
-        (void)hipGetLastError();
-        b->cuLds = b->ldsLimit = 65536 - 1024;
-    }
-    b->plainLds = maxPlainLds;
-    for (int r = 0; r < R; ++r) b->anySelf = b->anySelf || b->rods[r].self != 0;
+    LdsLimits lim;
+    if ((rc = lds_limits(lim))) return rc;
+    b->ldsLimit = lim.perWorkgroup;
+    b->cuLds = lim.perCU;
     bool allTileable = true;
     for (int r = 0; r < R; ++r) {
         if (b->tileable[r] && size_t(rod_tiled_lds(b->rods[r])) > b->ldsLimit) b->tileable[r] = 0;
@@ -707,23 +382,15 @@ int rp_batch_create(const rp_batch_desc* desc, void** out_batch) {
     if (strategy == RP_STRATEGY_AUTO) strategy = allTileable ? RP_STRATEGY_TILED : RP_STRATEGY_PLAIN;
     if (strategy == RP_STRATEGY_TILED && !allTileable)
         return fail(RP_ERR_CAPACITY, "a rod does not fit the TILED strategy (more than 2 colours or LDS budget)");
-    if (strategy == RP_STRATEGY_PLAIN && size_t(maxPlainLds) > b->ldsLimit)
-        return fail(RP_ERR_CAPACITY, "a rod needs " + std::to_string(maxPlainLds) + " bytes of LDS; the device has " +
-                                         std::to_string(b->ldsLimit) + " per workgroup");
+    if (strategy == RP_STRATEGY_PLAIN && size_t(rec.maxPlainLds) > b->ldsLimit)
+        return fail(RP_ERR_CAPACITY, "a rod needs " + std::to_string(rec.maxPlainLds) +
+                                         " bytes of LDS; the device has " + std::to_string(b->ldsLimit) +
+                                         " per workgroup");
     b->strategy = strategy;
     if (const char* e = std::getenv("RP_STEPS_PER_LAUNCH")) b->stepsPerLaunch = std::max(1, std::atoi(e));
 
     // ---- device arrays
     RP_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    std::vector<float> invMass(desc->inv_mass, desc->inv_mass + P);
-    std::vector<float4> iI4(F), I4(F);
-    for (int j = 0; j < F; ++j) {
-        iI4[j] = make_float4(desc->inv_inertia[3 * size_t(j)], desc->inv_inertia[3 * size_t(j) + 1],
-                             desc->inv_inertia[3 * size_t(j) + 2], 0.0f);
-        I4[j] = make_float4(desc->inertia[3 * size_t(j)], desc->inertia[3 * size_t(j) + 1],
-                            desc->inertia[3 * size_t(j) + 2], 0.0f);
-    }
-    int rc = 0;
     float *x, *v, *q, *om, *force, *kin, *torque, *cs, *fs;
     int* ovf;
     if ((rc = b->alloc(x, size_t(P) * 3)) || (rc = b->alloc(v, size_t(P) * 3)) || (rc = b->alloc(q, size_t(F) * 4)) ||
@@ -761,26 +428,27 @@ int rp_batch_create(const rp_batch_desc* desc, void** out_batch) {
     d.clock = b->clock;
     d.paths = nullptr;
     d.obs = nullptr;
+    d.stepLimit = nullptr;
     d.numRods = R;
     float *dInvMass, *dI4, *dII4, *dsPar, *dbPar, *dbRest;
     int *dsIdx, *dbIdx, *dSegEnd, *dCol;
     RodInfo* dRods;
     Prim* dPrims;
-    std::vector<float> sParF(size_t(S) * 4), bParF(size_t(B) * 4), bRestF(size_t(B) * 4), iIF(size_t(F) * 4),
-        IF(size_t(F) * 4);
-    std::vector<int> sIdxI(size_t(S) * 4), bIdxI(size_t(B) * 4);
-    std::memcpy(sParF.data(), sPar.data(), sParF.size() * 4);
-    std::memcpy(bParF.data(), bPar.data(), bParF.size() * 4);
-    std::memcpy(bRestF.data(), bRest4.data(), bRestF.size() * 4);
-    std::memcpy(iIF.data(), iI4.data(), iIF.size() * 4);
-    std::memcpy(IF.data(), I4.data(), IF.size() * 4);
-    std::memcpy(sIdxI.data(), sIdx.data(), sIdxI.size() * 4);
-    std::memcpy(bIdxI.data(), bIdx.data(), bIdxI.size() * 4);
-    if ((rc = b->upload(dInvMass, invMass)) || (rc = b->upload(dII4, iIF)) || (rc = b->upload(dI4, IF)) ||
-        (rc = b->upload(dsIdx, sIdxI)) || (rc = b->upload(dsPar, sParF)) || (rc = b->upload(dbIdx, bIdxI)) ||
-        (rc = b->upload(dbPar, bParF)) || (rc = b->upload(dbRest, bRestF)) || (rc = b->upload(dSegEnd, segEnd)) ||
-        (rc = b->upload(dCol, colStart)) || (rc = b->upload(dRods, b->rods)) || (rc = b->upload(dPrims, prims)))
+    int4 *sIdx4, *bIdx4;
+    float4 *iI4, *I4, *sPar4, *bPar4, *bRest4;
+    if ((rc = b->upload(dInvMass, rec.invMass)) || (rc = b->upload(iI4, rec.iI4)) || (rc = b->upload(I4, rec.I4)) ||
+        (rc = b->upload(sIdx4, rec.sIdx)) || (rc = b->upload(sPar4, rec.sPar)) || (rc = b->upload(bIdx4, rec.bIdx)) ||
+        (rc = b->upload(bPar4, rec.bPar)) || (rc = b->upload(bRest4, rec.bRest4)) ||
+        (rc = b->upload(dSegEnd, rec.segEnd)) || (rc = b->upload(dCol, rec.colStart)) ||
+        (rc = b->upload(dRods, b->rods)) || (rc = b->upload(dPrims, rec.prims)))
         return rc;
+    dII4 = reinterpret_cast<float*>(iI4);
+    dI4 = reinterpret_cast<float*>(I4);
+    dsIdx = reinterpret_cast<int*>(sIdx4);
+    dsPar = reinterpret_cast<float*>(sPar4);
+    dbIdx = reinterpret_cast<int*>(bIdx4);
+    dbPar = reinterpret_cast<float*>(bPar4);
+    dbRest = reinterpret_cast<float*>(bRest4);
     d.invMass = dInvMass;
     d.invInertia4 = dII4;
     d.inertia4 = dI4;
@@ -1128,7 +796,7 @@ int rp_batch_counters(void* batch, rp_counters* out) {
         out->lds_bytes_per_workgroup = size_t(b.plainLds);
         out->launches_per_step = 1;
     } else if (b.tiles) {
-        const TileConfig& c = kConfigs[b.tiles->config - 1];
+        const TileConfig& c = tile_config(b.tiles->config);
         out->tile_config = b.tiles->config;
         out->workgroups = int(b.tiles->tiles.size());
         out->lanes_busy = b.tiles->lanesBusy;

@@ -76,6 +76,7 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
     char* smem = reinterpret_cast<char*>(smem4);
     const int r = blockIdx.x;
     if (r >= d.numRods) return;
+    if (d.stepLimit && d.clock[r] >= d.stepLimit[r]) return;
     const RodInfo ri = d.rods[r];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nP = ri.nP, nF = ri.nF, nS = ri.nS, nB = ri.nB, nPrim = ri.nPrim;
@@ -440,6 +441,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2
     const v3 g = mk(sa.gx, sa.gy, sa.gz);
     const float h = sa.h;
     int clk = active ? d.clock[rod] : 0;
+    // A rod whose clock has reached its step limit stops changing; its lanes
+    // still take part in every barrier.
+    const int limit = (active && d.stepLimit) ? d.stepLimit[rod] : 0x7fffffff;
     const int drivenP = (active && d.paths) ? d.paths[rod].particle : -1;
     const bool driver = drivenP >= 0 && drivenP % G == l;
     const int obsP = (active && d.obs) ? d.obs[rod].particle : -1;
@@ -448,7 +452,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2
     __syncthreads();
 
     for (int step = 0; step < sa.steps; ++step) {
-        if (driver) kv4[drivenP] = vf4(path_velocity(d.paths[rod], clk), 0.0f);
+        const bool live = active && clk < limit;
+        if (driver && live) kv4[drivenP] = vf4(path_velocity(d.paths[rod], clk), 0.0f);
         for (int sub = 0; sub < sa.substeps; ++sub) {
             const bool regen = sub % sa.contactInterval == 0;
 #pragma unroll
@@ -462,7 +467,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2
 #pragma unroll
                 for (int t = 0; t < TP; ++t) {
                     const int i = l + G * t;
-                    if (active && i < nP)
+                    if (live && i < nP)
                         world_contacts_begin(cont + i * nPrim * kContactFloats, prims, nPrim, f4v(x4[i]), wP[t],
                                              ri.radius, ri.margin, fscale, regen);
                 }
@@ -470,8 +475,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2
             if (anySelf) {
                 if (regen) {
                     self_regen_group<TP>(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
-                                     ri.selfFriction * fscale, ri.self != 0, l, G);
-                } else if (ri.self && l == 0) {
+                                     ri.selfFriction * fscale, ri.self != 0 && live, l, G);
+                } else if (ri.self && l == 0 && live) {
                     self_reset(sl);
                 }
                 __syncthreads();
@@ -480,14 +485,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2
 #pragma unroll
             for (int t = 0; t < TP; ++t) {
                 const int i = l + G * t;
-                if (active && i < nP) {
+                if (live && i < nP) {
                     v3 x = f4v(x4[i]);
                     predict_particle(x, vel[t], xpv[t], wP[t], f4v(f4[i]), f4v(kv4[i]), h, g);
                     x4[i] = vf4(x, wP[t]);
                     if (SELF && ri.self) xp4[i] = vf4(xpv[t], wP[t]);
                 }
                 const int j = l + G * t;
-                if (active && j < nF) {
+                if (live && j < nF) {
                     q4 q = f4q(q4s[j]);
                     predict_frame(q, om[t], qpv[t], iIF[t], IF[t], f4v(tq4[j]), h);
                     q4s[j] = qf4(q);
@@ -500,7 +505,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2
                 for (int c = 0; c < 2; ++c) {
 #pragma unroll
                     for (int t = 0; t < TC; ++t) {
-                        if (sOn[c][t]) {
+                        if (live && sOn[c][t]) {
                             const float4 ax = x4[sA[c][t]], bx = x4[sB[c][t]];
                             v3 xa = f4v(ax), xb = f4v(bx);
                             q4 q = f4q(q4s[sF[c][t]]);
@@ -518,7 +523,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2
                 for (int c = 0; c < 2; ++c) {
 #pragma unroll
                     for (int t = 0; t < TC; ++t) {
-                        if (bOn[c][t]) {
+                        if (live && bOn[c][t]) {
                             q4 qa = f4q(q4s[bA_[c][t]]), qb = f4q(q4s[bB_[c][t]]);
                             if (project_bend(qa, qb, bLam[c][t], bIa[c][t], bIb[c][t], bRest[c][t], bComp[c][t],
                                              bRD[c][t], sa.invH2, cscale)) {
@@ -534,7 +539,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2
 #pragma unroll
                         for (int t = 0; t < TP; ++t) {
                             const int i = l + G * t;
-                            if (active && i < nP && wP[t] > 0.0f) {
+                            if (live && i < nP && wP[t] > 0.0f) {
                                 v3 x = f4v(x4[i]);
                                 world_contacts_project(cont + i * nPrim * kContactFloats, nPrim, x, xpv[t], wP[t]);
                                 x4[i] = vf4(x, wP[t]);
@@ -544,7 +549,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2
                     __syncthreads();
                 }
                 if (anySelf) {
-                    if (ri.self && l == 0) self_project(sl, x4, xp4);
+                    if (ri.self && l == 0 && live) self_project(sl, x4, xp4);
                     __syncthreads();
                 }
             }
@@ -553,12 +558,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2
 #pragma unroll
             for (int t = 0; t < TP; ++t) {
                 const int i = l + G * t;
-                if (active && i < nP)
+                if (live && i < nP)
                     vel[t] = finish_particle(f4v(x4[i]), xpv[t], wP[t], f4v(kv4[i]), h, sa.linDecay);
                 const int j = l + G * t;
-                if (active && j < nF) om[t] = finish_frame(f4q(q4s[j]), qpv[t], iIF[t], h, sa.angDecay);
+                if (live && j < nF) om[t] = finish_frame(f4q(q4s[j]), qpv[t], iIF[t], h, sa.angDecay);
             }
         }
+        if (!live) continue;
         ++clk;
         if (observer && clk % d.obs[rod].every == 0) {
             const double rise = double(x4[obsP].z) - d.obs[rod].z0;

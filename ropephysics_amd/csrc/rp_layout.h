@@ -121,6 +121,7 @@ struct DeviceArrays {
     int* clock;            // steps taken, per rod
     const PathRec* paths;  // per rod, or null
     ObsRec* obs;           // per rod, or null
+    const int* stepLimit;  // per rod: no step once the clock reaches it; null for none
     int numRods;
 };
 

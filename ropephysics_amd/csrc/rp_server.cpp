@@ -1,0 +1,497 @@
+// Continuous-batching server: simulation requests of any size arrive, are
+// admitted while the device memory set aside for them (an HBM budget) has
+// room, stay resident on the device between calls, and are stepped together:
+// every call of rp_server_step is one TILED launch over the rods of every
+// resident request, each rod stopping at its own request's horizon. Finished
+// requests are read back and released by the caller, freeing their ranges for
+// requests still waiting.
+//
+// The reference has no counterpart; the arithmetic is the batch's (the same
+// records and kernels), so a request's result does not depend on what else
+// was resident.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "rp_host.h"
+
+namespace {
+
+using namespace rp;
+using namespace rp::host;
+
+// First-fit allocator of index ranges in [0, capacity), coalescing on free.
+class RangePool {
+   public:
+    void reset(long long capacity) {
+        cap_ = capacity;
+        free_.clear();
+        if (capacity > 0) free_[0] = capacity;
+        used_ = 0;
+    }
+    // Start of a free range of `n` (n == 0: returns 0 and uses nothing), or -1.
+    long long take(long long n) {
+        if (n == 0) return 0;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            const long long start = it->first, len = it->second;
+            free_.erase(it);
+            if (len > n) free_[start + n] = len - n;
+            used_ += n;
+            return start;
+        }
+        return -1;
+    }
+    void give(long long start, long long n) {
+        if (n == 0) return;
+        used_ -= n;
+        auto next = free_.lower_bound(start);
+        if (next != free_.end() && start + n == next->first) {
+            n += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == start) {
+                prev->second += n;
+                return;
+            }
+        }
+        free_[start] = n;
+    }
+    long long used() const { return used_; }
+    long long capacity() const { return cap_; }
+
+   private:
+    std::map<long long, long long> free_;
+    long long cap_ = 0, used_ = 0;
+};
+
+enum Pool { kParticles, kFrames, kStretch, kBend, kRods, kPrims, kColours, kNumPools };
+
+// Device bytes per element of each pool.
+constexpr long long kBytesPer[kNumPools] = {
+    12 * 4 + 4,          // particle: x, v, force, kinVel (xyz); invMass
+    16 + 12 + 12 + 32,   // frame: q, omega, torque, invInertia4, inertia4
+    16 + 16 + 8,         // stretch: sIdx, sPar, segEnd
+    16 + 16 + 16,        // bend: bIdx, bPar, bRest4
+    int(sizeof(RodInfo)) + 6 * 4 + int(sizeof(Tile)) + int(sizeof(TileSlot)),  // rod: record, per-rod words, packing
+    int(sizeof(Prim)),   // primitive
+    4,                   // colour-start entry
+};
+
+struct Request {
+    int id = 0;
+    long long start[kNumPools] = {};
+    long long count[kNumPools] = {};
+    int steps = 0;       // horizon
+    int stepsDone = 0;   // steps every rod of the request has taken
+    long long bytes = 0;
+};
+
+struct DevArray {
+    void* p = nullptr;
+    ~DevArray() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+struct rp_server_impl {
+    rp_step_params params{};
+    int configId = 9;
+    int stepsPerLaunch = 64;
+    LdsLimits lim;
+    hipStream_t stream = nullptr;
+    RangePool pools[kNumPools];
+    std::vector<std::unique_ptr<DevArray>> owned;
+    long long deviceBytes = 0;
+    DeviceArrays d{};
+    int* stepLimit = nullptr;
+    Tile* dTiles = nullptr;
+    TileSlot* dSlots = nullptr;
+    std::vector<RodInfo> rods;  // host mirror, indexed by rod slot
+    std::vector<int> colStart;  // host mirror of the colour-start table
+    std::vector<unsigned char> selfOf;
+    std::map<int, Request> requests;  // resident, by id
+    std::vector<int> finished;        // ids whose horizon is reached, not yet released
+    int nextId = 1;
+    bool dirty = true;  // membership changed since the last packing
+    Packing packing;
+    long long launches = 0;
+    long long stepsTaken = 0;
+
+    ~rp_server_impl() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <typename T>
+    int alloc(T*& out, long long count) {
+        auto b = std::make_unique<DevArray>();
+        const size_t bytes = size_t(std::max<long long>(count, 1)) * sizeof(T);
+        const hipError_t e = hipMalloc(&b->p, bytes);
+        if (e != hipSuccess)
+            return fail(RP_ERR_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                           hipGetErrorString(e));
+        RP_HIP(hipMemset(b->p, 0, bytes));
+        deviceBytes += (long long)bytes;
+        out = static_cast<T*>(b->p);
+        owned.push_back(std::move(b));
+        return RP_OK;
+    }
+
+    // Copy `count` host elements to device array `dst` at element `at`.
+    template <typename T>
+    int put(T* dst, long long at, const T* src, long long count) {
+        if (count <= 0) return RP_OK;
+        RP_HIP(hipMemcpyAsync(dst + at, src, size_t(count) * sizeof(T), hipMemcpyHostToDevice, stream));
+        return RP_OK;
+    }
+};
+
+namespace {
+
+rp_server_impl* S(void* p) { return static_cast<rp_server_impl*>(p); }
+
+void release_ranges(rp_server_impl& s, const Request& q) {
+    for (int k = 0; k < kNumPools; ++k) s.pools[k].give(q.start[k], q.count[k]);
+    for (long long r = q.start[kRods]; r < q.start[kRods] + q.count[kRods]; ++r) s.selfOf[size_t(r)] = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rp_server_create(const rp_server_desc* desc, void** out_server) {
+    if (!desc || !out_server) return fail(RP_ERR_INVALID, "null argument");
+    *out_server = nullptr;
+    if (desc->hbm_budget_bytes <= 0) return fail(RP_ERR_INVALID, "hbm_budget_bytes must be positive");
+    if (desc->params.dt <= 0.0f || desc->params.substeps < 1 || desc->params.iterations < 1)
+        return fail(RP_ERR_INVALID, "dt, substeps and iterations must be positive");
+    if (desc->tile_config < 0 || desc->tile_config > num_tile_configs())
+        return fail(RP_ERR_INVALID, "unknown tile configuration");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(RP_ERR_DEVICE, "no usable HIP device");
+    auto s = std::make_unique<rp_server_impl>();
+    s->params = desc->params;
+    if (desc->tile_config > 0) s->configId = desc->tile_config;
+    if (desc->steps_per_launch > 0) s->stepsPerLaunch = desc->steps_per_launch;
+    int rc = lds_limits(s->lim);
+    if (rc) return rc;
+
+    // Capacities from the budget: per segment about one particle, one frame,
+    // one stretch and one bend constraint; rods, primitives and colour
+    // entries sized for rods of 16 segments on average.
+    const long long perSegment = kBytesPer[kParticles] + kBytesPer[kFrames] + kBytesPer[kStretch] + kBytesPer[kBend];
+    const long long perRodShare = (kBytesPer[kRods] + 2 * kBytesPer[kPrims] + 18 * kBytesPer[kColours]) / 16;
+    long long segs = desc->hbm_budget_bytes / (perSegment + perRodShare);
+    if (segs < 64) return fail(RP_ERR_INVALID, "hbm_budget_bytes below the smallest useful pool");
+    segs = std::min<long long>(segs, 1LL << 30);
+    long long rods = desc->max_rods > 0 ? desc->max_rods : std::max<long long>(1, segs / 16);
+    rods = std::min<long long>(rods, 1LL << 24);
+    const long long cap[kNumPools] = {segs, segs, segs, segs, rods, 2 * rods, 18 * rods};
+    for (int k = 0; k < kNumPools; ++k) s->pools[k].reset(cap[k]);
+
+    RP_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    DeviceArrays& d = s->d;
+    float *x, *v, *q, *om, *force, *kin, *torque, *invMass, *cs, *fs;
+    float4 *iI4, *I4, *sPar, *bPar, *bRest;
+    int4 *sIdx, *bIdx;
+    int *segEnd, *col, *ovf, *clock;
+    RodInfo* dRods;
+    Prim* dPrims;
+    const long long P = cap[kParticles], F = cap[kFrames], St = cap[kStretch], B = cap[kBend], R = cap[kRods];
+    if ((rc = s->alloc(x, P * 3)) || (rc = s->alloc(v, P * 3)) || (rc = s->alloc(force, P * 3)) ||
+        (rc = s->alloc(kin, P * 3)) || (rc = s->alloc(invMass, P)) || (rc = s->alloc(q, F * 4)) ||
+        (rc = s->alloc(om, F * 3)) || (rc = s->alloc(torque, F * 3)) || (rc = s->alloc(iI4, F)) ||
+        (rc = s->alloc(I4, F)) || (rc = s->alloc(sIdx, St)) || (rc = s->alloc(sPar, St)) ||
+        (rc = s->alloc(segEnd, 2 * St)) || (rc = s->alloc(bIdx, B)) || (rc = s->alloc(bPar, B)) ||
+        (rc = s->alloc(bRest, B)) || (rc = s->alloc(dRods, R)) || (rc = s->alloc(cs, R)) ||
+        (rc = s->alloc(fs, R)) || (rc = s->alloc(ovf, R)) || (rc = s->alloc(clock, R)) ||
+        (rc = s->alloc(s->stepLimit, R)) || (rc = s->alloc(s->dTiles, R)) || (rc = s->alloc(s->dSlots, R)) ||
+        (rc = s->alloc(dPrims, cap[kPrims])) || (rc = s->alloc(col, cap[kColours])))
+        return rc;
+    d.x = x;
+    d.v = v;
+    d.q = q;
+    d.omega = om;
+    d.force = force;
+    d.kinVel = kin;
+    d.torque = torque;
+    d.invMass = invMass;
+    d.invInertia4 = reinterpret_cast<float*>(iI4);
+    d.inertia4 = reinterpret_cast<float*>(I4);
+    d.sIdx = reinterpret_cast<int*>(sIdx);
+    d.sPar = reinterpret_cast<float*>(sPar);
+    d.bIdx = reinterpret_cast<int*>(bIdx);
+    d.bPar = reinterpret_cast<float*>(bPar);
+    d.bRest4 = reinterpret_cast<float*>(bRest);
+    d.segEnd = segEnd;
+    d.colStart = col;
+    d.rods = dRods;
+    d.prims = dPrims;
+    d.complianceScale = cs;
+    d.frictionScale = fs;
+    d.selfOverflow = ovf;
+    d.clock = clock;
+    d.paths = nullptr;
+    d.obs = nullptr;
+    d.stepLimit = s->stepLimit;
+    d.numRods = int(R);
+    s->rods.assign(size_t(R), RodInfo{});
+    s->colStart.assign(size_t(cap[kColours]), 0);
+    s->selfOf.assign(size_t(R), 0);
+    *out_server = s.release();
+    return RP_OK;
+}
+
+void rp_server_destroy(void* server) {
+    if (!server) return;
+    rp_server_impl* s = S(server);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    delete s;
+}
+
+int rp_server_submit(void* server, const rp_batch_desc* req, const rp_request_state* st, int steps, int* out_id) {
+    if (!server || !req || !st || !out_id) return fail(RP_ERR_INVALID, "null argument");
+    rp_server_impl& s = *S(server);
+    *out_id = 0;
+    if (steps < 1) return fail(RP_ERR_INVALID, "a request needs at least one step");
+    if (!st->x || !st->q) return fail(RP_ERR_INVALID, "a request needs initial positions and orientations");
+    Records rec;
+    int rc = build_records(req, rec);
+    if (rc) return rc;
+    const TileConfig& cfg = tile_config(s.configId);
+    for (int r = 0; r < rec.R; ++r) {
+        const RodInfo& ri = rec.rods[r];
+        if (!rec.colourFit[r]) return fail(RP_ERR_CAPACITY, "rod " + std::to_string(r) + " needs more than two colours");
+        if (size_t(rod_tiled_lds(ri)) > s.lim.perWorkgroup)
+            return fail(RP_ERR_CAPACITY, "rod " + std::to_string(r) + " needs more LDS than a workgroup has");
+        if (group_lanes(ri, rec.colStart, cfg.tp, cfg.tc) > 64 * cfg.waves)
+            return fail(RP_ERR_CAPACITY, "rod " + std::to_string(r) + " is too long for the tile shape");
+    }
+    Request q;
+    const long long need[kNumPools] = {rec.P, rec.F, rec.S, rec.B, rec.R, (long long)rec.prims.size(),
+                                       (long long)rec.colStart.size()};
+    for (int k = 0; k < kNumPools; ++k) {
+        if (need[k] > s.pools[k].capacity())
+            return fail(RP_ERR_CAPACITY, "request larger than the server's whole budget");
+        const long long at = s.pools[k].take(need[k]);
+        if (at < 0) {
+            for (int j = 0; j < k; ++j) s.pools[j].give(q.start[j], q.count[j]);
+            return fail(RP_ERR_CAPACITY, "no room for the request now; release finished requests and retry");
+        }
+        q.start[k] = at;
+        q.count[k] = need[k];
+        q.bytes += need[k] * kBytesPer[k];
+    }
+    // Rod records with offsets into the pools.
+    const long long p0 = q.start[kParticles], f0 = q.start[kFrames], s0 = q.start[kStretch], b0 = q.start[kBend];
+    const long long r0 = q.start[kRods], prim0 = q.start[kPrims], c0 = q.start[kColours];
+    std::vector<RodInfo> recs(rec.rods);
+    for (RodInfo& ri : recs) {
+        ri.pStart += int(p0);
+        ri.fStart += int(f0);
+        ri.sStart += int(s0);
+        ri.bStart += int(b0);
+        ri.primOff += int(prim0);
+        ri.colOff += int(c0);
+    }
+    const size_t P = size_t(rec.P), F = size_t(rec.F), R = size_t(rec.R);
+    std::vector<float> ones(R, 1.0f), comp(R, 1.0f), fric(R, 1.0f), zeros3(std::max(P, F) * 3, 0.0f);
+    for (size_t r = 0; r < R; ++r) {
+        if (st->youngs_scale) {
+            if (!(st->youngs_scale[r] > 0.0f)) {
+                release_ranges(s, q);
+                return fail(RP_ERR_INVALID, "youngs_scale must be positive");
+            }
+            // Compliance scales as the inverse of Young's modulus, as the batch forms it.
+            comp[r] = 1.0f / st->youngs_scale[r];
+        }
+        if (st->friction_scale) fric[r] = st->friction_scale[r];
+    }
+    std::vector<int> zerosR(R, 0), limit(R, steps);
+    const DeviceArrays& d = s.d;
+    if ((rc = s.put(d.x, p0 * 3, st->x, P * 3)) ||
+        (rc = s.put(d.v, p0 * 3, st->v ? st->v : zeros3.data(), P * 3)) ||
+        (rc = s.put(const_cast<float*>(d.force), p0 * 3, st->force ? st->force : zeros3.data(), P * 3)) ||
+        (rc = s.put(const_cast<float*>(d.kinVel), p0 * 3, st->kin_vel ? st->kin_vel : zeros3.data(), P * 3)) ||
+        (rc = s.put(const_cast<float*>(d.invMass), p0, rec.invMass.data(), P)) ||
+        (rc = s.put(d.q, f0 * 4, st->q, F * 4)) ||
+        (rc = s.put(d.omega, f0 * 3, st->omega ? st->omega : zeros3.data(), F * 3)) ||
+        (rc = s.put(const_cast<float*>(d.torque), f0 * 3, st->torque ? st->torque : zeros3.data(), F * 3)) ||
+        (rc = s.put(reinterpret_cast<float4*>(const_cast<float*>(d.invInertia4)), f0, rec.iI4.data(), F)) ||
+        (rc = s.put(reinterpret_cast<float4*>(const_cast<float*>(d.inertia4)), f0, rec.I4.data(), F)) ||
+        (rc = s.put(reinterpret_cast<int4*>(const_cast<int*>(d.sIdx)), s0, rec.sIdx.data(), rec.S)) ||
+        (rc = s.put(reinterpret_cast<float4*>(const_cast<float*>(d.sPar)), s0, rec.sPar.data(), rec.S)) ||
+        (rc = s.put(const_cast<int*>(d.segEnd), 2 * s0, rec.segEnd.data(), 2LL * rec.S)) ||
+        (rc = s.put(reinterpret_cast<int4*>(const_cast<int*>(d.bIdx)), b0, rec.bIdx.data(), rec.B)) ||
+        (rc = s.put(reinterpret_cast<float4*>(const_cast<float*>(d.bPar)), b0, rec.bPar.data(), rec.B)) ||
+        (rc = s.put(reinterpret_cast<float4*>(const_cast<float*>(d.bRest4)), b0, rec.bRest4.data(), rec.B)) ||
+        (rc = s.put(const_cast<Prim*>(d.prims), prim0, rec.prims.data(), (long long)rec.prims.size())) ||
+        (rc = s.put(const_cast<int*>(d.colStart), c0, rec.colStart.data(), (long long)rec.colStart.size())) ||
+        (rc = s.put(const_cast<RodInfo*>(d.rods), r0, recs.data(), (long long)R)) ||
+        (rc = s.put(d.complianceScale, r0, comp.data(), (long long)R)) ||
+        (rc = s.put(d.frictionScale, r0, fric.data(), (long long)R)) ||
+        (rc = s.put(d.selfOverflow, r0, zerosR.data(), (long long)R)) ||
+        (rc = s.put(d.clock, r0, zerosR.data(), (long long)R)) ||
+        (rc = s.put(s.stepLimit, r0, limit.data(), (long long)R))) {
+        release_ranges(s, q);
+        return rc;
+    }
+    // The host vectors above go out of scope: wait for the copies.
+    RP_HIP(hipStreamSynchronize(s.stream));
+    for (size_t r = 0; r < R; ++r) {
+        s.rods[size_t(r0) + r] = recs[r];
+        s.selfOf[size_t(r0) + r] = recs[r].self ? 1 : 0;
+    }
+    std::copy(rec.colStart.begin(), rec.colStart.end(), s.colStart.begin() + c0);
+    q.id = s.nextId++;
+    q.steps = steps;
+    s.requests[q.id] = q;
+    s.dirty = true;
+    *out_id = q.id;
+    return RP_OK;
+}
+
+int rp_server_step(void* server, int* num_finished) {
+    if (!server) return fail(RP_ERR_INVALID, "null server");
+    rp_server_impl& s = *S(server);
+    if (num_finished) *num_finished = 0;
+    std::vector<int> ids;
+    bool anySelf = false;
+    for (const auto& kv : s.requests) {
+        const Request& q = kv.second;
+        if (q.stepsDone >= q.steps) continue;
+        for (long long r = q.start[kRods]; r < q.start[kRods] + q.count[kRods]; ++r) {
+            ids.push_back(int(r));
+            anySelf = anySelf || s.selfOf[size_t(r)];
+        }
+    }
+    if (ids.empty()) return RP_OK;
+    const TileConfig& cfg = tile_config(s.configId);
+    if (s.dirty) {
+        // Two workgroups per compute unit when every rod fits that budget.
+        int rc = pack_rods(s.rods, ids, s.colStart, s.configId, lds_budget(s.lim, 2), s.packing);
+        if (rc) rc = pack_rods(s.rods, ids, s.colStart, s.configId, lds_budget(s.lim, 1), s.packing);
+        if (rc) return fail(rc, "resident rods do not pack");
+        RP_HIP(hipMemcpyAsync(s.dTiles, s.packing.tiles.data(), s.packing.tiles.size() * sizeof(Tile),
+                              hipMemcpyHostToDevice, s.stream));
+        RP_HIP(hipMemcpyAsync(s.dSlots, s.packing.slots.data(), s.packing.slots.size() * sizeof(TileSlot),
+                              hipMemcpyHostToDevice, s.stream));
+        RP_HIP(hipStreamSynchronize(s.stream));
+        s.dirty = false;
+    }
+    const StepArgs a = step_args(s.params, s.stepsPerLaunch);
+    RP_HIP(launch_tiled(cfg.tp, cfg.tc, anySelf, s.d, a, s.dTiles, s.dSlots, int(s.packing.tiles.size()),
+                        64 * cfg.waves, size_t(s.packing.maxLds), s.stream));
+    ++s.launches;
+    s.stepsTaken += s.stepsPerLaunch;
+    int done = 0;
+    for (auto& kv : s.requests) {
+        Request& q = kv.second;
+        if (q.stepsDone >= q.steps) continue;
+        q.stepsDone = std::min(q.steps, q.stepsDone + s.stepsPerLaunch);
+        if (q.stepsDone >= q.steps) {
+            s.finished.push_back(q.id);
+            s.dirty = true;
+            ++done;
+        }
+    }
+    if (num_finished) *num_finished = done;
+    return RP_OK;
+}
+
+int rp_server_sync(void* server) {
+    if (!server) return fail(RP_ERR_INVALID, "null server");
+    RP_HIP(hipStreamSynchronize(S(server)->stream));
+    return RP_OK;
+}
+
+int rp_server_finished(void* server, int* ids, int capacity) {
+    if (!server) return fail(RP_ERR_INVALID, "null server");
+    rp_server_impl& s = *S(server);
+    const int n = int(s.finished.size());
+    for (int i = 0; ids && i < std::min(n, capacity); ++i) ids[i] = s.finished[size_t(i)];
+    return n;
+}
+
+int rp_server_request_info(void* server, int id, rp_request_info* out) {
+    if (!server || !out) return fail(RP_ERR_INVALID, "null argument");
+    rp_server_impl& s = *S(server);
+    auto it = s.requests.find(id);
+    if (it == s.requests.end()) return fail(RP_ERR_INVALID, "unknown request " + std::to_string(id));
+    const Request& q = it->second;
+    out->num_rods = int(q.count[kRods]);
+    out->num_particles = int(q.count[kParticles]);
+    out->num_frames = int(q.count[kFrames]);
+    out->steps = q.steps;
+    out->steps_done = q.stepsDone;
+    out->device_bytes = q.bytes;
+    return RP_OK;
+}
+
+int rp_server_result(void* server, int id, float* x, float* q, float* v, float* omega) {
+    if (!server) return fail(RP_ERR_INVALID, "null server");
+    rp_server_impl& s = *S(server);
+    auto it = s.requests.find(id);
+    if (it == s.requests.end()) return fail(RP_ERR_INVALID, "unknown request " + std::to_string(id));
+    const Request& r = it->second;
+    if (r.stepsDone < r.steps) return fail(RP_ERR_STATE, "request " + std::to_string(id) + " has not finished");
+    const size_t p0 = size_t(r.start[kParticles]), f0 = size_t(r.start[kFrames]);
+    const size_t P = size_t(r.count[kParticles]), F = size_t(r.count[kFrames]);
+    if (x) RP_HIP(hipMemcpyAsync(x, s.d.x + p0 * 3, P * 12, hipMemcpyDeviceToHost, s.stream));
+    if (v) RP_HIP(hipMemcpyAsync(v, s.d.v + p0 * 3, P * 12, hipMemcpyDeviceToHost, s.stream));
+    if (q) RP_HIP(hipMemcpyAsync(q, s.d.q + f0 * 4, F * 16, hipMemcpyDeviceToHost, s.stream));
+    if (omega) RP_HIP(hipMemcpyAsync(omega, s.d.omega + f0 * 3, F * 12, hipMemcpyDeviceToHost, s.stream));
+    RP_HIP(hipStreamSynchronize(s.stream));
+    return RP_OK;
+}
+
+int rp_server_release(void* server, int id) {
+    if (!server) return fail(RP_ERR_INVALID, "null server");
+    rp_server_impl& s = *S(server);
+    auto it = s.requests.find(id);
+    if (it == s.requests.end()) return fail(RP_ERR_INVALID, "unknown request " + std::to_string(id));
+    // The ranges may be reused by the next submission: let queued work on
+    // them finish first.
+    RP_HIP(hipStreamSynchronize(s.stream));
+    release_ranges(s, it->second);
+    s.finished.erase(std::remove(s.finished.begin(), s.finished.end(), id), s.finished.end());
+    if (it->second.stepsDone < it->second.steps) s.dirty = true;
+    s.requests.erase(it);
+    return RP_OK;
+}
+
+int rp_server_get_stats(void* server, rp_server_stats* out) {
+    if (!server || !out) return fail(RP_ERR_INVALID, "null argument");
+    rp_server_impl& s = *S(server);
+    std::memset(out, 0, sizeof(*out));
+    for (const auto& kv : s.requests) {
+        const Request& q = kv.second;
+        ++out->resident_requests;
+        if (q.stepsDone < q.steps) {
+            ++out->running_requests;
+            out->running_rods += int(q.count[kRods]);
+        }
+        out->bytes_in_use += q.bytes;
+    }
+    out->device_bytes = s.deviceBytes;
+    out->capacity_particles = s.pools[kParticles].capacity();
+    out->capacity_rods = s.pools[kRods].capacity();
+    out->used_particles = s.pools[kParticles].used();
+    out->used_rods = s.pools[kRods].used();
+    out->launches = s.launches;
+    out->workgroups = int(s.packing.tiles.size());
+    out->tile_config = s.configId;
+    out->steps_per_launch = s.stepsPerLaunch;
+    return RP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,146 @@
+"""GPU tests of the continuous-batching server: each request's result is
+bitwise the one the same rods get alone (in a Batch, and alone in a server),
+whatever else is resident and whenever it was admitted; horizons are honoured
+to the step; admission follows the HBM budget and freed ranges are reused."""
+import numpy as np
+import pytest
+
+import ropephysics_amd as rp
+from helpers import benchmark_rod, bits, compare_bitwise, oracle_params, oracle_state, oracle_twin, oracle_world
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = rp.SolverParams(dt=1e-3, substeps=10, iterations=1, gravity=(0.0, 0.0, -9.81))
+
+
+def _world(seed: int, self_collision: bool) -> rp.CollisionWorld:
+    rng = np.random.default_rng(seed)
+    prims = [rp.Primitive.plane((0, 0, -0.02), (0, 0, 1), float(rng.uniform(0.3, 0.9)))]
+    prims.append(rp.Primitive.sphere((float(rng.uniform(0.2, 0.6)), 0.0, -0.09), 0.08, 0.4))
+    prims.append(rp.Primitive.capsule((0.3, -0.1, -0.03), (0.3, 0.1, -0.03), 0.012, 0.5))
+    prims.append(rp.Primitive.box((0.7, 0.0, -0.035), (0.05, 0.05, 0.02), (1.0, 0.0, 0.0, 0.0), 0.6))
+    return rp.CollisionWorld(primitives=prims, self_collision=self_collision, self_friction=0.3,
+                             hash_table_size=64)
+
+
+def _request(seed: int):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 6))
+    rods = [benchmark_rod(int(rng.integers(8, 80))) for _ in range(n)]
+    ys = rng.uniform(0.5, 2.0, n)
+    fs = rng.uniform(0.5, 1.5, n)
+    world = _world(seed, bool(rng.integers(0, 2)))
+    steps = int(rng.integers(10, 150))
+    return rods, world, steps, ys, fs
+
+
+def _alone_in_batch(rods, world, steps, ys, fs):
+    b = rp.Batch(rods, strategy="tiled", world=world, tile_config=rp.server.DEFAULT_TILE_CONFIG)
+    b.set_material_scales(ys, fs)
+    b.step(PARAMS, steps)
+    b.synchronize()
+    return [b.download(i) for i in range(len(rods))]
+
+
+def _split(res, rods):
+    out, p0, f0 = [], 0, 0
+    for r in rods:
+        P, F = r.num_particles, r.num_frames
+        out.append({"x": res["x"][p0:p0 + P], "v": res["v"][p0:p0 + P], "q": res["q"][f0:f0 + F],
+                    "omega": res["omega"][f0:f0 + F]})
+        p0 += P
+        f0 += F
+    return out
+
+
+def _assert_same(got, want):
+    for g, w in zip(got, want):
+        d = compare_bitwise(g, w)
+        assert all(v == 0 for v in d.values()), d
+
+
+def _serve(server, requests, arrivals):
+    """Submit request k once `arrivals[k]` chunks have run (admission retried
+    when the budget is full), step until all are done; results by index."""
+    pending = list(range(len(requests)))
+    ids, results, chunk = {}, {}, 0
+    while pending or len(results) < len(requests):
+        for k in list(pending):
+            if arrivals[k] > chunk:
+                continue
+            rods, world, steps, ys, fs = requests[k]
+            try:
+                ids[server.submit(rods, world, steps, ys, fs)] = k
+                pending.remove(k)
+            except rp.ServerFull:
+                break
+        server.step()
+        chunk += 1
+        server.synchronize()
+        for rid in server.finished():
+            results[ids[rid]] = server.result(rid)
+            server.release(rid)
+        assert chunk < 1000
+    return results
+
+
+def test_request_results_equal_alone():
+    requests = [_request(s) for s in range(8)]
+    server = rp.Server(PARAMS, hbm_budget_bytes=64 << 20, steps_per_launch=32)
+    results = _serve(server, requests, arrivals=[0, 0, 1, 1, 2, 3, 5, 5])
+    for k, (rods, world, steps, ys, fs) in enumerate(requests):
+        _assert_same(_split(results[k], rods), _alone_in_batch(rods, world, steps, ys, fs))
+    # And alone in a fresh server.
+    rods, world, steps, ys, fs = requests[3]
+    solo = rp.Server(PARAMS, hbm_budget_bytes=8 << 20, steps_per_launch=32)
+    r = _serve(solo, [requests[3]], arrivals=[0])[0]
+    for k in r:
+        assert np.array_equal(bits(r[k]), bits(results[3][k]))
+
+
+def test_horizon_is_exact_and_matches_fp32_oracle():
+    """A horizon that is not a multiple of the chunk: the rods stop exactly
+    there (checked against the FP32 oracle stepping that many steps)."""
+    rod = benchmark_rod(33)
+    world = _world(7, True)
+    server = rp.Server(PARAMS, hbm_budget_bytes=8 << 20, steps_per_launch=16)
+    rid = server.submit([rod], world, 37)
+    for _ in range(3):
+        server.step()
+    server.synchronize()
+    assert server.finished() == [rid]
+    assert server.info(rid)["steps_done"] == 37
+    got = server.result(rid)
+    o = oracle_twin(rod, 32)
+    o.step(oracle_params(PARAMS), oracle_world(world), 37)
+    d = compare_bitwise(got, oracle_state(o))
+    assert all(v == 0 for v in d.values()), d
+
+
+def test_admission_by_budget_and_reuse():
+    """A budget that holds one of these requests at a time: the second waits
+    for the first to be released, then runs in the freed ranges and still
+    gives its alone result."""
+    big = [benchmark_rod(120) for _ in range(6)]
+    world = _world(3, False)
+    budget = 250_000
+    server = rp.Server(PARAMS, hbm_budget_bytes=budget, steps_per_launch=64)
+    s = server.stats()
+    assert s["device_bytes"] <= budget * 1.01
+    a = server.submit(big, world, 64)
+    with pytest.raises(rp.ServerFull):
+        server.submit(big, world, 64)
+    server.step()
+    server.synchronize()
+    assert server.finished() == [a]
+    first = server.result(a)
+    server.release(a)
+    assert server.stats()["used_particles"] == 0
+    b = server.submit(big, world, 64)
+    server.step()
+    server.synchronize()
+    second = server.result(b)
+    for k in first:
+        assert np.array_equal(bits(first[k]), bits(second[k]))
+    with pytest.raises(rp.ServerFull):
+        server.submit([benchmark_rod(120)] * 40, world, 8)  # larger than the whole budget
