@@ -262,9 +262,11 @@ const char* rp_last_error(void);
  * collision world, described exactly as for rp_batch_create, plus an initial
  * state and a horizon in steps) are admitted while the device memory reserved
  * for them (an HBM budget fixed at creation) has room, stay resident between
- * calls, and advance together: each rp_server_step is one TILED launch of
- * steps_per_launch steps over the rods of every resident unfinished request,
- * every rod stopping at its own request's horizon. One step definition
+ * calls, and advance together: each rp_server_step launches steps_per_launch
+ * steps over the rods of every resident unfinished request (rods without
+ * self-collision in one TILED launch, rods with it in another on a second
+ * stream, the two sharing the device), every rod stopping at its own
+ * request's horizon. One step definition
  * (rp_step_params) serves every request. A request's result is bitwise the
  * one it gets alone: the per-rod arithmetic does not depend on its neighbours.
  *
@@ -275,7 +277,8 @@ typedef struct rp_server_desc {
     long long hbm_budget_bytes;  /* device memory for resident requests' state, constants and records */
     int max_rods;                /* rod records (0: one per 16 segments of the budget) */
     rp_step_params params;       /* the step every request takes */
-    int tile_config;             /* 0: tp2-tc1-w4 */
+    int tile_config;             /* 0: tp3-tc1-w4 for rods without self-collision, tp1-tc1-w8 for
+                                    rods with it; else this shape for both */
     int steps_per_launch;        /* steps per rp_server_step (0: 64) */
 } rp_server_desc;
 
@@ -310,8 +313,8 @@ typedef struct rp_server_stats {
     long long capacity_particles, used_particles;
     long long capacity_rods, used_rods;
     long long launches;
-    int workgroups;  /* of the last packing */
-    int tile_config;
+    int workgroups;  /* of the last packing, both launches */
+    int tile_config; /* shape of the rods without self-collision */
     int steps_per_launch;
 } rp_server_stats;
 

@@ -39,6 +39,7 @@ const TileConfig kConfigs[] = {
     {2, 1, 1, "tp2-tc1-w1"}, {3, 1, 1, "tp3-tc1-w1"}, {3, 2, 1, "tp3-tc2-w1"}, {5, 2, 1, "tp5-tc2-w1"},
     {2, 1, 2, "tp2-tc1-w2"}, {3, 1, 2, "tp3-tc1-w2"}, {3, 2, 2, "tp3-tc2-w2"}, {5, 2, 2, "tp5-tc2-w2"},
     {2, 1, 4, "tp2-tc1-w4"}, {3, 1, 4, "tp3-tc1-w4"}, {1, 1, 2, "tp1-tc1-w2"}, {1, 1, 4, "tp1-tc1-w4"},
+    {1, 1, 8, "tp1-tc1-w8"},
 };
 constexpr int kNumConfigs = int(sizeof(kConfigs) / sizeof(kConfigs[0]));
 

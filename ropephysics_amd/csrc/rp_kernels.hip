@@ -301,10 +301,11 @@ __host__ __device__ inline TiledLayout tiled_layout(int nP, int nF, int nS, int 
 // without, whose smaller register footprint keeps two waves per SIMD.
 //
 // Shapes with TP <= 2 are held to 256 registers, so two waves fit on a SIMD
-// (the self-collision variant spills a few bytes to do so).
+// (the self-collision variant spills a few bytes to do so); TP = 1 also runs
+// in 512-lane workgroups, for rods of up to 511 particles at one per lane.
 template <int TP, int TC, bool SELF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2 : 1))) void k_tiled_step(
-    DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slots) {
+__global__ __launch_bounds__(TP == 1 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2 : 1))) void
+k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slots) {
     extern __shared__ float4 smem4[];
     char* smem = reinterpret_cast<char*>(smem4);
     const Tile tile = tiles[blockIdx.x];

@@ -299,24 +299,28 @@ __device__ inline void self_query(const SelfLds& s, const float4* x4, const int*
 // (they contain barriers); `on` is false for lanes whose rod has no
 // self-collision.
 
-// Exclusive prefix sum of in[0, n) into out[0, n) by lanes l < G; returns the
-// total. Fixed barrier count whatever G, n or `on`.
+// Exclusive prefix sum of in[0, n) into out[0, n) by the first
+// min(G, kScanLanes) lanes of the group (each summing a chunk); returns the
+// total. Fixed barrier count whatever G, n or `on`; lanes past kScanLanes only
+// take part in the barriers.
 __device__ inline int group_scan(const int* in, int* out, int n, int* scratch, int l, int G, bool on) {
-    const int chunk = (n + G - 1) / max(G, 1);
-    const int lo = on ? min(n, l * chunk) : 0;
-    const int hi = on ? min(n, lo + chunk) : 0;
+    const int P = min(G, kScanLanes);
+    const bool part = on && l < P;
+    const int chunk = (n + P - 1) / max(P, 1);
+    const int lo = part ? min(n, l * chunk) : 0;
+    const int hi = part ? min(n, lo + chunk) : 0;
     int sum = 0;
     for (int i = lo; i < hi; ++i) sum += in[i];
-    if (on) scratch[l] = sum;
+    if (part) scratch[l] = sum;
     __syncthreads();
     for (int off = 1; off < kScanLanes; off <<= 1) {
-        const int v = (on && l >= off) ? scratch[l - off] : 0;
+        const int v = (part && l >= off) ? scratch[l - off] : 0;
         __syncthreads();
-        if (on) scratch[l] += v;
+        if (part) scratch[l] += v;
         __syncthreads();
     }
-    const int total = on ? scratch[G - 1] : 0;
-    int run = on ? scratch[l] - sum : 0;
+    const int total = on ? scratch[P - 1] : 0;
+    int run = part ? scratch[l] - sum : 0;
     for (int i = lo; i < hi; ++i) {
         const int v = in[i];
         out[i] = run;

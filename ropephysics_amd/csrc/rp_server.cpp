@@ -105,10 +105,16 @@ struct DevArray {
 
 struct rp_server_impl {
     rp_step_params params{};
-    int configId = 9;
+    // Rods without self-collision run the lean kernel, the others the
+    // self-collision variant, each packed in its own tile shape and launched
+    // on its own stream so the two launches share the device.
+    int configLean = 10;  // tp3-tc1-w4
+    int configSelf = 13;  // tp1-tc1-w8
     int stepsPerLaunch = 64;
     LdsLimits lim;
     hipStream_t stream = nullptr;
+    hipStream_t selfStream = nullptr;
+    hipEvent_t evReady = nullptr, evSelfDone = nullptr;
     RangePool pools[kNumPools];
     std::vector<std::unique_ptr<DevArray>> owned;
     long long deviceBytes = 0;
@@ -123,11 +129,14 @@ struct rp_server_impl {
     std::vector<int> finished;        // ids whose horizon is reached, not yet released
     int nextId = 1;
     bool dirty = true;  // membership changed since the last packing
-    Packing packing;
+    Packing packLean, packSelf;
     long long launches = 0;
     long long stepsTaken = 0;
 
     ~rp_server_impl() {
+        if (evReady) (void)hipEventDestroy(evReady);
+        if (evSelfDone) (void)hipEventDestroy(evSelfDone);
+        if (selfStream) (void)hipStreamDestroy(selfStream);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -180,7 +189,7 @@ int rp_server_create(const rp_server_desc* desc, void** out_server) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(RP_ERR_DEVICE, "no usable HIP device");
     auto s = std::make_unique<rp_server_impl>();
     s->params = desc->params;
-    if (desc->tile_config > 0) s->configId = desc->tile_config;
+    if (desc->tile_config > 0) s->configLean = s->configSelf = desc->tile_config;
     if (desc->steps_per_launch > 0) s->stepsPerLaunch = desc->steps_per_launch;
     int rc = lds_limits(s->lim);
     if (rc) return rc;
@@ -199,6 +208,9 @@ int rp_server_create(const rp_server_desc* desc, void** out_server) {
     for (int k = 0; k < kNumPools; ++k) s->pools[k].reset(cap[k]);
 
     RP_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    RP_HIP(hipStreamCreateWithFlags(&s->selfStream, hipStreamNonBlocking));
+    RP_HIP(hipEventCreateWithFlags(&s->evReady, hipEventDisableTiming));
+    RP_HIP(hipEventCreateWithFlags(&s->evSelfDone, hipEventDisableTiming));
     DeviceArrays& d = s->d;
     float *x, *v, *q, *om, *force, *kin, *torque, *invMass, *cs, *fs;
     float4 *iI4, *I4, *sPar, *bPar, *bRest;
@@ -214,7 +226,7 @@ int rp_server_create(const rp_server_desc* desc, void** out_server) {
         (rc = s->alloc(segEnd, 2 * St)) || (rc = s->alloc(bIdx, B)) || (rc = s->alloc(bPar, B)) ||
         (rc = s->alloc(bRest, B)) || (rc = s->alloc(dRods, R)) || (rc = s->alloc(cs, R)) ||
         (rc = s->alloc(fs, R)) || (rc = s->alloc(ovf, R)) || (rc = s->alloc(clock, R)) ||
-        (rc = s->alloc(s->stepLimit, R)) || (rc = s->alloc(s->dTiles, R)) || (rc = s->alloc(s->dSlots, R)) ||
+        (rc = s->alloc(s->stepLimit, R)) || (rc = s->alloc(s->dTiles, 2 * R)) || (rc = s->alloc(s->dSlots, 2 * R)) ||
         (rc = s->alloc(dPrims, cap[kPrims])) || (rc = s->alloc(col, cap[kColours])))
         return rc;
     d.x = x;
@@ -267,9 +279,9 @@ int rp_server_submit(void* server, const rp_batch_desc* req, const rp_request_st
     Records rec;
     int rc = build_records(req, rec);
     if (rc) return rc;
-    const TileConfig& cfg = tile_config(s.configId);
     for (int r = 0; r < rec.R; ++r) {
         const RodInfo& ri = rec.rods[r];
+        const TileConfig& cfg = tile_config(ri.self ? s.configSelf : s.configLean);
         if (!rec.colourFit[r]) return fail(RP_ERR_CAPACITY, "rod " + std::to_string(r) + " needs more than two colours");
         if (size_t(rod_tiled_lds(ri)) > s.lim.perWorkgroup)
             return fail(RP_ERR_CAPACITY, "rod " + std::to_string(r) + " needs more LDS than a workgroup has");
@@ -364,34 +376,54 @@ int rp_server_step(void* server, int* num_finished) {
     if (!server) return fail(RP_ERR_INVALID, "null server");
     rp_server_impl& s = *S(server);
     if (num_finished) *num_finished = 0;
-    std::vector<int> ids;
-    bool anySelf = false;
+    std::vector<int> lean, self;
     for (const auto& kv : s.requests) {
         const Request& q = kv.second;
         if (q.stepsDone >= q.steps) continue;
-        for (long long r = q.start[kRods]; r < q.start[kRods] + q.count[kRods]; ++r) {
-            ids.push_back(int(r));
-            anySelf = anySelf || s.selfOf[size_t(r)];
-        }
+        for (long long r = q.start[kRods]; r < q.start[kRods] + q.count[kRods]; ++r)
+            (s.selfOf[size_t(r)] ? self : lean).push_back(int(r));
     }
-    if (ids.empty()) return RP_OK;
-    const TileConfig& cfg = tile_config(s.configId);
+    if (lean.empty() && self.empty()) return RP_OK;
+    const long long R = s.pools[kRods].capacity();
     if (s.dirty) {
         // Two workgroups per compute unit when every rod fits that budget.
-        int rc = pack_rods(s.rods, ids, s.colStart, s.configId, lds_budget(s.lim, 2), s.packing);
-        if (rc) rc = pack_rods(s.rods, ids, s.colStart, s.configId, lds_budget(s.lim, 1), s.packing);
-        if (rc) return fail(rc, "resident rods do not pack");
-        RP_HIP(hipMemcpyAsync(s.dTiles, s.packing.tiles.data(), s.packing.tiles.size() * sizeof(Tile),
-                              hipMemcpyHostToDevice, s.stream));
-        RP_HIP(hipMemcpyAsync(s.dSlots, s.packing.slots.data(), s.packing.slots.size() * sizeof(TileSlot),
-                              hipMemcpyHostToDevice, s.stream));
+        Packing* packs[2] = {&s.packLean, &s.packSelf};
+        const std::vector<int>* ids[2] = {&lean, &self};
+        const int cfgs[2] = {s.configLean, s.configSelf};
+        for (int k = 0; k < 2; ++k) {
+            packs[k]->tiles.clear();
+            packs[k]->slots.clear();
+            if (ids[k]->empty()) continue;
+            int rc = pack_rods(s.rods, *ids[k], s.colStart, cfgs[k], lds_budget(s.lim, 2), *packs[k]);
+            if (rc) rc = pack_rods(s.rods, *ids[k], s.colStart, cfgs[k], lds_budget(s.lim, 1), *packs[k]);
+            if (rc) return fail(rc, "resident rods do not pack");
+            RP_HIP(hipMemcpyAsync(s.dTiles + k * R, packs[k]->tiles.data(), packs[k]->tiles.size() * sizeof(Tile),
+                                  hipMemcpyHostToDevice, s.stream));
+            RP_HIP(hipMemcpyAsync(s.dSlots + k * R, packs[k]->slots.data(),
+                                  packs[k]->slots.size() * sizeof(TileSlot), hipMemcpyHostToDevice, s.stream));
+        }
         RP_HIP(hipStreamSynchronize(s.stream));
         s.dirty = false;
     }
     const StepArgs a = step_args(s.params, s.stepsPerLaunch);
-    RP_HIP(launch_tiled(cfg.tp, cfg.tc, anySelf, s.d, a, s.dTiles, s.dSlots, int(s.packing.tiles.size()),
-                        64 * cfg.waves, size_t(s.packing.maxLds), s.stream));
-    ++s.launches;
+    if (!self.empty()) {
+        // Uploads and tiles are on the main stream: the second stream starts
+        // after them, and the main stream continues after its launch.
+        const TileConfig& cfg = tile_config(s.configSelf);
+        RP_HIP(hipEventRecord(s.evReady, s.stream));
+        RP_HIP(hipStreamWaitEvent(s.selfStream, s.evReady, 0));
+        RP_HIP(launch_tiled(cfg.tp, cfg.tc, true, s.d, a, s.dTiles + R, s.dSlots + R,
+                            int(s.packSelf.tiles.size()), 64 * cfg.waves, size_t(s.packSelf.maxLds), s.selfStream));
+        RP_HIP(hipEventRecord(s.evSelfDone, s.selfStream));
+        ++s.launches;
+    }
+    if (!lean.empty()) {
+        const TileConfig& cfg = tile_config(s.configLean);
+        RP_HIP(launch_tiled(cfg.tp, cfg.tc, false, s.d, a, s.dTiles, s.dSlots, int(s.packLean.tiles.size()),
+                            64 * cfg.waves, size_t(s.packLean.maxLds), s.stream));
+        ++s.launches;
+    }
+    if (!self.empty()) RP_HIP(hipStreamWaitEvent(s.stream, s.evSelfDone, 0));
     s.stepsTaken += s.stepsPerLaunch;
     int done = 0;
     for (auto& kv : s.requests) {
@@ -488,8 +520,8 @@ int rp_server_get_stats(void* server, rp_server_stats* out) {
     out->used_particles = s.pools[kParticles].used();
     out->used_rods = s.pools[kRods].used();
     out->launches = s.launches;
-    out->workgroups = int(s.packing.tiles.size());
-    out->tile_config = s.configId;
+    out->workgroups = int(s.packLean.tiles.size() + s.packSelf.tiles.size());
+    out->tile_config = s.configLean;
     out->steps_per_launch = s.stepsPerLaunch;
     return RP_OK;
 }

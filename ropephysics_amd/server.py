@@ -3,10 +3,12 @@
 Requests (rods with their own collision world, an initial state, per-rod
 stiffness and friction scales and a horizon in steps) are admitted while the
 server's HBM budget has room, stay resident on the device between calls, and
-advance together: every ``step()`` is one launch over the rods of every running
-request, each rod stopping at its own horizon. There is no counterpart in the
-reference, whose callers build one ``Batch`` per workload; the arithmetic is
-the batch's, so a request's result equals the one it gets alone, bit for bit.
+advance together: every ``step()`` launches one chunk of steps over the rods
+of every running request (those without self-collision and those with it in
+two launches that share the device), each rod stopping at its own horizon.
+There is no counterpart in the reference, whose callers build one ``Batch``
+per workload; the arithmetic is the batch's, so a request's result equals the
+one it gets alone, bit for bit.
 """
 from __future__ import annotations
 
@@ -18,9 +20,11 @@ from . import _abi
 from .batch import _f32, describe, step_params
 from .rod import SolverParams
 
-# Default tile shape of the server: two particles per lane, 256-lane
-# workgroups (rods of up to 511 segments).
-DEFAULT_TILE_CONFIG = 9
+# Tile shapes the server uses by default: three particles per lane in 256-lane
+# workgroups for rods without self-collision, one per lane in 512-lane
+# workgroups for the others.
+DEFAULT_TILE_CONFIG = 13
+DEFAULT_LEAN_TILE_CONFIG = 10
 
 
 class ServerFull(RuntimeError):
