@@ -12,7 +12,11 @@ launch shape) is timed over the same steps of the same workload beside it; the
 reference CPU path runs a bounded sample of it.
 
 Secondary: the uniform batch of 4096 rods x 64 segments over a floor plane with
-per-rod stiffness and friction, 1000 steps, TILED and PLAIN.
+per-rod stiffness and friction, 1000 steps, TILED and PLAIN; and the serving
+stream (requests of 1-256 rods of 32-256 segments, 0.5-8 s horizons, random
+contact shapes, self-collision in half) through the continuous-batching
+server: its capacity with every request arriving at once, then p50/p99 latency
+under Poisson arrivals at a fraction of that capacity.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -196,6 +200,45 @@ def cpu_baseline_routing(w: scenes.Workload, stride: int, steps: int, threads: i
             "seconds": dt}
 
 
+# ------------------------------------------------------------------ serving stream (latency)
+
+def serving_leg(num: int, load: float, budget_gib: float) -> dict:
+    """The serving workload: `num` requests, first all at once (the server's
+    capacity in requests/s), then as a Poisson stream at `load` times that
+    rate; latency from arrival to result on the host."""
+    from ropephysics_amd import serving
+
+    reqs = serving.make_requests(num)
+    work = sum(q.work for q in reqs)
+    server = rp.Server(serving.PARAMS, hbm_budget_bytes=int(budget_gib * (1 << 30)))
+    serving.run_stream(server, serving.make_requests(2, seed=5), np.zeros(2))  # code objects, first launches
+    sat = serving.run_stream(server, reqs, np.zeros(num))
+    capacity = num / sat["wall"]
+    rate = load * capacity
+    run = serving.run_stream(server, reqs, serving.poisson_arrivals(num, rate))
+    lat = run["latency"]
+    horizon = np.array([q.steps * serving.PARAMS.dt for q in reqs])
+    out = {
+        "requests": num, "segments_mean": float(np.mean([q.segments for q in reqs])),
+        "rods_mean": float(np.mean([q.info["rods"] for q in reqs])),
+        "horizon_s_mean": float(horizon.mean()), "self_collision_share": float(np.mean(
+            [q.info["self_collision"] for q in reqs])),
+        "hbm_budget_bytes": server.stats()["device_bytes"],
+        "saturated": {"wall_s": sat["wall"], "value": work / sat["wall"], "unit": "segment-substeps/s",
+                      "requests_per_s": capacity, "launches": sat["launches"]},
+        "poisson": {"rate_requests_per_s": rate, "load": load, "wall_s": run["wall"], "value": work / run["wall"],
+                    "unit": "segment-substeps/s", "p50_latency_s": float(np.percentile(lat, 50)),
+                    "p99_latency_s": float(np.percentile(lat, 99)), "max_latency_s": float(lat.max()),
+                    "p50_latency_over_horizon": float(np.percentile(lat / horizon, 50)),
+                    "max_resident_requests": run["max_resident"], "launches": run["launches"]},
+    }
+    log(f"serving: {num} requests; saturated {work / sat['wall'] / 1e9:.3f} G seg-substeps/s "
+        f"({capacity:.2f} req/s); Poisson at {rate:.2f} req/s: p50 {out['poisson']['p50_latency_s']:.3f} s, "
+        f"p99 {out['poisson']['p99_latency_s']:.3f} s")
+    server.destroy()
+    return out
+
+
 # ------------------------------------------------------------------ uniform batch (secondary)
 
 def uniform_leg(steps: int, warmup: int, no_plain: bool) -> dict:
@@ -253,6 +296,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6000, help="routing steps of the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=40, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--serve-load", type=float, default=0.8, help="Poisson rate as a fraction of capacity")
+    ap.add_argument("--serve-budget-gib", type=float, default=4.0, help="server HBM budget")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-traffic kernel for "
                          "scaling hardware byte counters)")
@@ -291,6 +337,8 @@ def main():
                     "routing" if not args.tile_config else None)
 
     uniform = uniform_leg(args.uniform_steps, W, args.no_plain) if args.uniform_steps > 0 else None
+    serving_out = serving_leg(args.serve_requests, args.serve_load, args.serve_budget_gib) \
+        if args.serve_requests > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -320,6 +368,9 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "uniform": uniform,
+        "serving": serving_out,
+        "p50_latency_s": serving_out["poisson"]["p50_latency_s"] if serving_out else None,
+        "p99_latency_s": serving_out["poisson"]["p99_latency_s"] if serving_out else None,
         "device": {"arch": info["arch"], "compute_units": info["compute_units"]},
     }
     print(json.dumps(out))
