@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libropephysics.so")
-SOURCES = ["rp_kernels.hip", "rp_host.cpp", "rp_batch.cpp", "rp_server.cpp"]
+SOURCES = ["rp_kernels.hip", "rp_kernels64.hip", "rp_host.cpp", "rp_batch.cpp", "rp_server.cpp"]
 ARCH = os.environ.get("RP_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -14,15 +14,36 @@
 
 namespace rp {
 
-// Kernels and their launch wrappers (rp_kernels.hip).
-int plain_lds_bytes(int nP, int nF, int nS, int nB, int nPrim, int self, int table, int cap);
-int tiled_lds_bytes(int nP, int nF, int nS, int nPrim, int self, int table, int cap);
-hipError_t launch_plain(const DeviceArrays& d, const StepArgs& sa, size_t lds, hipStream_t s);
-hipError_t allow_dynamic_lds(int bytes, int* granted);
-hipError_t launch_tiled(int tp, int tc, bool self, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
-                        const TileSlot* slots, int numTiles, int threads, size_t lds, hipStream_t s);
-hipError_t launch_fixed_frames(float* q, const float* src, const float* invInertia4, int f0, int count,
-                               hipStream_t s);
+// Kernels and their launch wrappers (rp_kernels.hip), compiled once per
+// precision: rp::f32 (rp_kernels.hip) and rp::f64 (rp_kernels64.hip).
+#define RP_KERNEL_API(R)                                                                                       \
+    int plain_lds_bytes(int nP, int nF, int nS, int nB, int nPrim, int self, int table, int cap);              \
+    int tiled_lds_bytes(int nP, int nF, int nS, int nPrim, int self, int table, int cap);                      \
+    hipError_t launch_plain(const DeviceArraysT<R>& d, const StepArgsT<R>& sa, size_t lds, hipStream_t s);      \
+    hipError_t allow_dynamic_lds(int bytes, int* granted);                                                     \
+    hipError_t launch_tiled(int tp, int tc, bool self, const DeviceArraysT<R>& d, const StepArgsT<R>& sa,      \
+                            const Tile* tiles, const TileSlot* slots, int numTiles, int threads, size_t lds,   \
+                            hipStream_t s);                                                                    \
+    hipError_t launch_fixed_frames(R* q, const R* src, const R* invInertia4, int f0, int count, hipStream_t s);
+namespace f32 {
+RP_KERNEL_API(float)
+}
+namespace f64 {
+RP_KERNEL_API(double)
+}
+#undef RP_KERNEL_API
+
+// The host side below works on FP32 records.
+using f32::DeviceArrays;
+using f32::Prim;
+using f32::RodInfo;
+using f32::StepArgs;
+using f32::allow_dynamic_lds;
+using f32::launch_fixed_frames;
+using f32::launch_plain;
+using f32::launch_tiled;
+using f32::plain_lds_bytes;
+using f32::tiled_lds_bytes;
 
 namespace host {
 

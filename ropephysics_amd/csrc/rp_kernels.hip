@@ -23,6 +23,11 @@
 #include "rp_phases.h"
 
 namespace rp {
+namespace RP_PREC {
+
+// Bytes of a real and of a 4-vector of reals in LDS.
+constexpr int kR = int(sizeof(real));
+constexpr int kR4 = int(sizeof(real4));
 
 // ------------------------------------------------------------------ PLAIN
 
@@ -36,43 +41,43 @@ __host__ __device__ inline PlainLayout plain_layout(int nP, int nF, int nS, int 
     PlainLayout L;
     int o = 0;
     L.x = o;
-    o += nP * 16;
+    o += nP * kR4;
     L.xp = o;
-    o += nP * 16;
+    o += nP * kR4;
     L.v = o;
-    o += nP * 16;
+    o += nP * kR4;
     L.f = o;
-    o += nP * 16;
+    o += nP * kR4;
     L.kv = o;
-    o += nP * 16;
+    o += nP * kR4;
     L.q = o;
-    o += nF * 16;
+    o += nF * kR4;
     L.qp = o;
-    o += nF * 16;
+    o += nF * kR4;
     L.om = o;
-    o += nF * 16;
+    o += nF * kR4;
     L.tq = o;
-    o += nF * 16;
+    o += nF * kR4;
     L.iI = o;
-    o += nF * 16;
+    o += nF * kR4;
     L.I = o;
-    o += nF * 16;
+    o += nF * kR4;
     L.lamS = o;
-    o += nS * 16;
+    o += nS * kR4;
     L.lamB = o;
-    o += nB * 16;
+    o += nB * kR4;
     L.cont = o;
-    o += nP * nPrim * kContactFloats * 4;
+    o += nP * nPrim * kContactFloats * kR;
     L.prims = o;
-    o += nPrim * kPrimFloats * 4;
+    o += nPrim * kPrimFloats * kR;
     L.self = o;
     if (self) o += self_lds_bytes(nS, table, cap);
-    L.total = (o + 15) & ~15;
+    L.total = (o + kR4 - 1) & ~(kR4 - 1);
     return L;
 }
 
 __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa) {
-    extern __shared__ float4 smem4[];
+    extern __shared__ real4 smem4[];
     char* smem = reinterpret_cast<char*>(smem4);
     const int r = blockIdx.x;
     if (r >= d.numRods) return;
@@ -81,55 +86,55 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nP = ri.nP, nF = ri.nF, nS = ri.nS, nB = ri.nB, nPrim = ri.nPrim;
     const PlainLayout L = plain_layout(nP, nF, nS, nB, nPrim, ri.self, ri.table, ri.selfCap);
-    float4* x4 = reinterpret_cast<float4*>(smem + L.x);
-    float4* xp4 = reinterpret_cast<float4*>(smem + L.xp);
-    float4* v4 = reinterpret_cast<float4*>(smem + L.v);
-    float4* f4 = reinterpret_cast<float4*>(smem + L.f);
-    float4* kv4 = reinterpret_cast<float4*>(smem + L.kv);
-    float4* q4s = reinterpret_cast<float4*>(smem + L.q);
-    float4* qp4 = reinterpret_cast<float4*>(smem + L.qp);
-    float4* om4 = reinterpret_cast<float4*>(smem + L.om);
-    float4* tq4 = reinterpret_cast<float4*>(smem + L.tq);
-    float4* iI4 = reinterpret_cast<float4*>(smem + L.iI);
-    float4* I4 = reinterpret_cast<float4*>(smem + L.I);
-    float4* lamS = reinterpret_cast<float4*>(smem + L.lamS);
-    float4* lamB = reinterpret_cast<float4*>(smem + L.lamB);
-    float* cont = reinterpret_cast<float*>(smem + L.cont);
-    float* prims = reinterpret_cast<float*>(smem + L.prims);
+    real4* x4 = reinterpret_cast<real4*>(smem + L.x);
+    real4* xp4 = reinterpret_cast<real4*>(smem + L.xp);
+    real4* v4 = reinterpret_cast<real4*>(smem + L.v);
+    real4* f4 = reinterpret_cast<real4*>(smem + L.f);
+    real4* kv4 = reinterpret_cast<real4*>(smem + L.kv);
+    real4* q4s = reinterpret_cast<real4*>(smem + L.q);
+    real4* qp4 = reinterpret_cast<real4*>(smem + L.qp);
+    real4* om4 = reinterpret_cast<real4*>(smem + L.om);
+    real4* tq4 = reinterpret_cast<real4*>(smem + L.tq);
+    real4* iI4 = reinterpret_cast<real4*>(smem + L.iI);
+    real4* I4 = reinterpret_cast<real4*>(smem + L.I);
+    real4* lamS = reinterpret_cast<real4*>(smem + L.lamS);
+    real4* lamB = reinterpret_cast<real4*>(smem + L.lamB);
+    real* cont = reinterpret_cast<real*>(smem + L.cont);
+    real* prims = reinterpret_cast<real*>(smem + L.prims);
     SelfLds sl;
     if (ri.self) sl = self_carve(smem + L.self, nS, ri.table, ri.selfCap);
 
     const int p0 = ri.pStart, f0 = ri.fStart;
     for (int i = tid; i < nP; i += nt) {
         x4[i] = vf4(ld3(d.x, p0 + i), d.invMass[p0 + i]);
-        v4[i] = vf4(ld3(d.v, p0 + i), 0.0f);
-        f4[i] = vf4(ld3(d.force, p0 + i), 0.0f);
-        kv4[i] = vf4(ld3(d.kinVel, p0 + i), 0.0f);
+        v4[i] = vf4(ld3(d.v, p0 + i), real(0.0));
+        f4[i] = vf4(ld3(d.force, p0 + i), real(0.0));
+        kv4[i] = vf4(ld3(d.kinVel, p0 + i), real(0.0));
     }
     for (int j = tid; j < nF; j += nt) {
         q4s[j] = qf4(ld4q(d.q, f0 + j));
-        om4[j] = vf4(ld3(d.omega, f0 + j), 0.0f);
-        tq4[j] = vf4(ld3(d.torque, f0 + j), 0.0f);
-        iI4[j] = reinterpret_cast<const float4*>(d.invInertia4)[f0 + j];
-        I4[j] = reinterpret_cast<const float4*>(d.inertia4)[f0 + j];
+        om4[j] = vf4(ld3(d.omega, f0 + j), real(0.0));
+        tq4[j] = vf4(ld3(d.torque, f0 + j), real(0.0));
+        iI4[j] = reinterpret_cast<const real4*>(d.invInertia4)[f0 + j];
+        I4[j] = reinterpret_cast<const real4*>(d.inertia4)[f0 + j];
     }
     for (int k = tid; k < nPrim; k += nt) stage_prim(prims + k * kPrimFloats, d.prims[ri.primOff + k]);
     if (ri.self && tid == 0) {
         sl.misc[0] = 0;
         sl.misc[1] = 0;
     }
-    const float cscale = d.complianceScale[r];
-    const float fscale = d.frictionScale[r];
+    const real cscale = d.complianceScale[r];
+    const real fscale = d.frictionScale[r];
     const int* colS = d.colStart + ri.colOff;
     const int* colB = colS + ri.nSCol + 1;
     const int4* sIdx = reinterpret_cast<const int4*>(d.sIdx) + ri.sStart;
-    const float4* sPar = reinterpret_cast<const float4*>(d.sPar) + ri.sStart;
+    const real4* sPar = reinterpret_cast<const real4*>(d.sPar) + ri.sStart;
     const int4* bIdx = reinterpret_cast<const int4*>(d.bIdx) + ri.bStart;
-    const float4* bPar = reinterpret_cast<const float4*>(d.bPar) + ri.bStart;
-    const float4* bRst = reinterpret_cast<const float4*>(d.bRest4) + ri.bStart;
+    const real4* bPar = reinterpret_cast<const real4*>(d.bPar) + ri.bStart;
+    const real4* bRst = reinterpret_cast<const real4*>(d.bRest4) + ri.bStart;
     const int* segEnd = d.segEnd + 2 * ri.sStart;
     const v3 g = mk(sa.gx, sa.gy, sa.gz);
-    const float h = sa.h;
+    const real h = sa.h;
     int clk = d.clock[r];
     const int drivenP = d.paths ? d.paths[r].particle : -1;
     const int obsP = d.obs ? d.obs[r].particle : -1;
@@ -137,11 +142,11 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
     __syncthreads();
 
     for (int step = 0; step < sa.steps; ++step) {
-        if (drivenP >= 0 && drivenP % nt == tid) kv4[drivenP] = vf4(path_velocity(d.paths[r], clk), 0.0f);
+        if (drivenP >= 0 && drivenP % nt == tid) kv4[drivenP] = vf4(path_velocity(d.paths[r], clk), real(0.0));
         for (int sub = 0; sub < sa.substeps; ++sub) {
             const bool regen = sub % sa.contactInterval == 0;
-            for (int k = tid; k < nS; k += nt) lamS[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            for (int k = tid; k < nB; k += nt) lamB[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int k = tid; k < nS; k += nt) lamS[k] = make_real4(real(0.), real(0.), real(0.), real(0.));
+            for (int k = tid; k < nB; k += nt) lamB[k] = make_real4(real(0.), real(0.), real(0.), real(0.));
             if (nPrim > 0)
                 for (int i = tid; i < nP; i += nt)
                     world_contacts_begin(cont + i * nPrim * kContactFloats, prims, nPrim, f4v(x4[i]), x4[i].w,
@@ -166,10 +171,10 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
             }
             for (int i = tid; i < nP; i += nt) {
                 v3 x = f4v(x4[i]), v = f4v(v4[i]), xp;
-                const float w = x4[i].w;
+                const real w = x4[i].w;
                 predict_particle(x, v, xp, w, f4v(f4[i]), f4v(kv4[i]), h, g);
                 x4[i] = vf4(x, w);
-                v4[i] = vf4(v, 0.0f);
+                v4[i] = vf4(v, real(0.0));
                 xp4[i] = vf4(xp, w);
             }
             for (int j = tid; j < nF; j += nt) {
@@ -178,7 +183,7 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
                 predict_frame(q, om, qp, f4v(iI4[j]), f4v(I4[j]), f4v(tq4[j]), h);
                 q4s[j] = qf4(q);
                 qp4[j] = qf4(qp);
-                om4[j] = vf4(om, 0.0f);
+                om4[j] = vf4(om, real(0.0));
             }
             __syncthreads();
 
@@ -186,9 +191,9 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
                 for (int c = 0; c < ri.nSCol; ++c) {
                     for (int k = colS[c] + tid; k < colS[c + 1]; k += nt) {
                         const int4 id = sIdx[k];
-                        const float4 par = sPar[k];
+                        const real4 par = sPar[k];
                         v3 xa = f4v(x4[id.x]), xb = f4v(x4[id.y]);
-                        const float wa = x4[id.x].w, wb = x4[id.y].w;
+                        const real wa = x4[id.x].w, wb = x4[id.y].w;
                         q4 q = f4q(q4s[id.z]);
                         v3 lam = f4v(lamS[k]);
                         if (project_stretch(xa, xb, q, lam, wa, wb, f4v(iI4[id.z]), par.x, mk(par.y, par.z, par.w),
@@ -196,7 +201,7 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
                             x4[id.x] = vf4(xa, wa);
                             x4[id.y] = vf4(xb, wb);
                             q4s[id.z] = qf4(q);
-                            lamS[k] = vf4(lam, 0.0f);
+                            lamS[k] = vf4(lam, real(0.0));
                         }
                     }
                     __syncthreads();
@@ -204,14 +209,14 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
                 for (int c = 0; c < ri.nBCol; ++c) {
                     for (int k = colB[c] + tid; k < colB[c + 1]; k += nt) {
                         const int4 id = bIdx[k];
-                        const float4 par = bPar[k];
+                        const real4 par = bPar[k];
                         q4 qa = f4q(q4s[id.x]), qb = f4q(q4s[id.y]);
                         v3 lam = f4v(lamB[k]);
                         if (project_bend(qa, qb, lam, f4v(iI4[id.x]), f4v(iI4[id.y]), par.x, mk(par.y, par.z, par.w),
                                          f4v(bRst[k]), sa.invH2, cscale)) {
                             q4s[id.x] = qf4(qa);
                             q4s[id.y] = qf4(qb);
-                            lamB[k] = vf4(lam, 0.0f);
+                            lamB[k] = vf4(lam, real(0.0));
                         }
                     }
                     __syncthreads();
@@ -219,8 +224,8 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
                 if (nPrim > 0) {
                     for (int i = tid; i < nP; i += nt) {
                         v3 x = f4v(x4[i]);
-                        const float w = x4[i].w;
-                        if (w <= 0.0f) continue;
+                        const real w = x4[i].w;
+                        if (w <= real(0.0)) continue;
                         world_contacts_project(cont + i * nPrim * kContactFloats, nPrim, x, f4v(xp4[i]), w);
                         x4[i] = vf4(x, w);
                     }
@@ -234,11 +239,11 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
 
             for (int i = tid; i < nP; i += nt) {
                 const v3 v = finish_particle(f4v(x4[i]), f4v(xp4[i]), x4[i].w, f4v(kv4[i]), h, sa.linDecay);
-                v4[i] = vf4(v, 0.0f);
+                v4[i] = vf4(v, real(0.0));
             }
             for (int j = tid; j < nF; j += nt) {
                 const v3 om = finish_frame(f4q(q4s[j]), f4q(qp4[j]), f4v(iI4[j]), h, sa.angDecay);
-                om4[j] = vf4(om, 0.0f);
+                om4[j] = vf4(om, real(0.0));
             }
             __syncthreads();
         }
@@ -273,24 +278,24 @@ __host__ __device__ inline TiledLayout tiled_layout(int nP, int nF, int nS, int 
     TiledLayout L;
     int o = 0;
     L.x = o;
-    o += nP * 16;
+    o += nP * kR4;
     L.q = o;
-    o += nF * 16;
+    o += nF * kR4;
     L.f = o;
-    o += nP * 16;
+    o += nP * kR4;
     L.kv = o;
-    o += nP * 16;
+    o += nP * kR4;
     L.tq = o;
-    o += nF * 16;
+    o += nF * kR4;
     L.cont = o;
-    o += nP * nPrim * kContactFloats * 4;
+    o += nP * nPrim * kContactFloats * kR;
     L.prims = o;
-    o += nPrim * kPrimFloats * 4;
+    o += nPrim * kPrimFloats * kR;
     L.xp = o;
-    if (self) o += nP * 16;
+    if (self) o += nP * kR4;
     L.self = o;
     if (self) o += self_lds_bytes(nS, table, cap);
-    L.total = (o + 15) & ~15;
+    L.total = (o + kR4 - 1) & ~(kR4 - 1);
     return L;
 }
 
@@ -300,13 +305,18 @@ __host__ __device__ inline TiledLayout tiled_layout(int nP, int nF, int nS, int 
 // SELF: compiled with rod self-collision; batches without it run the variant
 // without, whose smaller register footprint keeps two waves per SIMD.
 //
-// Shapes with TP <= 2 are held to 256 registers, so two waves fit on a SIMD
-// (the self-collision variant spills a few bytes to do so); TP = 1 also runs
-// in 512-lane workgroups, for rods of up to 511 particles at one per lane.
+// In FP32, shapes with TP <= 2 are held to 256 registers, so two waves fit
+// on a SIMD (the self-collision variant spills a few bytes to do so), and TP
+// = 1 also runs in 512-lane workgroups, for rods of up to 511 particles at
+// one per lane. In FP64, where every real takes two registers, workgroups
+// have at most 256 lanes and a wave may use the whole register file.
+constexpr int tiled_max_lanes(int tp) { return (!RP_FP64 && tp == 1) ? 512 : 256; }
+constexpr int tiled_waves_per_simd(int tp) { return (!RP_FP64 && tp <= 2) ? 2 : 1; }
+
 template <int TP, int TC, bool SELF>
-__global__ __launch_bounds__(TP == 1 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(TP <= 2 ? 2 : 1))) void
+__global__ __launch_bounds__(tiled_max_lanes(TP)) __attribute__((amdgpu_waves_per_eu(tiled_waves_per_simd(TP)))) void
 k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slots) {
-    extern __shared__ float4 smem4[];
+    extern __shared__ real4 smem4[];
     char* smem = reinterpret_cast<char*>(smem4);
     const Tile tile = tiles[blockIdx.x];
     const int lane = threadIdx.x;
@@ -334,21 +344,21 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
         ri.selfCap = 0;
         ri.pStart = ri.fStart = ri.sStart = ri.bStart = 0;
         ri.colOff = 0;
-        ri.radius = ri.margin = ri.selfFriction = 0.0f;
+        ri.radius = ri.margin = ri.selfFriction = real(0.0);
         ri.selfGap = 1;
         ri.primOff = 0;
     }
     const int nP = ri.nP, nF = ri.nF, nS = ri.nS, nPrim = ri.nPrim;
     const TiledLayout L = tiled_layout(nP, nF, nS, nPrim, ri.self, ri.table, ri.selfCap);
     char* base = smem + ldsOff;
-    float4* x4 = reinterpret_cast<float4*>(base + L.x);
-    float4* q4s = reinterpret_cast<float4*>(base + L.q);
-    float4* f4 = reinterpret_cast<float4*>(base + L.f);
-    float4* kv4 = reinterpret_cast<float4*>(base + L.kv);
-    float4* tq4 = reinterpret_cast<float4*>(base + L.tq);
-    float* cont = reinterpret_cast<float*>(base + L.cont);
-    float* prims = reinterpret_cast<float*>(base + L.prims);
-    float4* xp4 = reinterpret_cast<float4*>(base + L.xp);
+    real4* x4 = reinterpret_cast<real4*>(base + L.x);
+    real4* q4s = reinterpret_cast<real4*>(base + L.q);
+    real4* f4 = reinterpret_cast<real4*>(base + L.f);
+    real4* kv4 = reinterpret_cast<real4*>(base + L.kv);
+    real4* tq4 = reinterpret_cast<real4*>(base + L.tq);
+    real* cont = reinterpret_cast<real*>(base + L.cont);
+    real* prims = reinterpret_cast<real*>(base + L.prims);
+    real4* xp4 = reinterpret_cast<real4*>(base + L.xp);
     SelfLds sl;
     if (SELF && ri.self) sl = self_carve(base + L.self, nS, ri.table, ri.selfCap);
     // Any lane group in the workgroup may run self-collision; the barriers
@@ -359,7 +369,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     const int p0 = ri.pStart, f0 = ri.fStart;
     // ---- lane-owned state and constants, loaded once per launch
     v3 vel[TP], xpv[TP];
-    float wP[TP];
+    real wP[TP];
     v3 om[TP], iIF[TP], IF[TP];
     q4 qpv[TP];
 #pragma unroll
@@ -369,29 +379,29 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
             vel[t] = ld3(d.v, p0 + i);
             wP[t] = d.invMass[p0 + i];
             x4[i] = vf4(ld3(d.x, p0 + i), wP[t]);
-            f4[i] = vf4(ld3(d.force, p0 + i), 0.0f);
-            kv4[i] = vf4(ld3(d.kinVel, p0 + i), 0.0f);
+            f4[i] = vf4(ld3(d.force, p0 + i), real(0.0));
+            kv4[i] = vf4(ld3(d.kinVel, p0 + i), real(0.0));
         }
         const int j = l + G * t;
         if (active && j < nF) {
             om[t] = ld3(d.omega, f0 + j);
-            const float4 a = reinterpret_cast<const float4*>(d.invInertia4)[f0 + j];
-            const float4 b = reinterpret_cast<const float4*>(d.inertia4)[f0 + j];
+            const real4 a = reinterpret_cast<const real4*>(d.invInertia4)[f0 + j];
+            const real4 b = reinterpret_cast<const real4*>(d.inertia4)[f0 + j];
             iIF[t] = f4v(a);
             IF[t] = f4v(b);
             q4s[j] = qf4(ld4q(d.q, f0 + j));
-            tq4[j] = vf4(ld3(d.torque, f0 + j), 0.0f);
+            tq4[j] = vf4(ld3(d.torque, f0 + j), real(0.0));
         }
     }
     for (int k = l; active && k < nPrim; k += G) stage_prim(prims + k * kPrimFloats, d.prims[ri.primOff + k]);
 
     // Stretch constraints: colour c, trip t -> position colS[c] + l + G t.
     int sA[2][TC], sB[2][TC], sF[2][TC];
-    float sRest[2][TC];
+    real sRest[2][TC];
     v3 sComp[2][TC], sIIv[2][TC], sLam[2][TC];
     bool sOn[2][TC];
     int bA_[2][TC], bB_[2][TC];
-    float bRest[2][TC];
+    real bRest[2][TC];
     v3 bComp[2][TC], bRD[2][TC], bIa[2][TC], bIb[2][TC], bLam[2][TC];
     bool bOn[2][TC];
     const int* colS = d.colStart + ri.colOff;
@@ -406,41 +416,41 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                 const int k = colS[c] + l + G * t;
                 if (k < colS[c + 1]) {
                     const int4 id = reinterpret_cast<const int4*>(d.sIdx)[ri.sStart + k];
-                    const float4 par = reinterpret_cast<const float4*>(d.sPar)[ri.sStart + k];
+                    const real4 par = reinterpret_cast<const real4*>(d.sPar)[ri.sStart + k];
                     sOn[c][t] = true;
                     sA[c][t] = id.x;
                     sB[c][t] = id.y;
                     sF[c][t] = id.z;
                     sRest[c][t] = par.x;
                     sComp[c][t] = mk(par.y, par.z, par.w);
-                    sIIv[c][t] = f4v(reinterpret_cast<const float4*>(d.invInertia4)[f0 + id.z]);
+                    sIIv[c][t] = f4v(reinterpret_cast<const real4*>(d.invInertia4)[f0 + id.z]);
                 }
             }
             if (active && c < ri.nBCol) {
                 const int k = colB[c] + l + G * t;
                 if (k < colB[c + 1]) {
                     const int4 id = reinterpret_cast<const int4*>(d.bIdx)[ri.bStart + k];
-                    const float4 par = reinterpret_cast<const float4*>(d.bPar)[ri.bStart + k];
+                    const real4 par = reinterpret_cast<const real4*>(d.bPar)[ri.bStart + k];
                     bOn[c][t] = true;
                     bA_[c][t] = id.x;
                     bB_[c][t] = id.y;
                     bRest[c][t] = par.x;
                     bComp[c][t] = mk(par.y, par.z, par.w);
-                    bRD[c][t] = f4v(reinterpret_cast<const float4*>(d.bRest4)[ri.bStart + k]);
-                    bIa[c][t] = f4v(reinterpret_cast<const float4*>(d.invInertia4)[f0 + id.x]);
-                    bIb[c][t] = f4v(reinterpret_cast<const float4*>(d.invInertia4)[f0 + id.y]);
+                    bRD[c][t] = f4v(reinterpret_cast<const real4*>(d.bRest4)[ri.bStart + k]);
+                    bIa[c][t] = f4v(reinterpret_cast<const real4*>(d.invInertia4)[f0 + id.x]);
+                    bIb[c][t] = f4v(reinterpret_cast<const real4*>(d.invInertia4)[f0 + id.y]);
                 }
             }
         }
-    const float cscale = active ? d.complianceScale[rod] : 1.0f;
-    const float fscale = active ? d.frictionScale[rod] : 1.0f;
+    const real cscale = active ? d.complianceScale[rod] : real(1.0);
+    const real fscale = active ? d.frictionScale[rod] : real(1.0);
     const int* segEnd = d.segEnd + 2 * ri.sStart;
     if (SELF && ri.self && l == 0) {
         sl.misc[0] = 0;
         sl.misc[1] = 0;
     }
     const v3 g = mk(sa.gx, sa.gy, sa.gz);
-    const float h = sa.h;
+    const real h = sa.h;
     int clk = active ? d.clock[rod] : 0;
     // A rod whose clock has reached its step limit stops changing; its lanes
     // still take part in every barrier.
@@ -454,15 +464,15 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
 
     for (int step = 0; step < sa.steps; ++step) {
         const bool live = active && clk < limit;
-        if (driver && live) kv4[drivenP] = vf4(path_velocity(d.paths[rod], clk), 0.0f);
+        if (driver && live) kv4[drivenP] = vf4(path_velocity(d.paths[rod], clk), real(0.0));
         for (int sub = 0; sub < sa.substeps; ++sub) {
             const bool regen = sub % sa.contactInterval == 0;
 #pragma unroll
             for (int c = 0; c < 2; ++c)
 #pragma unroll
                 for (int t = 0; t < TC; ++t) {
-                    sLam[c][t] = mk(0.f, 0.f, 0.f);
-                    bLam[c][t] = mk(0.f, 0.f, 0.f);
+                    sLam[c][t] = mk(real(0.), real(0.), real(0.));
+                    bLam[c][t] = mk(real(0.), real(0.), real(0.));
                 }
             if (anyPrim && nPrim > 0) {
 #pragma unroll
@@ -507,7 +517,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
 #pragma unroll
                     for (int t = 0; t < TC; ++t) {
                         if (live && sOn[c][t]) {
-                            const float4 ax = x4[sA[c][t]], bx = x4[sB[c][t]];
+                            const real4 ax = x4[sA[c][t]], bx = x4[sB[c][t]];
                             v3 xa = f4v(ax), xb = f4v(bx);
                             q4 q = f4q(q4s[sF[c][t]]);
                             if (project_stretch(xa, xb, q, sLam[c][t], ax.w, bx.w, sIIv[c][t], sRest[c][t],
@@ -540,7 +550,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
 #pragma unroll
                         for (int t = 0; t < TP; ++t) {
                             const int i = l + G * t;
-                            if (live && i < nP && wP[t] > 0.0f) {
+                            if (live && i < nP && wP[t] > real(0.0)) {
                                 v3 x = f4v(x4[i]);
                                 world_contacts_project(cont + i * nPrim * kContactFloats, nPrim, x, xpv[t], wP[t]);
                                 x4[i] = vf4(x, wP[t]);
@@ -594,19 +604,21 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
 
 // Copy the orientation of zero-inertia frames from `src` (frames * 4, packed
 // for frames [f0, f0 + count)).
-__global__ void k_apply_fixed_frames(float* q, const float* src, const float* invInertia4, int f0, int count) {
+__global__ void k_apply_fixed_frames(real* q, const real* src, const real* invInertia4, int f0, int count) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= count) return;
-    const float4 iI = reinterpret_cast<const float4*>(invInertia4)[f0 + j];
-    if (norm2(mk(iI.x, iI.y, iI.z)) == 0.0f)
+    const real4 iI = reinterpret_cast<const real4*>(invInertia4)[f0 + j];
+    if (norm2(mk(iI.x, iI.y, iI.z)) == real(0.0))
         for (int c = 0; c < 4; ++c) q[4 * (f0 + j) + c] = src[4 * j + c];
 }
 
+}  // namespace RP_PREC
 }  // namespace rp
 
 // ------------------------------------------------------------------ host-side launch wrappers
 
 namespace rp {
+namespace RP_PREC {
 
 int plain_lds_bytes(int nP, int nF, int nS, int nB, int nPrim, int self, int table, int cap) {
     return plain_layout(nP, nF, nS, nB, nPrim, self, table, cap).total;
@@ -676,7 +688,7 @@ hipError_t launch_tiled(int tp, int tc, bool self, const DeviceArrays& d, const 
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_fixed_frames(float* q, const float* src, const float* invInertia4, int f0, int count,
+hipError_t launch_fixed_frames(real* q, const real* src, const real* invInertia4, int f0, int count,
                                hipStream_t s) {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_apply_fixed_frames, dim3((count + 255) / 256), dim3(256), 0, s, q, src, invInertia4, f0,
@@ -684,4 +696,5 @@ hipError_t launch_fixed_frames(float* q, const float* src, const float* invInert
     return hipGetLastError();
 }
 
+}  // namespace RP_PREC
 }  // namespace rp

@@ -22,7 +22,8 @@ constexpr int kPrimFloats = 20;    // one primitive staged in LDS
 constexpr int kContactFloats = 8;  // one world-contact slot
 constexpr int kSelfFloats = 16;    // one self-contact slot
 
-struct RodInfo {
+template <typename R>
+struct RodInfoT {
     int pStart, nP, fStart, nF;
     int sStart, nS, bStart, nB;
     int world;     // index into worlds, -1 for none
@@ -34,16 +35,17 @@ struct RodInfo {
     int colOff;    // into colStart: stretch colours then bend colours
     int nSCol, nBCol;
     int primOff;   // into prims
-    float radius;
-    float margin;  // contact margin (margin radii * radius)
-    float selfFriction;
+    R radius;
+    R margin;  // contact margin (margin radii * radius)
+    R selfFriction;
     int pad[3];
 };
 
-struct Prim {
+template <typename R>
+struct PrimT {
     int type;
-    float a[3], b[3], normal[3], half[3], rot[4];
-    float radius, friction;
+    R a[3], b[3], normal[3], half[3], rot[4];
+    R radius, friction;
 };
 
 // A lane group of a TILED workgroup: `lanes` consecutive lanes starting at
@@ -92,31 +94,32 @@ struct ObsRec {
     double maxRise;
 };
 
-struct DeviceArrays {
+template <typename R>
+struct DeviceArraysT {
     // state
-    float* x;
-    float* v;
-    float* q;
-    float* omega;
+    R* x;
+    R* v;
+    R* q;
+    R* omega;
     // loads
-    const float* force;
-    const float* kinVel;
-    const float* torque;
+    const R* force;
+    const R* kinVel;
+    const R* torque;
     // constants
-    const float* invMass;
-    const float* invInertia4;
-    const float* inertia4;
-    const int* sIdx;      // int4
-    const float* sPar;    // float4
-    const int* bIdx;      // int4
-    const float* bPar;    // float4
-    const float* bRest4;  // float4
-    const int* segEnd;    // int2
+    const R* invMass;
+    const R* invInertia4;
+    const R* inertia4;
+    const int* sIdx;  // int4
+    const R* sPar;    // 4 reals
+    const int* bIdx;  // int4
+    const R* bPar;    // 4 reals
+    const R* bRest4;  // 4 reals
+    const int* segEnd;  // int2
     const int* colStart;
-    const RodInfo* rods;
-    const Prim* prims;
-    float* complianceScale;
-    float* frictionScale;
+    const RodInfoT<R>* rods;
+    const PrimT<R>* prims;
+    R* complianceScale;
+    R* frictionScale;
     int* selfOverflow;
     int* clock;            // steps taken, per rod
     const PathRec* paths;  // per rod, or null
@@ -125,15 +128,34 @@ struct DeviceArrays {
     int numRods;
 };
 
-struct StepArgs {
-    float h;
-    float invH2;
-    float gx, gy, gz;
-    float linDecay, angDecay;
+template <typename R>
+struct StepArgsT {
+    R h;
+    R invH2;
+    R gx, gy, gz;
+    R linDecay, angDecay;
     int substeps;
     int iterations;
     int contactInterval;
     int steps;  // steps in this launch
 };
+
+// The two arithmetic precisions. Device code is compiled once per precision
+// inside its namespace (f32: the reference GPU path's FP32; f64: the
+// reference CPU path's FP64), with these names for the records.
+namespace f32 {
+using real = float;
+using RodInfo = RodInfoT<float>;
+using Prim = PrimT<float>;
+using DeviceArrays = DeviceArraysT<float>;
+using StepArgs = StepArgsT<float>;
+}  // namespace f32
+namespace f64 {
+using real = double;
+using RodInfo = RodInfoT<double>;
+using Prim = PrimT<double>;
+using DeviceArrays = DeviceArraysT<double>;
+using StepArgs = StepArgsT<double>;
+}  // namespace f64
 
 }  // namespace rp

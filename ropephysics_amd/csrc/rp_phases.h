@@ -9,21 +9,22 @@
 #include "rp_math.h"
 
 namespace rp {
+namespace RP_PREC {
 
-__device__ __forceinline__ v3 f4v(float4 a) { return mk(a.x, a.y, a.z); }
-__device__ __forceinline__ q4 f4q(float4 a) { return q4{a.x, a.y, a.z, a.w}; }
-__device__ __forceinline__ float4 vf4(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
-__device__ __forceinline__ float4 qf4(q4 a) { return make_float4(a.w, a.x, a.y, a.z); }
-__device__ __forceinline__ v3 ld3(const float* p, int i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
-__device__ __forceinline__ void st3(float* p, int i, v3 a) {
+__device__ __forceinline__ v3 f4v(real4 a) { return mk(a.x, a.y, a.z); }
+__device__ __forceinline__ q4 f4q(real4 a) { return q4{a.x, a.y, a.z, a.w}; }
+__device__ __forceinline__ real4 vf4(v3 a, real w) { return make_real4(a.x, a.y, a.z, w); }
+__device__ __forceinline__ real4 qf4(q4 a) { return make_real4(a.w, a.x, a.y, a.z); }
+__device__ __forceinline__ v3 ld3(const real* p, int i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+__device__ __forceinline__ void st3(real* p, int i, v3 a) {
     p[3 * i] = a.x;
     p[3 * i + 1] = a.y;
     p[3 * i + 2] = a.z;
 }
-__device__ __forceinline__ q4 ld4q(const float* p, int i) {
+__device__ __forceinline__ q4 ld4q(const real* p, int i) {
     return q4{p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]};
 }
-__device__ __forceinline__ void st4q(float* p, int i, q4 a) {
+__device__ __forceinline__ void st4q(real* p, int i, q4 a) {
     p[4 * i] = a.w;
     p[4 * i + 1] = a.x;
     p[4 * i + 2] = a.y;
@@ -32,10 +33,10 @@ __device__ __forceinline__ void st4q(float* p, int i, q4 a) {
 
 // Symplectic-Euler prediction of one particle. x is updated in place and the
 // substep-start position returned through xPrev.
-__device__ __forceinline__ void predict_particle(v3& x, v3& v, v3& xPrev, float w, v3 force, v3 kinVel, float h,
+__device__ __forceinline__ void predict_particle(v3& x, v3& v, v3& xPrev, real w, v3 force, v3 kinVel, real h,
                                                  v3 g) {
     xPrev = x;
-    if (w == 0.0f) {
+    if (w == real(0.0)) {
         x = add(x, scale(kinVel, h));
         return;
     }
@@ -44,29 +45,29 @@ __device__ __forceinline__ void predict_particle(v3& x, v3& v, v3& xPrev, float 
 }
 
 // Gyroscopic angular-velocity update and quaternion integration of one frame.
-__device__ __forceinline__ void predict_frame(q4& q, v3& omega, q4& qPrev, v3 iI, v3 I, v3 torque, float h) {
+__device__ __forceinline__ void predict_frame(q4& q, v3& omega, q4& qPrev, v3 iI, v3 I, v3 torque, real h) {
     qPrev = q;
-    if (norm2(iI) == 0.0f) return;
+    if (norm2(iI) == real(0.0)) return;
     const v3 w = omega;
     const v3 tau = rotate_inv(q, torque);
     omega = add(w, scale(cwise(iI, sub(tau, cross(w, cwise(I, w)))), h));
-    q = qnormalize(qadd(q, qscale(qmul(q, mkq(0.0f, omega)), 0.5f * h)));
+    q = qnormalize(qadd(q, qscale(qmul(q, mkq(real(0.0), omega)), real(0.5) * h)));
 }
 
-__device__ __forceinline__ v3 finish_particle(v3 x, v3 xPrev, float w, v3 kinVel, float h, float decay) {
-    if (w == 0.0f) return kinVel;
+__device__ __forceinline__ v3 finish_particle(v3 x, v3 xPrev, real w, v3 kinVel, real h, real decay) {
+    if (w == real(0.0)) return kinVel;
     return scale(divs(sub(x, xPrev), h), decay);
 }
 
-__device__ __forceinline__ v3 finish_frame(q4 q, q4 qPrev, v3 iI, float h, float decay) {
-    if (norm2(iI) == 0.0f) return mk(0.0f, 0.0f, 0.0f);
+__device__ __forceinline__ v3 finish_frame(q4 q, q4 qPrev, v3 iI, real h, real decay) {
+    if (norm2(iI) == real(0.0)) return mk(real(0.0), real(0.0), real(0.0));
     const q4 dq = hemi_identity(qmul(conj(qPrev), q));
-    return scale(scale(im(dq), 2.0f / h), decay);
+    return scale(scale(im(dq), real(2.0) / h), decay);
 }
 
-__device__ __forceinline__ prim load_prim(const float* s) {
+__device__ __forceinline__ prim load_prim(const real* s) {
     prim p;
-    p.type = __float_as_int(s[0]);
+    p.type = slot_int(s[0]);
     p.a = mk(s[1], s[2], s[3]);
     p.b = mk(s[4], s[5], s[6]);
     p.normal = mk(s[7], s[8], s[9]);
@@ -76,8 +77,8 @@ __device__ __forceinline__ prim load_prim(const float* s) {
     p.friction = s[18];
     return p;
 }
-__device__ __forceinline__ void stage_prim(float* s, const Prim& g) {
-    s[0] = __int_as_float(g.type);
+__device__ __forceinline__ void stage_prim(real* s, const Prim& g) {
+    s[0] = int_slot(g.type);
     for (int c = 0; c < 3; ++c) {
         s[1 + c] = g.a[c];
         s[4 + c] = g.b[c];
@@ -87,20 +88,20 @@ __device__ __forceinline__ void stage_prim(float* s, const Prim& g) {
     for (int c = 0; c < 4; ++c) s[13 + c] = g.rot[c];
     s[17] = g.radius;
     s[18] = g.friction;
-    s[19] = 0.0f;
+    s[19] = real(0.0);
 }
 
 // World contacts of one particle: regenerate (or just reset) its P slots.
-__device__ __forceinline__ void world_contacts_begin(float* slots, const float* prims, int nPrim, v3 x, float w,
-                                                     float radius, float margin, float fscale, bool regen) {
+__device__ __forceinline__ void world_contacts_begin(real* slots, const real* prims, int nPrim, v3 x, real w,
+                                                     real radius, real margin, real fscale, bool regen) {
     for (int k = 0; k < nPrim; ++k) {
-        float* s = slots + k * kContactFloats;
-        s[5] = 0.0f;  // lamN
-        s[6] = 0.0f;  // applied
+        real* s = slots + k * kContactFloats;
+        s[5] = real(0.0);  // lamN
+        s[6] = real(0.0);  // applied
         if (!regen) continue;
         wcontact c;
         world_contact_generate(c, load_prim(prims + k * kPrimFloats), x, w, radius, margin, fscale);
-        s[7] = __int_as_float(c.active);
+        s[7] = int_slot(c.active);
         if (c.active) {
             s[0] = c.n.x;
             s[1] = c.n.y;
@@ -111,11 +112,11 @@ __device__ __forceinline__ void world_contacts_begin(float* slots, const float* 
     }
 }
 
-__device__ __forceinline__ void world_contacts_project(float* slots, int nPrim, v3& x, v3 xPrev, float w) {
+__device__ __forceinline__ void world_contacts_project(real* slots, int nPrim, v3& x, v3 xPrev, real w) {
     for (int k = 0; k < nPrim; ++k) {
-        float* s = slots + k * kContactFloats;
+        real* s = slots + k * kContactFloats;
         wcontact c;
-        c.active = __float_as_int(s[7]);
+        c.active = slot_int(s[7]);
         if (!c.active) continue;
         c.n = mk(s[0], s[1], s[2]);
         c.offset = s[3];
@@ -157,7 +158,7 @@ __device__ inline v3 path_velocity(const PathRec& p, int step) {
     path_point(p, double(tick + 1) * p.period, x1, y1);
     path_point(p, double(tick) * p.period, x0, y0);
     const double inv = 1.0 / p.period;
-    return mk(float((x1 - x0) * inv), float((y1 - y0) * inv), float((p.height - p.height) * inv));
+    return mk(real((x1 - x0) * inv), real((y1 - y0) * inv), real((p.height - p.height) * inv));
 }
 
 // ------------------------------------------------------------------ self-collision
@@ -165,29 +166,35 @@ __device__ inline v3 path_velocity(const PathRec& p, int step) {
 // One rod's working set in LDS. Segments are visited in the rod's original
 // stretch-constraint order, as the CPU broadphase does.
 struct SelfLds {
-    float4* cen;     // nS centres and lengths (after a lane-group build: in bucket order)
+    real4* cen;     // nS centres and lengths (after a lane-group build: in bucket order)
     int* cellOf;     // nS
     int* sorted;     // nS
     int* segCount;   // nS
     int* segStart;   // nS
     int* cellStart;  // table + 1
-    float* pool;     // cap * kSelfFloats
-    int* misc;       // [0] live, [1] overflow, [2] cell size, [3] longest segment (float bits)
+    real* pool;     // cap * kSelfFloats
+    int* misc;       // [0] live, [1] overflow
+    rbits* rmisc;    // [0] cell size, [1] longest segment, as bits (non-negative: ordered as integers)
     int* scan;       // kScanLanes, scratch of the group-parallel scans
 };
 
 constexpr int kScanLanes = 256;
 
+// misc then rmisc, padded so the int arrays after them stay 16-byte aligned.
+constexpr int kMiscBytes = RP_FP64 ? 32 : 16;
+
 __host__ __device__ inline int self_lds_bytes(int nS, int table, int cap) {
-    return nS * 16 + 4 * nS * 4 + (table + 1) * 4 + cap * kSelfFloats * 4 + 16 + kScanLanes * 4;
+    return nS * int(sizeof(real4)) + 4 * nS * 4 + (table + 1) * 4 + cap * kSelfFloats * int(sizeof(real)) +
+           kMiscBytes + kScanLanes * 4;
 }
 
 __device__ inline SelfLds self_carve(char* base, int nS, int table, int cap) {
     SelfLds s;
-    s.cen = reinterpret_cast<float4*>(base);
-    s.pool = reinterpret_cast<float*>(base + nS * 16);
-    s.misc = reinterpret_cast<int*>(base + nS * 16 + cap * kSelfFloats * 4);
-    s.cellOf = s.misc + 4;
+    s.cen = reinterpret_cast<real4*>(base);
+    s.pool = reinterpret_cast<real*>(base + nS * int(sizeof(real4)));
+    s.misc = reinterpret_cast<int*>(base + nS * int(sizeof(real4)) + cap * kSelfFloats * int(sizeof(real)));
+    s.rmisc = reinterpret_cast<rbits*>(s.misc + 2);
+    s.cellOf = s.misc + kMiscBytes / 4;
     s.sorted = s.cellOf + nS;
     s.segCount = s.sorted + nS;
     s.segStart = s.segCount + nS;
@@ -197,17 +204,17 @@ __device__ inline SelfLds self_carve(char* base, int nS, int table, int cap) {
 }
 
 // Centres, cell size and the counting sort into buckets; one lane.
-__device__ inline void self_build_hash(const SelfLds& s, const float4* x4, const int* segEnd, int nS, int table,
-                                       float radius) {
-    float maxSeg = 0.0f;
+__device__ inline void self_build_hash(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
+                                       real radius) {
+    real maxSeg = real(0.0);
     for (int j = 0; j < nS; ++j) {
         const v3 a = f4v(x4[segEnd[2 * j]]), b = f4v(x4[segEnd[2 * j + 1]]);
-        s.cen[j] = vf4(scale(add(a, b), 0.5f), 0.0f);
-        const float len = norm(sub(b, a));
+        s.cen[j] = vf4(scale(add(a, b), real(0.5)), real(0.0));
+        const real len = norm(sub(b, a));
         maxSeg = maxSeg < len ? len : maxSeg;
     }
-    const float cell = maxSeg + 2.0f * radius;
-    s.misc[2] = __float_as_int(cell);
+    const real cell = maxSeg + real(2.0) * radius;
+    s.rmisc[0] = real_bits(cell);
     for (int t = 0; t <= table; ++t) s.cellStart[t] = 0;
     for (int j = 0; j < nS; ++j) {
         const v3 c = f4v(s.cen[j]);
@@ -225,28 +232,28 @@ __device__ inline void self_build_hash(const SelfLds& s, const float4* x4, const
 // segment k = particles k0, k1): a contact when the closest points are nearer
 // than 2r. Each contact takes the next slot from slotBase; with `fill`, a slot
 // below the capacity receives the contact record.
-__device__ __forceinline__ void self_pair(const SelfLds& s, const float4* x4, int j0, int j1, v3 a0, v3 a1, int k0,
-                                          int k1, float radius, float friction, bool fill, int slotBase, int cap,
+__device__ __forceinline__ void self_pair(const SelfLds& s, const real4* x4, int j0, int j1, v3 a0, v3 a1, int k0,
+                                          int k1, real radius, real friction, bool fill, int slotBase, int cap,
                                           int& count) {
     const v3 b0 = f4v(x4[k0]), b1 = f4v(x4[k1]);
-    float u, v;
+    real u, v;
     seg_seg(a0, a1, b0, b1, u, v);
     const v3 pa = add(a0, scale(sub(a1, a0), u));
     const v3 pb = add(b0, scale(sub(b1, b0), v));
     const v3 delta = sub(pa, pb);
-    const float dist = norm(delta);
-    const float sep = dist - 2.0f * radius;
-    if (sep >= 0.0f) return;
+    const real dist = norm(delta);
+    const real sep = dist - real(2.0) * radius;
+    if (sep >= real(0.0)) return;
     const int slot = slotBase + count++;
     if (!fill || slot >= cap) return;
-    const v3 n = dist > 1e-12f ? divs(delta, dist) : mk(0.0f, 0.0f, 1.0f);
+    const v3 n = dist > real(1e-12) ? divs(delta, dist) : mk(real(0.0), real(0.0), real(1.0));
     const int idx[4] = {j0, j1, k0, k1};
-    const float wt[4] = {1.0f - u, u, -(1.0f - v), -v};
-    float dotSum = 0.0f;
+    const real wt[4] = {real(1.0) - u, u, -(real(1.0) - v), -v};
+    real dotSum = real(0.0);
     for (int m = 0; m < 4; ++m) dotSum += wt[m] * dot(f4v(x4[idx[m]]), n);
-    float* c = s.pool + slot * kSelfFloats;
+    real* c = s.pool + slot * kSelfFloats;
     for (int m = 0; m < 4; ++m) {
-        c[m] = __int_as_float(idx[m]);
+        c[m] = int_slot(idx[m]);
         c[4 + m] = wt[m];
     }
     c[8] = n.x;
@@ -254,18 +261,18 @@ __device__ __forceinline__ void self_pair(const SelfLds& s, const float4* x4, in
     c[10] = n.z;
     c[11] = dotSum - sep;  // offset
     c[12] = friction;
-    c[13] = 0.0f;  // lamN
-    c[14] = 0.0f;  // applied
+    c[13] = real(0.0);  // lamN
+    c[14] = real(0.0);  // applied
 }
 
 // Contacts of segment j with later segments in the CPU's candidate order
 // (27 cells dz, dy, dx; bucket items ascending; a bucket reached twice is
 // visited twice). Counts them, or with `fill` writes them at j's pool offset.
-__device__ inline void self_query(const SelfLds& s, const float4* x4, const int* segEnd, int nS, int table,
-                                  int gap, int cap, float radius, float friction, int j, bool fill) {
+__device__ inline void self_query(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
+                                  int gap, int cap, real radius, real friction, int j, bool fill) {
     int count = 0;
     if (nS > gap) {
-        const float cell = __int_as_float(s.misc[2]);
+        const real cell = bits_real(s.rmisc[0]);
         const v3 cj = f4v(s.cen[j]);
         const int cx = cell_coord(cj.x, cell), cy = cell_coord(cj.y, cell), cz = cell_coord(cj.z, cell);
         const int j0 = segEnd[2 * j], j1 = segEnd[2 * j + 1];
@@ -291,7 +298,7 @@ __device__ inline void self_query(const SelfLds& s, const float4* x4, const int*
 //
 // The same hash, candidate order and pool layout as self_build_hash /
 // self_allocate, built by the G lanes of a rod's lane group: the longest
-// segment by an atomic max on the (non-negative) float bits, bucket counts by
+// segment by an atomic max on the (non-negative) real bits, bucket counts by
 // atomic adds, prefix sums by a chunked scan, and each bucket's items placed
 // in ascending segment order by counting the earlier segments in the same
 // bucket. Every step is exact integer work or an order-free max, so the
@@ -330,7 +337,7 @@ __device__ inline int group_scan(const int* in, int* out, int n, int* scratch, i
     return total;
 }
 
-__device__ __forceinline__ float abs_max3(v3 a) { return fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z))); }
+__device__ __forceinline__ real abs_max3(v3 a) { return r_max(r_abs(a.x), r_max(r_abs(a.y), r_abs(a.z))); }
 __device__ __forceinline__ int count_eq4(int4 c, int h) {
     return (c.x == h ? 1 : 0) + (c.y == h ? 1 : 0) + (c.z == h ? 1 : 0) + (c.w == h ? 1 : 0);
 }
@@ -347,12 +354,12 @@ __device__ __forceinline__ int count_eq4(int4 c, int h) {
 // a pair's computed separation is positive and it would not be a contact: the
 // contact list is unchanged. Returns the number of contacts.
 template <bool kFill>
-__device__ inline int self_query_rows(const SelfLds& s, const float4* x4, const int* segEnd, int table, int gap,
-                                      int cap, float radius, float friction, float cell, int j, float4 cj4,
+__device__ inline int self_query_rows(const SelfLds& s, const real4* x4, const int* segEnd, int table, int gap,
+                                      int cap, real radius, real friction, real cell, int j, real4 cj4,
                                       int slotBase) {
     const v3 cj = f4v(cj4);
-    const float reachJ = 0.5f * cj4.w + 2.0f * radius;
-    const float scaleJ = abs_max3(cj);
+    const real reachJ = real(0.5) * cj4.w + real(2.0) * radius;
+    const real scaleJ = abs_max3(cj);
     const int cx = cell_coord(cj.x, cell), cy = cell_coord(cj.y, cell), cz = cell_coord(cj.z, cell);
     const int j0 = segEnd[2 * j], j1 = segEnd[2 * j + 1];
     const v3 a0 = f4v(x4[j0]), a1 = f4v(x4[j1]);
@@ -382,7 +389,7 @@ __device__ inline int self_query_rows(const SelfLds& s, const float4* x4, const 
             const int n0 = hi[0] - lo[0], n01 = n0 + hi[1] - lo[1], total = n01 + hi[2] - lo[2];
             for (int base = 0; base < total; base += 4) {
                 int kq[4];
-                float4 cq[4];
+                real4 cq[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int q = min(base + u, total - 1);
@@ -394,8 +401,8 @@ __device__ inline int self_query_rows(const SelfLds& s, const float4* x4, const 
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const v3 ck = f4v(cq[u]);
-                    const float lim =
-                        1.001f * (reachJ + 0.5f * cq[u].w) + 1e-6f * (1.0f + fmaxf(scaleJ, abs_max3(ck)));
+                    const real lim =
+                        real(1.001) * (reachJ + real(0.5) * cq[u].w) + real(1e-6) * (real(1.0) + r_max(scaleJ, abs_max3(ck)));
                     const bool keep = (base + u < total) & (kq[u] - j >= gap) & !(norm2(sub(ck, cj)) > lim * lim);
                     pass |= keep ? (1u << u) : 0u;
                 }
@@ -415,27 +422,27 @@ __device__ inline int self_query_rows(const SelfLds& s, const float4* x4, const 
 // l owns segments j = l + G t, t < TP (the host guarantees nS <= TP G), and
 // keeps their centres in registers. s.cen ends up in bucket order.
 template <int TP>
-__device__ inline void self_regen_group(const SelfLds& s, const float4* x4, const int* segEnd, int nS, int table,
-                                        int gap, int cap, float radius, float friction, bool on, int l, int G) {
-    float4 cj[TP];
+__device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
+                                        int gap, int cap, real radius, real friction, bool on, int l, int G) {
+    real4 cj[TP];
     if (on) {
         for (int t = l; t <= table; t += G) s.cellStart[t] = 0;
-        if (l == 0) s.misc[3] = 0;
+        if (l == 0) s.rmisc[1] = 0;
     }
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
-        cj[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+        cj[t] = make_real4(real(0.), real(0.), real(0.), real(0.));
         if (on && j < nS) {
             const v3 a = f4v(x4[segEnd[2 * j]]), b = f4v(x4[segEnd[2 * j + 1]]);
-            const float len = norm(sub(b, a));
-            cj[t] = vf4(scale(add(a, b), 0.5f), len);
-            atomicMax(&s.misc[3], __float_as_int(len));
+            const real len = norm(sub(b, a));
+            cj[t] = vf4(scale(add(a, b), real(0.5)), len);
+            atomicMax(&s.rmisc[1], real_bits(len));
         }
     }
     __syncthreads();
-    const float cell = on ? __int_as_float(s.misc[3]) + 2.0f * radius : 0.0f;
+    const real cell = on ? bits_real(s.rmisc[1]) + real(2.0) * radius : real(0.0);
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
@@ -446,7 +453,7 @@ __device__ inline void self_regen_group(const SelfLds& s, const float4* x4, cons
             atomicAdd(&s.cellStart[h], 1);
         }
     }
-    if (on && l == 0) s.misc[2] = __float_as_int(cell);
+    if (on && l == 0) s.rmisc[0] = real_bits(cell);
     __syncthreads();
     const int items = group_scan(s.cellStart, s.cellStart, on ? table : 0, s.scan, l, G, on);
     if (on && l == 0) s.cellStart[table] = items;
@@ -509,50 +516,50 @@ __device__ inline void self_allocate(const SelfLds& s, int nS, int cap) {
 __device__ inline void self_reset(const SelfLds& s) {
     const int live = s.misc[0];
     for (int c = 0; c < live; ++c) {
-        s.pool[c * kSelfFloats + 13] = 0.0f;
-        s.pool[c * kSelfFloats + 14] = 0.0f;
+        s.pool[c * kSelfFloats + 13] = real(0.0);
+        s.pool[c * kSelfFloats + 14] = real(0.0);
     }
 }
 
 // Four-particle contacts, sequentially in pool order (= the CPU's list order).
 // xp4 holds substep-start positions.
-__device__ inline void self_project(const SelfLds& s, float4* x4, const float4* xp4) {
+__device__ inline void self_project(const SelfLds& s, real4* x4, const real4* xp4) {
     const int live = s.misc[0];
     for (int ci = 0; ci < live; ++ci) {
-        float* c = s.pool + ci * kSelfFloats;
+        real* c = s.pool + ci * kSelfFloats;
         int idx[4];
-        float wt[4], im[4];
+        real wt[4], im[4];
         for (int m = 0; m < 4; ++m) {
-            idx[m] = __float_as_int(c[m]);
+            idx[m] = slot_int(c[m]);
             wt[m] = c[4 + m];
             im[m] = x4[idx[m]].w;
         }
         const v3 n = mk(c[8], c[9], c[10]);
-        float wEff = 0.0f;
+        real wEff = real(0.0);
         for (int m = 0; m < 4; ++m) wEff += wt[m] * wt[m] * im[m];
-        if (wEff <= 0.0f) continue;
-        float C = -c[11];
+        if (wEff <= real(0.0)) continue;
+        real C = -c[11];
         for (int m = 0; m < 4; ++m) C += wt[m] * dot(f4v(x4[idx[m]]), n);
-        float dl = -C / wEff;
-        const float lamN0 = c[13];
-        const float cl = max0(lamN0 + dl);
+        real dl = -C / wEff;
+        const real lamN0 = c[13];
+        const real cl = max0(lamN0 + dl);
         dl = cl - lamN0;
         c[13] = cl;
-        if (dl != 0.0f)
+        if (dl != real(0.0))
             for (int m = 0; m < 4; ++m) {
                 const v3 xm = add(f4v(x4[idx[m]]), scale(n, im[m] * wt[m] * dl));
                 x4[idx[m]] = vf4(xm, im[m]);
             }
-        const float friction = c[12];
-        if (friction <= 0.0f || cl <= 0.0f) continue;
-        v3 dp = mk(0.0f, 0.0f, 0.0f);
+        const real friction = c[12];
+        if (friction <= real(0.0) || cl <= real(0.0)) continue;
+        v3 dp = mk(real(0.0), real(0.0), real(0.0));
         for (int m = 0; m < 4; ++m) dp = add(dp, scale(sub(f4v(x4[idx[m]]), f4v(xp4[idx[m]])), wt[m]));
         const v3 tang = sub(dp, scale(n, dot(dp, n)));
-        const float slide = norm(tang);
-        if (slide < 1e-15f) continue;
-        const float budget = friction * cl * wEff - c[14];
-        if (budget <= 0.0f) continue;
-        const float capped = min_ab(slide, budget);
+        const real slide = norm(tang);
+        if (slide < real(1e-15)) continue;
+        const real budget = friction * cl * wEff - c[14];
+        if (budget <= real(0.0)) continue;
+        const real capped = min_ab(slide, budget);
         c[14] = c[14] + capped;
         const v3 corr = scale(tang, -capped / slide);
         for (int m = 0; m < 4; ++m) {
@@ -562,4 +569,5 @@ __device__ inline void self_project(const SelfLds& s, float4* x4, const float4* 
     }
 }
 
+}  // namespace RP_PREC
 }  // namespace rp
