@@ -16,9 +16,11 @@
  *   - A batch is used from one host thread at a time. Batches are
  *     independent; each has its own HIP stream, so work on two batches
  *     overlaps on the device. rp_batch_sync waits for one batch only.
- *   - Arrays are FP32. Vectors are xyz triples, quaternions wxyz quadruples
- *     (body -> world). Per-element arrays of a rod range are packed in rod
- *     order, each rod contributing its own element count (ragged CSR).
+ *   - Arrays are FP32, except in a batch made by rp_batch_create64, whose
+ *     values are FP64 throughout (the *64 calls below). Vectors are xyz
+ *     triples, quaternions wxyz quadruples (body -> world). Per-element
+ *     arrays of a rod range are packed in rod order, each rod contributing
+ *     its own element count (ragged CSR).
  */
 #ifndef ROPEPHYSICS_H
 #define ROPEPHYSICS_H
@@ -254,6 +256,95 @@ int rp_tile_config_count(void);
 
 /* Thread-local message for the last failed call on this thread. */
 const char* rp_last_error(void);
+
+/* ------------------------------------------------------------------ FP64 batches
+ *
+ * The same batch in double precision: every real array, parameter and state
+ * value is FP64 on the host and on the device, and the kernels perform the
+ * reference CPU solver's arithmetic in that solver's own default precision
+ * (crs::Real = double, src/core/math3.h:14-22) -- where FP32's spacing at
+ * metre scale (about 6e-8 m) is coarser than one substep's gravity drop, a
+ * hold/slide or routed/not-routed verdict can differ between FP32 and FP64;
+ * an FP64 batch gives the CPU path's verdicts, computed on the GPU. Each rod's
+ * results equal the reference CPU step's in colour order, bit for bit.
+ *
+ * The structs mirror their FP32 counterparts field for field. Calls not
+ * listed here (destroy, sync, counters, paths, clock, observers, timing, tune
+ * report) take either kind of batch. FP32 state and step calls refuse an FP64
+ * batch, and these refuse an FP32 one (RP_ERR_STATE). TILED FP64 batches use
+ * the shapes with one constraint trip per colour and at most 256 lanes. */
+typedef struct rp_primitive64 {
+    int type;
+    double a[3];
+    double b[3];
+    double normal[3];
+    double half_extents[3];
+    double rotation[4];
+    double radius;
+    double friction;
+} rp_primitive64;
+
+typedef struct rp_world_desc64 {
+    int num_prims;
+    const rp_primitive64* prims;
+    double contact_margin_radii;
+    int self_collision;
+    double self_friction;
+    int self_skip;
+    int hash_table_size;
+    int self_capacity;
+} rp_world_desc64;
+
+typedef struct rp_batch_desc64 {
+    int num_rods;
+    const int* particle_start;
+    const int* frame_start;
+    const int* stretch_start;
+    const int* bend_start;
+    const double* inv_mass;
+    const double* inv_inertia;
+    const double* inertia;
+    const int* stretch_p0;
+    const int* stretch_p1;
+    const int* stretch_seg;
+    const double* stretch_rest;
+    const double* stretch_compliance;
+    const int* bend_a;
+    const int* bend_b;
+    const double* bend_rest;
+    const double* bend_compliance;
+    const double* bend_rest_darboux;
+    const double* radius;
+    int num_worlds;
+    const rp_world_desc64* worlds;
+    const int* world_of_rod;
+    int strategy;
+    int tile_config;
+} rp_batch_desc64;
+
+typedef struct rp_step_params64 {
+    double dt;
+    int substeps;
+    int iterations;
+    double gravity[3];
+    double linear_damping;
+    double angular_damping;
+    int contact_interval;
+} rp_step_params64;
+
+/* As rp_batch_create, rp_batch_set_state, rp_batch_set_loads,
+ * rp_batch_set_scales, rp_batch_step and rp_batch_get_state, in FP64. */
+int rp_batch_create64(const rp_batch_desc64* desc, void** out_batch);
+int rp_batch_set_state64(void* batch, int first_rod, int num_rods, const double* x, const double* v,
+                         const double* q, const double* omega);
+int rp_batch_set_loads64(void* batch, int first_rod, int num_rods, const double* force, const double* kin_vel,
+                         const double* torque, const double* fixed_q);
+int rp_batch_set_scales64(void* batch, const double* youngs_scale, const double* friction_scale);
+int rp_batch_step64(void* batch, const rp_step_params64* params, int num_steps);
+int rp_batch_get_state64(void* batch, int first_rod, int num_rods, double* x, double* v, double* q, double* omega);
+
+/* Bits of a batch's arithmetic: 32 or 64; a negative status for NULL. */
+int rp_batch_precision(void* batch);
 
 /* ------------------------------------------------------------------ serving
  *

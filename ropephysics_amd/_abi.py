@@ -16,34 +16,45 @@ STRATEGY = {"plain": 0, "tiled": 1, "auto": 2}
 STRATEGY_NAME = {v: k for k, v in STRATEGY.items()}
 
 _fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 
-class RpPrimitive(C.Structure):
-    _fields_ = [("type", C.c_int), ("a", C.c_float * 3), ("b", C.c_float * 3), ("normal", C.c_float * 3),
-                ("half_extents", C.c_float * 3), ("rotation", C.c_float * 4), ("radius", C.c_float),
-                ("friction", C.c_float)]
+def _batch_structs(real, suffix):
+    """The primitive, world, batch and step-parameter structs over one real
+    type: rp_primitive ... rp_step_params (FP32) or their *64 twins."""
+    rp = C.POINTER(real)
+
+    class Primitive(C.Structure):
+        _fields_ = [("type", C.c_int), ("a", real * 3), ("b", real * 3), ("normal", real * 3),
+                    ("half_extents", real * 3), ("rotation", real * 4), ("radius", real), ("friction", real)]
+
+    class WorldDesc(C.Structure):
+        _fields_ = [("num_prims", C.c_int), ("prims", C.POINTER(Primitive)), ("contact_margin_radii", real),
+                    ("self_collision", C.c_int), ("self_friction", real), ("self_skip", C.c_int),
+                    ("hash_table_size", C.c_int), ("self_capacity", C.c_int)]
+
+    class BatchDesc(C.Structure):
+        _fields_ = [("num_rods", C.c_int), ("particle_start", _ip), ("frame_start", _ip), ("stretch_start", _ip),
+                    ("bend_start", _ip), ("inv_mass", rp), ("inv_inertia", rp), ("inertia", rp),
+                    ("stretch_p0", _ip), ("stretch_p1", _ip), ("stretch_seg", _ip), ("stretch_rest", rp),
+                    ("stretch_compliance", rp), ("bend_a", _ip), ("bend_b", _ip), ("bend_rest", rp),
+                    ("bend_compliance", rp), ("bend_rest_darboux", rp), ("radius", rp), ("num_worlds", C.c_int),
+                    ("worlds", C.POINTER(WorldDesc)), ("world_of_rod", _ip), ("strategy", C.c_int),
+                    ("tile_config", C.c_int)]
+
+    class StepParams(C.Structure):
+        _fields_ = [("dt", real), ("substeps", C.c_int), ("iterations", C.c_int), ("gravity", real * 3),
+                    ("linear_damping", real), ("angular_damping", real), ("contact_interval", C.c_int)]
+
+    for cls, name in ((Primitive, "RpPrimitive"), (WorldDesc, "RpWorldDesc"), (BatchDesc, "RpBatchDesc"),
+                      (StepParams, "RpStepParams")):
+        cls.__name__ = cls.__qualname__ = name + suffix
+    return Primitive, WorldDesc, BatchDesc, StepParams
 
 
-class RpWorldDesc(C.Structure):
-    _fields_ = [("num_prims", C.c_int), ("prims", C.POINTER(RpPrimitive)), ("contact_margin_radii", C.c_float),
-                ("self_collision", C.c_int), ("self_friction", C.c_float), ("self_skip", C.c_int),
-                ("hash_table_size", C.c_int), ("self_capacity", C.c_int)]
-
-
-class RpBatchDesc(C.Structure):
-    _fields_ = [("num_rods", C.c_int), ("particle_start", _ip), ("frame_start", _ip), ("stretch_start", _ip),
-                ("bend_start", _ip), ("inv_mass", _fp), ("inv_inertia", _fp), ("inertia", _fp),
-                ("stretch_p0", _ip), ("stretch_p1", _ip), ("stretch_seg", _ip), ("stretch_rest", _fp),
-                ("stretch_compliance", _fp), ("bend_a", _ip), ("bend_b", _ip), ("bend_rest", _fp),
-                ("bend_compliance", _fp), ("bend_rest_darboux", _fp), ("radius", _fp), ("num_worlds", C.c_int),
-                ("worlds", C.POINTER(RpWorldDesc)), ("world_of_rod", _ip), ("strategy", C.c_int),
-                ("tile_config", C.c_int)]
-
-
-class RpStepParams(C.Structure):
-    _fields_ = [("dt", C.c_float), ("substeps", C.c_int), ("iterations", C.c_int), ("gravity", C.c_float * 3),
-                ("linear_damping", C.c_float), ("angular_damping", C.c_float), ("contact_interval", C.c_int)]
+RpPrimitive, RpWorldDesc, RpBatchDesc, RpStepParams = _batch_structs(C.c_float, "")
+RpPrimitive64, RpWorldDesc64, RpBatchDesc64, RpStepParams64 = _batch_structs(C.c_double, "64")
 
 
 class RpPathDesc(C.Structure):
@@ -112,6 +123,13 @@ SIGNATURES = {
     "rp_batch_get_clock": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip]),
     "rp_batch_set_observers": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(RpObserverDesc)]),
     "rp_batch_get_observations": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "rp_batch_create64": (C.c_int, [C.POINTER(RpBatchDesc64), C.POINTER(C.c_void_p)]),
+    "rp_batch_set_state64": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
+    "rp_batch_set_loads64": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
+    "rp_batch_set_scales64": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "rp_batch_step64": (C.c_int, [C.c_void_p, C.POINTER(RpStepParams64), C.c_int]),
+    "rp_batch_get_state64": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
+    "rp_batch_precision": (C.c_int, [C.c_void_p]),
     "rp_tile_config_name": (C.c_char_p, [C.c_int]),
     "rp_tile_config_count": (C.c_int, []),
     "rp_last_error": (C.c_char_p, []),
@@ -158,6 +176,10 @@ def check(rc: int) -> None:
 
 def fptr(a):
     return None if a is None else a.ctypes.data_as(_fp)
+
+
+def dptr(a):
+    return None if a is None else a.ctypes.data_as(_dp)
 
 
 def iptr(a):

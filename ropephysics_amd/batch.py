@@ -6,7 +6,8 @@ Reference interface: ``src/gpu/gpu_solver.h:66-145``. Method names follow it
 ``set_material_scales``/``self_contact_overflow``/``launches_per_step``), with
 two extensions: a batch may hold rods of different shapes (a list of rods
 instead of one replicated prototype), and each rod may collide with its own
-world.
+world. ``precision=64`` makes an FP64 batch (``rp_batch_create64``): the
+reference CPU solver's own precision, computed on the GPU.
 """
 from __future__ import annotations
 
@@ -26,8 +27,39 @@ def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
-def _world_desc(w: CollisionWorld):
-    prims = (_abi.RpPrimitive * max(1, len(w.primitives)))()
+class _Prec:
+    """The ABI pieces of one precision."""
+
+    def __init__(self, bits):
+        self.bits = bits
+        self.dtype = np.float32 if bits == 32 else np.float64
+        self.suffix = "" if bits == 32 else "64"
+        self.ptr = _abi.fptr if bits == 32 else _abi.dptr
+        self.real = C.c_float if bits == 32 else C.c_double
+        self.Primitive = getattr(_abi, "RpPrimitive" + self.suffix)
+        self.WorldDesc = getattr(_abi, "RpWorldDesc" + self.suffix)
+        self.BatchDesc = getattr(_abi, "RpBatchDesc" + self.suffix)
+        self.StepParams = getattr(_abi, "RpStepParams" + self.suffix)
+
+    def arr(self, a):
+        return np.ascontiguousarray(a, dtype=self.dtype)
+
+    def fn(self, name):
+        return getattr(_abi.lib(), name + self.suffix)
+
+
+_PREC = {32: _Prec(32), 64: _Prec(64)}
+
+
+def _prec(bits) -> _Prec:
+    if bits not in _PREC:
+        raise ValueError("precision must be 32 or 64")
+    return _PREC[bits]
+
+
+def _world_desc(w: CollisionWorld, precision: int = 32):
+    pr = _prec(precision)
+    prims = (pr.Primitive * max(1, len(w.primitives)))()
     for k, p in enumerate(w.primitives):
         c = prims[k]
         c.type = int(p.type)
@@ -38,17 +70,20 @@ def _world_desc(w: CollisionWorld):
         c.rotation[:] = list(map(float, p.rotation))
         c.radius = float(p.radius)
         c.friction = float(p.friction)
-    d = _abi.RpWorldDesc(len(w.primitives), C.cast(prims, C.POINTER(_abi.RpPrimitive)),
+    d = pr.WorldDesc(len(w.primitives), C.cast(prims, C.POINTER(pr.Primitive)),
                          float(w.contact_margin_radii), int(w.self_collision), float(w.self_friction),
                          int(w.self_skip), int(w.hash_table_size), int(w.self_capacity))
     return d, prims
 
 
-def describe(rods: list, world=None, world_of_rod=None):
-    """An ``rp_batch_desc`` for ``rods`` (ragged CSR over the list) and their
-    world(s): one ``CollisionWorld`` for every rod, or a list chosen per rod by
+def describe(rods: list, world=None, world_of_rod=None, precision: int = 32):
+    """An ``rp_batch_desc`` (``rp_batch_desc64`` for precision 64) for
+    ``rods`` (ragged CSR over the list) and their world(s): one
+    ``CollisionWorld`` for every rod, or a list chosen per rod by
     ``world_of_rod``. Returns the descriptor, the arrays it points into (keep
     them alive while it is used) and the four CSR start arrays."""
+    pr = _prec(precision)
+    _r = pr.arr
     R = len(rods)
     if R == 0:
         raise ValueError("a batch needs at least one rod")
@@ -68,12 +103,12 @@ def describe(rods: list, world=None, world_of_rod=None):
 
     keep = [starts]
     arr = {
-        "inv_mass": _f32(cat("invMass")), "inv_inertia": _f32(cat("invInertia")), "inertia": _f32(cat("inertia")),
+        "inv_mass": _r(cat("invMass")), "inv_inertia": _r(cat("invInertia")), "inertia": _r(cat("inertia")),
         "stretch_p0": _i32(cat("sP0")), "stretch_p1": _i32(cat("sP1")), "stretch_seg": _i32(cat("sSeg")),
-        "stretch_rest": _f32(cat("sRest")), "stretch_compliance": _f32(cat("sComp")),
-        "bend_a": _i32(cat("bA")), "bend_b": _i32(cat("bB")), "bend_rest": _f32(cat("bRest")),
-        "bend_compliance": _f32(cat("bComp")), "bend_rest_darboux": _f32(cat("bRestD")),
-        "radius": _f32([r.material.radius for r in rods]),
+        "stretch_rest": _r(cat("sRest")), "stretch_compliance": _r(cat("sComp")),
+        "bend_a": _i32(cat("bA")), "bend_b": _i32(cat("bB")), "bend_rest": _r(cat("bRest")),
+        "bend_compliance": _r(cat("bComp")), "bend_rest_darboux": _r(cat("bRestD")),
+        "radius": _r([r.material.radius for r in rods]),
     }
     keep.append(arr)
     if world is None:
@@ -82,9 +117,9 @@ def describe(rods: list, world=None, world_of_rod=None):
         worlds = [world]
     else:
         worlds = list(world)
-    wdescs = (_abi.RpWorldDesc * max(1, len(worlds)))()
+    wdescs = (pr.WorldDesc * max(1, len(worlds)))()
     for i, w in enumerate(worlds):
-        d, prims = _world_desc(w)
+        d, prims = _world_desc(w, precision)
         keep.append(prims)
         wdescs[i] = d
     keep.append(wdescs)
@@ -92,24 +127,26 @@ def describe(rods: list, world=None, world_of_rod=None):
     if world_of_rod is not None:
         wor = _i32(world_of_rod)
         keep.append(wor)
-    desc = _abi.RpBatchDesc()
+    desc = pr.BatchDesc()
     desc.num_rods = R
     desc.particle_start = _abi.iptr(starts[0])
     desc.frame_start = _abi.iptr(starts[1])
     desc.stretch_start = _abi.iptr(starts[2])
     desc.bend_start = _abi.iptr(starts[3])
     for k, a in arr.items():
-        setattr(desc, k, _abi.fptr(a) if a.dtype == np.float32 else _abi.iptr(a))
+        setattr(desc, k, pr.ptr(a) if a.dtype == pr.dtype else _abi.iptr(a))
     desc.num_worlds = len(worlds)
-    desc.worlds = C.cast(wdescs, C.POINTER(_abi.RpWorldDesc))
+    desc.worlds = C.cast(wdescs, C.POINTER(pr.WorldDesc))
     desc.world_of_rod = _abi.iptr(wor)
     return desc, keep, starts
 
 
-def step_params(p: SolverParams) -> _abi.RpStepParams:
-    g = (C.c_float * 3)(*map(float, p.gravity))
-    return _abi.RpStepParams(float(p.dt), int(p.substeps), int(p.iterations), g, float(p.linear_damping),
-                             float(p.angular_damping), int(p.contact_interval))
+def step_params(p: SolverParams, precision: int = 32):
+    """``rp_step_params`` (or ``rp_step_params64``) for ``p``."""
+    pr = _prec(precision)
+    g = (pr.real * 3)(*map(float, p.gravity))
+    return pr.StepParams(float(p.dt), int(p.substeps), int(p.iterations), g, float(p.linear_damping),
+                         float(p.angular_damping), int(p.contact_interval))
 
 
 class Batch:
@@ -118,25 +155,28 @@ class Batch:
     ``Batch(prototype, num_rods)`` replicates one rod as the reference's
     ``Batch::create`` does; ``Batch([rod_a, rod_b, ...])`` packs rods of any
     shapes. ``worlds`` is one ``CollisionWorld`` for every rod or a list, with
-    ``world_of_rod`` choosing per rod.
+    ``world_of_rod`` choosing per rod. ``precision`` is 32 (the reference GPU
+    path's FP32) or 64 (its CPU path's FP64); state arrays come back in it.
     """
 
     def __init__(self, rods, num_rods: int | None = None, strategy: str = "auto", world=None,
-                 world_of_rod=None, tile_config: int = 0):
+                 world_of_rod=None, tile_config: int = 0, precision: int = 32):
         self._h = None
+        self._p = _prec(precision)
+        self.precision = precision
         if isinstance(rods, Rod):
             rods = [rods] * int(num_rods if num_rods is not None else 1)
         self.rods = list(rods)
         R = len(self.rods)
         if R == 0:
             raise ValueError("a batch needs at least one rod")
-        desc, keep, starts = describe(self.rods, world, world_of_rod)
+        desc, keep, starts = describe(self.rods, world, world_of_rod, precision)
         self._keep = keep
         self.particle_start, self.frame_start, self.stretch_start, self.bend_start = starts
         desc.strategy = _abi.STRATEGY[strategy]
         desc.tile_config = int(tile_config)
         h = C.c_void_p()
-        _abi.check(_abi.lib().rp_batch_create(C.byref(desc), C.byref(h)))
+        _abi.check(self._p.fn("rp_batch_create")(C.byref(desc), C.byref(h)))
         self._h = h
         self.num_rods = R
         self.upload_all(self.rods)
@@ -161,29 +201,30 @@ class Batch:
     def set_state(self, first: int, count: int, x=None, v=None, q=None, omega=None):
         """Bulk state upload for rods [first, first + count), packed per rod."""
         P, F = self._range_sizes(first, count)
-        arrs = [None if a is None else _f32(a).reshape(-1) for a in (x, v, q, omega)]
+        arrs = [None if a is None else self._p.arr(a).reshape(-1) for a in (x, v, q, omega)]
         for a, n in zip(arrs, (P * 3, P * 3, F * 4, F * 3)):
             if a is not None and a.size != n:
                 raise ValueError(f"state array has {a.size} values, expected {n}")
-        _abi.check(_abi.lib().rp_batch_set_state(self._h, first, count, *[_abi.fptr(a) for a in arrs]))
+        _abi.check(self._p.fn("rp_batch_set_state")(self._h, first, count, *[self._p.ptr(a) for a in arrs]))
 
     def get_state(self, first: int = 0, count: int | None = None) -> dict:
         """Bulk read-back of x, v, q, omega for rods [first, first + count)."""
         count = self.num_rods - first if count is None else count
         P, F = self._range_sizes(first, count)
-        out = {"x": np.empty((P, 3), np.float32), "v": np.empty((P, 3), np.float32),
-               "q": np.empty((F, 4), np.float32), "omega": np.empty((F, 3), np.float32)}
-        _abi.check(_abi.lib().rp_batch_get_state(self._h, first, count, *[_abi.fptr(out[k]) for k in
-                                                                            ("x", "v", "q", "omega")]))
+        dt = self._p.dtype
+        out = {"x": np.empty((P, 3), dt), "v": np.empty((P, 3), dt), "q": np.empty((F, 4), dt),
+               "omega": np.empty((F, 3), dt)}
+        _abi.check(self._p.fn("rp_batch_get_state")(self._h, first, count,
+                                                    *[self._p.ptr(out[k]) for k in ("x", "v", "q", "omega")]))
         return out
 
     def set_loads_range(self, first: int, count: int, force=None, kin_vel=None, torque=None, fixed_q=None):
         P, F = self._range_sizes(first, count)
-        arrs = [None if a is None else _f32(a).reshape(-1) for a in (force, kin_vel, torque, fixed_q)]
+        arrs = [None if a is None else self._p.arr(a).reshape(-1) for a in (force, kin_vel, torque, fixed_q)]
         for a, n in zip(arrs, (P * 3, P * 3, F * 3, F * 4)):
             if a is not None and a.size != n:
                 raise ValueError(f"load array has {a.size} values, expected {n}")
-        _abi.check(_abi.lib().rp_batch_set_loads(self._h, first, count, *[_abi.fptr(a) for a in arrs]))
+        _abi.check(self._p.fn("rp_batch_set_loads")(self._h, first, count, *[self._p.ptr(a) for a in arrs]))
 
     def upload_all(self, rods):
         """Every rod's full state and loads, one bulk copy per field."""
@@ -215,16 +256,16 @@ class Batch:
         self.set_loads_range(0, self.num_rods, kin_vel=rod_major)
 
     def set_material_scales(self, youngs_scale=None, friction_scale=None):
-        ys = None if youngs_scale is None or len(youngs_scale) == 0 else _f32(youngs_scale)
-        fs = None if friction_scale is None or len(friction_scale) == 0 else _f32(friction_scale)
+        ys = None if youngs_scale is None or len(youngs_scale) == 0 else self._p.arr(youngs_scale)
+        fs = None if friction_scale is None or len(friction_scale) == 0 else self._p.arr(friction_scale)
         for a in (ys, fs):
             if a is not None and a.size != self.num_rods:
                 raise ValueError(f"scale array has {a.size} values for {self.num_rods} rods")
-        _abi.check(_abi.lib().rp_batch_set_scales(self._h, _abi.fptr(ys), _abi.fptr(fs)))
+        _abi.check(self._p.fn("rp_batch_set_scales")(self._h, self._p.ptr(ys), self._p.ptr(fs)))
 
     def step(self, params: SolverParams, steps: int = 1):
-        sp = step_params(params)
-        _abi.check(_abi.lib().rp_batch_step(self._h, C.byref(sp), int(steps)))
+        sp = step_params(params, self.precision)
+        _abi.check(self._p.fn("rp_batch_step")(self._h, C.byref(sp), int(steps)))
 
     # ------------------------------------------------------------------ drivers and observers
 
@@ -292,8 +333,8 @@ class Batch:
 
     def download_positions(self) -> np.ndarray:
         P = int(self.particle_start[-1])
-        x = np.empty((P, 3), np.float32)
-        _abi.check(_abi.lib().rp_batch_get_state(self._h, 0, self.num_rods, _abi.fptr(x), None, None, None))
+        x = np.empty((P, 3), self._p.dtype)
+        _abi.check(self._p.fn("rp_batch_get_state")(self._h, 0, self.num_rods, self._p.ptr(x), None, None, None))
         return x
 
     def counters(self) -> dict:

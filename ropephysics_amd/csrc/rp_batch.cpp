@@ -1,6 +1,7 @@
 // Host side of the C ABI in include/ropephysics.h: validation, colouring,
 // layout, tile packing, the tile-shape autotuner, HIP-graph replay of step
-// sequences, and bulk transfers.
+// sequences, and bulk transfers, written once over the scalar for the FP32
+// and FP64 batches.
 //
 // Reference counterpart: crs::gpu::Batch (src/gpu/gpu_batch.cpp:121-484). The
 // differences are deliberate: rods may differ in topology (ragged CSR instead
@@ -11,14 +12,12 @@
 
 #include <algorithm>
 #include <array>
-#include <cstdlib>
-#include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
-#include <set>
 #include <string>
 #include <vector>
 
@@ -48,7 +47,7 @@ struct PackedTiles : Packing {
 
 struct GraphKey {
     int strategy, config, steps;
-    float dt, gx, gy, gz, lin, ang;
+    double dt, gx, gy, gz, lin, ang;
     int substeps, iterations, interval;
     bool operator<(const GraphKey& o) const {
         return std::memcmp(this, &o, sizeof(GraphKey)) < 0;
@@ -57,10 +56,11 @@ struct GraphKey {
 
 }  // namespace
 
+// What a batch holds whatever its precision.
 struct rp_batch_impl {
+    int bits = 32;  // arithmetic precision
     int numRods = 0;
     std::vector<int> pStart, fStart, sStart, bStart;
-    std::vector<RodInfo> rods;
     std::vector<unsigned char> tileable;
     int strategy = RP_STRATEGY_PLAIN;
     int requestedConfig = 0;
@@ -71,13 +71,14 @@ struct rp_batch_impl {
 
     // device
     std::vector<std::unique_ptr<DevBuf>> owned;
-    DeviceArrays d{};
-    float* stateBackup = nullptr;  // autotuner scratch (x, v, q, omega)
-    float* fixedStage = nullptr;
+    void* stateBackup = nullptr;  // autotuner scratch (x, v, q, omega, clocks, counters, observers)
     size_t deviceBytes = 0;
+    int* selfOverflow = nullptr;
+    int* clock = nullptr;
+    PathRec* paths = nullptr;
+    ObsRec* obs = nullptr;
 
     std::unique_ptr<PackedTiles> tiles;
-    bool tuned = false;
     int stepsPerLaunch = 64;
     long long stepsTaken = 0;
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -89,9 +90,6 @@ struct rp_batch_impl {
     bool evPending = false;
     int lastLaunches = 0;
     float lastMs = 0.0f;
-    PathRec* paths = nullptr;
-    ObsRec* obs = nullptr;
-    int* clock = nullptr;
 
     // Device pointers are baked into captured graphs; drop them when one changes.
     void drop_graphs() {
@@ -100,7 +98,10 @@ struct rp_batch_impl {
         graphs.clear();
     }
 
-    ~rp_batch_impl() {
+    // Point the kernels' argument block at the path and observer tables.
+    virtual void bind_tables() = 0;
+
+    virtual ~rp_batch_impl() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -131,24 +132,49 @@ struct rp_batch_impl {
 
 namespace {
 
-}  // namespace
+// A batch whose kernels compute in Real.
+template <typename Real>
+struct BatchT final : rp_batch_impl {
+    DeviceArraysT<Real> d{};
+    std::vector<RodInfoT<Real>> rods;
+    Real* fixedStage = nullptr;
 
-namespace {
+    BatchT() { bits = Prec<Real>::bits; }
+    void bind_tables() override {
+        d.paths = paths;
+        d.obs = obs;
+    }
+};
+
+// The batch behind a handle, which must compute in Real.
+template <typename Real>
+int as_precision(void* batch, BatchT<Real>*& out) {
+    out = nullptr;
+    if (!batch) return fail(RP_ERR_INVALID, "null batch");
+    rp_batch_impl* base = static_cast<rp_batch_impl*>(batch);
+    if (base->bits != Prec<Real>::bits)
+        return fail(RP_ERR_STATE, "an FP" + std::to_string(base->bits) + " batch takes the " +
+                                      (base->bits == 64 ? "*64 calls" : "FP32 calls") + " for state and steps");
+    out = static_cast<BatchT<Real>*>(base);
+    return RP_OK;
+}
 
 int lds_budget(const rp_batch_impl& b, int perCU) { return lds_budget(LdsLimits{b.ldsLimit, b.cuLds}, perCU); }
 
 // All rods of the batch packed at one LDS budget; rods that cannot run TILED
 // make it fail.
-int pack_tiles(rp_batch_impl& b, int configId, int ldsCap, const std::vector<int>& colStart, PackedTiles& out) {
+template <typename Real>
+int pack_tiles(BatchT<Real>& b, int configId, int ldsCap, PackedTiles& out) {
+    if (!tile_config_available<Real>(configId)) return RP_ERR_CAPACITY;
     std::vector<int> ids(b.numRods);
     for (int r = 0; r < b.numRods; ++r) {
         if (!b.tileable[r]) return RP_ERR_CAPACITY;
         ids[r] = r;
     }
-    return pack_rods(b.rods, ids, colStart, configId, ldsCap, out);
+    return pack_rods(b.rods, ids, b.colStart, configId, ldsCap, out);
 }
 
-int upload_tiles(rp_batch_impl& b, PackedTiles& t) {
+int upload_tiles(PackedTiles& t) {
     auto put = [&](DevBuf& buf, const void* src, size_t bytes) -> int {
         if (buf.p) (void)hipFree(buf.p);
         buf.p = nullptr;
@@ -159,25 +185,25 @@ int upload_tiles(rp_batch_impl& b, PackedTiles& t) {
     };
     int rc = put(t.dTiles, t.tiles.data(), t.tiles.size() * sizeof(Tile));
     if (rc) return rc;
-    rc = put(t.dSlots, t.slots.data(), t.slots.size() * sizeof(TileSlot));
-    (void)b;
-    return rc;
+    return put(t.dSlots, t.slots.data(), t.slots.size() * sizeof(TileSlot));
 }
 
-int launch_steps(rp_batch_impl& b, const rp_step_params& p, int steps, int strategy, PackedTiles* t) {
+template <typename Real>
+int launch_steps(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int steps, int strategy,
+                 PackedTiles* t) {
     if (strategy == RP_STRATEGY_PLAIN) {
-        const StepArgs a = step_args(p, 1);
-        for (int s = 0; s < steps; ++s) RP_HIP(launch_plain(b.d, a, size_t(b.plainLds), b.stream));
+        const StepArgsT<Real> a = step_args<Real>(p, 1);
+        for (int s = 0; s < steps; ++s) RP_HIP(Prec<Real>::launch_plain(b.d, a, size_t(b.plainLds), b.stream));
         return RP_OK;
     }
     const TileConfig& cfg = tile_config(t->config);
     int left = steps;
     while (left > 0) {
         const int n = std::min(left, b.stepsPerLaunch);
-        const StepArgs a = step_args(p, n);
-        RP_HIP(launch_tiled(cfg.tp, cfg.tc, b.anySelf, b.d, a, static_cast<const Tile*>(t->dTiles.p),
-                            static_cast<const TileSlot*>(t->dSlots.p), int(t->tiles.size()), 64 * cfg.waves,
-                            size_t(t->maxLds), b.stream));
+        const StepArgsT<Real> a = step_args<Real>(p, n);
+        RP_HIP(Prec<Real>::launch_tiled(cfg.tp, cfg.tc, b.anySelf, b.d, a, static_cast<const Tile*>(t->dTiles.p),
+                                        static_cast<const TileSlot*>(t->dSlots.p), int(t->tiles.size()),
+                                        64 * cfg.waves, size_t(t->maxLds), b.stream));
         left -= n;
     }
     return RP_OK;
@@ -185,23 +211,19 @@ int launch_steps(rp_batch_impl& b, const rp_step_params& p, int steps, int strat
 
 // Device state a trial run changes: positions, velocities, orientations,
 // angular velocities, clocks, overflow counters and observers.
-struct StateBackup {
-    std::vector<std::pair<void*, size_t>> fields;
-    void* store = nullptr;
-    size_t bytes = 0;
-};
-
-size_t backup_bytes(const rp_batch_impl& b) {
+template <typename Real>
+size_t backup_bytes(const BatchT<Real>& b) {
     const size_t P = size_t(b.pStart[b.numRods]), F = size_t(b.fStart[b.numRods]), R = size_t(b.numRods);
-    return (P * 6 + F * 7) * sizeof(float) + 2 * R * sizeof(int) + (b.obs ? R * sizeof(ObsRec) : 0);
+    return (P * 6 + F * 7) * sizeof(Real) + 2 * R * sizeof(int) + (b.obs ? R * sizeof(ObsRec) : 0);
 }
 
-int backup_state(rp_batch_impl& b, bool restore) {
+template <typename Real>
+int backup_state(BatchT<Real>& b, bool restore) {
     const size_t P = size_t(b.pStart[b.numRods]), F = size_t(b.fStart[b.numRods]), R = size_t(b.numRods);
-    char* s = reinterpret_cast<char*>(b.stateBackup);
-    std::pair<void*, size_t> fields[] = {{b.d.x, P * 12},          {b.d.v, P * 12},
-                                         {b.d.q, F * 16},          {b.d.omega, F * 12},
-                                         {b.clock, R * 4},         {b.d.selfOverflow, R * 4},
+    const size_t w = sizeof(Real);
+    char* s = static_cast<char*>(b.stateBackup);
+    std::pair<void*, size_t> fields[] = {{b.d.x, P * 3 * w},  {b.d.v, P * 3 * w},         {b.d.q, F * 4 * w},
+                                         {b.d.omega, F * 3 * w}, {b.clock, R * 4},       {b.selfOverflow, R * 4},
                                          {b.obs, b.obs ? R * sizeof(ObsRec) : 0}};
     for (const auto& f : fields) {
         if (!f.first || !f.second) continue;
@@ -215,18 +237,21 @@ int backup_state(rp_batch_impl& b, bool restore) {
     return RP_OK;
 }
 
-// Time every feasible tile configuration on this batch's own state for a few
-// steps, keep the fastest, and restore the state so the tuning leaves no trace.
-// Results do not depend on the configuration: each rod's arithmetic is the
-// same whatever lanes and workgroup it lands in.
 // Time each tile shape (or only `onlyConfig`) at LDS budgets for 1 to 4
-// workgroups per compute unit: the budget decides how many workgroups there
-// are and how many are resident at once, so a batch whose workgroups spill
-// just past a multiple of the resident count pays a whole extra round.
-int autotune(rp_batch_impl& b, const rp_step_params& p, int onlyConfig) {
+// workgroups per compute unit on this batch's own state for a few steps, keep
+// the fastest, and restore the state so the tuning leaves no trace. The
+// budget decides how many workgroups there are and how many are resident at
+// once, so a batch whose workgroups spill just past a multiple of the
+// resident count pays a whole extra round. Results do not depend on the
+// shape: each rod's arithmetic is the same whatever lanes and workgroup it
+// lands in.
+template <typename Real>
+int autotune(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int onlyConfig) {
     if (!b.stateBackup) {
-        int rc = b.alloc(b.stateBackup, (backup_bytes(b) + sizeof(float) * 2 + 0) / sizeof(float));
+        char* store = nullptr;
+        int rc = b.alloc(store, backup_bytes(b));
         if (rc) return rc;
+        b.stateBackup = store;
     }
     int rc = backup_state(b, false);
     if (rc) return rc;
@@ -237,32 +262,33 @@ int autotune(rp_batch_impl& b, const rp_step_params& p, int onlyConfig) {
     RP_HIP(hipEventCreate(&e1));
     const int trialSteps = 8;
     b.tuneLog.clear();
-    for (int id = onlyConfig > 0 ? onlyConfig : 1; id <= (onlyConfig > 0 ? onlyConfig : num_tile_configs()); ++id)
-      for (int perCU = 1, lastTiles = -1; perCU <= 4; ++perCU) {
-        auto t = std::make_unique<PackedTiles>();
-        if (pack_tiles(b, id, lds_budget(b, perCU), b.colStart, *t) != RP_OK) continue;
-        // A budget that does not change the packing needs no second trial.
-        if (int(t->tiles.size()) == lastTiles) continue;
-        lastTiles = int(t->tiles.size());
-        if (upload_tiles(b, *t) != RP_OK) continue;
-        // A warm-up launch (code object load), then the best of two timed ones.
-        if (launch_steps(b, p, 1, RP_STRATEGY_TILED, t.get()) != RP_OK) continue;
-        float ms = 1e30f;
-        for (int rep = 0; rep < 2; ++rep) {
-            RP_HIP(hipEventRecord(e0, b.stream));
-            if (launch_steps(b, p, trialSteps, RP_STRATEGY_TILED, t.get()) != RP_OK) break;
-            RP_HIP(hipEventRecord(e1, b.stream));
-            RP_HIP(hipEventSynchronize(e1));
-            float m = 0;
-            RP_HIP(hipEventElapsedTime(&m, e0, e1));
-            ms = std::min(ms, m);
+    const int first = onlyConfig > 0 ? onlyConfig : 1, last = onlyConfig > 0 ? onlyConfig : num_tile_configs();
+    for (int id = first; id <= last; ++id)
+        for (int perCU = 1, lastTiles = -1; perCU <= 4; ++perCU) {
+            auto t = std::make_unique<PackedTiles>();
+            if (pack_tiles(b, id, lds_budget(b, perCU), *t) != RP_OK) continue;
+            // A budget that does not change the packing needs no second trial.
+            if (int(t->tiles.size()) == lastTiles) continue;
+            lastTiles = int(t->tiles.size());
+            if (upload_tiles(*t) != RP_OK) continue;
+            // A warm-up launch (code object load), then the best of two timed ones.
+            if (launch_steps(b, p, 1, RP_STRATEGY_TILED, t.get()) != RP_OK) continue;
+            float ms = 1e30f;
+            for (int rep = 0; rep < 2; ++rep) {
+                RP_HIP(hipEventRecord(e0, b.stream));
+                if (launch_steps(b, p, trialSteps, RP_STRATEGY_TILED, t.get()) != RP_OK) break;
+                RP_HIP(hipEventRecord(e1, b.stream));
+                RP_HIP(hipEventSynchronize(e1));
+                float m = 0;
+                RP_HIP(hipEventElapsedTime(&m, e0, e1));
+                ms = std::min(ms, m);
+            }
+            b.tuneLog.push_back(std::make_pair(id, ms));
+            if (ms < best) {
+                best = ms;
+                bestTiles = std::move(t);
+            }
         }
-        b.tuneLog.push_back(std::make_pair(id, ms));
-        if (ms < best) {
-            best = ms;
-            bestTiles = std::move(t);
-        }
-      }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     rc = backup_state(b, true);
@@ -272,17 +298,14 @@ int autotune(rp_batch_impl& b, const rp_step_params& p, int onlyConfig) {
     return RP_OK;
 }
 
-}  // namespace
-
-namespace {
-
-int resolve_tiles(rp_batch_impl& b, const rp_step_params& p) {
+template <typename Real>
+int resolve_tiles(BatchT<Real>& b, const typename Prec<Real>::StepParams& p) {
     if (b.tiles) return RP_OK;
     if (b.requestedConfig > 0) {
         PackedTiles probe;
-        const int rc = pack_tiles(b, b.requestedConfig, lds_budget(b, 1), b.colStart, probe);
+        const int rc = pack_tiles(b, b.requestedConfig, lds_budget(b, 1), probe);
         if (rc) return fail(rc, std::string("tile configuration ") + tile_config(b.requestedConfig).name +
-                                    " does not fit this batch");
+                                    " does not fit this batch in FP" + std::to_string(b.bits));
         return autotune(b, p, b.requestedConfig);
     }
     return autotune(b, p, 0);
@@ -295,68 +318,25 @@ int check_range(const rp_batch_impl& b, int first, int count) {
     return RP_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char* rp_last_error(void) { return last_error(); }
-
-int rp_tile_config_count(void) { return num_tile_configs(); }
-
-const char* rp_tile_config_name(int id) {
-    if (id < 1 || id > num_tile_configs()) return nullptr;
-    return tile_config(id).name;
-}
-
-int rp_device_query(rp_device_info* out) {
-    if (!out) return fail(RP_ERR_INVALID, "null output");
-    std::memset(out, 0, sizeof(*out));
-    int n = 0;
-    const hipError_t e = hipGetDeviceCount(&n);
-    if (e != hipSuccess || n == 0)
-        return fail(RP_ERR_DEVICE, std::string("no usable HIP device: ") +
-                                       (e != hipSuccess ? hipGetErrorString(e) : "device count is 0"));
-    hipDeviceProp_t prop;
-    RP_HIP(hipGetDeviceProperties(&prop, 0));
-    std::snprintf(out->name, sizeof(out->name), "%s", prop.name);
-    std::snprintf(out->arch, sizeof(out->arch), "%s", prop.gcnArchName);
-    out->compute_units = prop.multiProcessorCount;
-    out->wavefront_size = prop.warpSize;
-    out->lds_per_workgroup = prop.sharedMemPerBlock;
-    out->total_global_mem = prop.totalGlobalMem;
-    out->clock_khz = prop.clockRate;
-    // Longest clamped chain (n + 1 particles, n + 1 frames, n stretch and n
-    // bend constraints, two colours of about n / 2 + 1) some configuration
-    // takes.
-    int best = 0;
-    for (int n2 = 4; n2 <= 4096; ++n2) {
-        const int lds = tiled_lds_bytes(n2 + 1, n2 + 1, n2, 0, 0, 1, 0);
-        if (size_t(lds) > prop.sharedMemPerBlock) break;
-        for (int id = 1; id <= num_tile_configs(); ++id) {
-            const TileConfig& c = tile_config(id);
-            const int lanes = std::max((n2 + 1 + c.tp - 1) / c.tp, (n2 / 2 + 1 + c.tc - 1) / c.tc);
-            if (lanes <= 64 * c.waves) best = n2;
-        }
-    }
-    out->max_tiled_segments = best;
-    return RP_OK;
-}
-
-int rp_batch_create(const rp_batch_desc* desc, void** out_batch) {
+template <typename Real>
+int create(const typename Prec<Real>::BatchDesc* desc, void** out_batch) {
     if (!desc || !out_batch) return fail(RP_ERR_INVALID, "null argument");
     *out_batch = nullptr;
     if (desc->strategy < RP_STRATEGY_PLAIN || desc->strategy > RP_STRATEGY_AUTO)
         return fail(RP_ERR_INVALID, "unknown strategy");
     if (desc->tile_config < 0 || desc->tile_config > num_tile_configs())
         return fail(RP_ERR_INVALID, "unknown tile configuration");
+    if (desc->tile_config > 0 && !tile_config_available<Real>(desc->tile_config))
+        return fail(RP_ERR_INVALID, std::string("tile configuration ") + tile_config(desc->tile_config).name +
+                                        " is not built for FP" + std::to_string(Prec<Real>::bits));
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(RP_ERR_DEVICE, "no usable HIP device");
 
-    Records rec;
-    int rc = build_records(desc, rec);
+    RecordsT<Real> rec;
+    int rc = build_records<Real>(desc, rec);
     if (rc) return rc;
     const int R = rec.R, P = rec.P, F = rec.F;
-    auto b = std::make_unique<rp_batch_impl>();
+    auto b = std::make_unique<BatchT<Real>>();
     b->numRods = R;
     b->requestedConfig = desc->tile_config;
     b->pStart = rec.pStart;
@@ -391,30 +371,30 @@ int rp_batch_create(const rp_batch_desc* desc, void** out_batch) {
 
     // ---- device arrays
     RP_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    float *x, *v, *q, *om, *force, *kin, *torque, *cs, *fs;
-    int* ovf;
+    Real *x, *v, *q, *om, *force, *kin, *torque, *cs, *fs;
     if ((rc = b->alloc(x, size_t(P) * 3)) || (rc = b->alloc(v, size_t(P) * 3)) || (rc = b->alloc(q, size_t(F) * 4)) ||
         (rc = b->alloc(om, size_t(F) * 3)) || (rc = b->alloc(force, size_t(P) * 3)) ||
         (rc = b->alloc(kin, size_t(P) * 3)) || (rc = b->alloc(torque, size_t(F) * 3)) || (rc = b->alloc(cs, R)) ||
-        (rc = b->alloc(fs, R)) || (rc = b->alloc(ovf, R)) || (rc = b->alloc(b->fixedStage, size_t(F) * 4)) ||
-        (rc = b->alloc(b->clock, R)))
+        (rc = b->alloc(fs, R)) || (rc = b->alloc(b->selfOverflow, R)) ||
+        (rc = b->alloc(b->fixedStage, size_t(F) * 4)) || (rc = b->alloc(b->clock, R)))
         return rc;
+    const size_t w = sizeof(Real);
     RP_HIP(hipMemset(b->clock, 0, size_t(R) * 4));
-    RP_HIP(hipMemset(x, 0, size_t(P) * 3 * 4));
-    RP_HIP(hipMemset(v, 0, size_t(P) * 3 * 4));
-    RP_HIP(hipMemset(om, 0, size_t(F) * 3 * 4));
-    RP_HIP(hipMemset(force, 0, size_t(P) * 3 * 4));
-    RP_HIP(hipMemset(kin, 0, size_t(P) * 3 * 4));
-    RP_HIP(hipMemset(torque, 0, size_t(F) * 3 * 4));
-    RP_HIP(hipMemset(ovf, 0, size_t(R) * 4));
+    RP_HIP(hipMemset(b->selfOverflow, 0, size_t(R) * 4));
+    RP_HIP(hipMemset(x, 0, size_t(P) * 3 * w));
+    RP_HIP(hipMemset(v, 0, size_t(P) * 3 * w));
+    RP_HIP(hipMemset(om, 0, size_t(F) * 3 * w));
+    RP_HIP(hipMemset(force, 0, size_t(P) * 3 * w));
+    RP_HIP(hipMemset(kin, 0, size_t(P) * 3 * w));
+    RP_HIP(hipMemset(torque, 0, size_t(F) * 3 * w));
     {
-        std::vector<float> qid(size_t(F) * 4, 0.0f), ones(R, 1.0f);
-        for (int j = 0; j < F; ++j) qid[4 * size_t(j)] = 1.0f;
-        RP_HIP(hipMemcpy(q, qid.data(), qid.size() * 4, hipMemcpyHostToDevice));
-        RP_HIP(hipMemcpy(cs, ones.data(), R * 4, hipMemcpyHostToDevice));
-        RP_HIP(hipMemcpy(fs, ones.data(), R * 4, hipMemcpyHostToDevice));
+        std::vector<Real> qid(size_t(F) * 4, Real(0)), ones(R, Real(1));
+        for (int j = 0; j < F; ++j) qid[4 * size_t(j)] = Real(1);
+        RP_HIP(hipMemcpy(q, qid.data(), qid.size() * w, hipMemcpyHostToDevice));
+        RP_HIP(hipMemcpy(cs, ones.data(), R * w, hipMemcpyHostToDevice));
+        RP_HIP(hipMemcpy(fs, ones.data(), R * w, hipMemcpyHostToDevice));
     }
-    DeviceArrays& d = b->d;
+    DeviceArraysT<Real>& d = b->d;
     d.x = x;
     d.v = v;
     d.q = q;
@@ -424,131 +404,148 @@ int rp_batch_create(const rp_batch_desc* desc, void** out_batch) {
     d.torque = torque;
     d.complianceScale = cs;
     d.frictionScale = fs;
-    d.selfOverflow = ovf;
+    d.selfOverflow = b->selfOverflow;
     d.clock = b->clock;
     d.paths = nullptr;
     d.obs = nullptr;
     d.stepLimit = nullptr;
     d.numRods = R;
-    float *dInvMass, *dI4, *dII4, *dsPar, *dbPar, *dbRest;
-    int *dsIdx, *dbIdx, *dSegEnd, *dCol;
-    RodInfo* dRods;
-    Prim* dPrims;
+    using Vec4 = typename Prec<Real>::Vec4;
+    Real* dInvMass;
+    int *dSegEnd, *dCol;
+    RodInfoT<Real>* dRods;
+    PrimT<Real>* dPrims;
     int4 *sIdx4, *bIdx4;
-    float4 *iI4, *I4, *sPar4, *bPar4, *bRest4;
+    Vec4 *iI4, *I4, *sPar4, *bPar4, *bRest4;
     if ((rc = b->upload(dInvMass, rec.invMass)) || (rc = b->upload(iI4, rec.iI4)) || (rc = b->upload(I4, rec.I4)) ||
         (rc = b->upload(sIdx4, rec.sIdx)) || (rc = b->upload(sPar4, rec.sPar)) || (rc = b->upload(bIdx4, rec.bIdx)) ||
         (rc = b->upload(bPar4, rec.bPar)) || (rc = b->upload(bRest4, rec.bRest4)) ||
         (rc = b->upload(dSegEnd, rec.segEnd)) || (rc = b->upload(dCol, rec.colStart)) ||
         (rc = b->upload(dRods, b->rods)) || (rc = b->upload(dPrims, rec.prims)))
         return rc;
-    dII4 = reinterpret_cast<float*>(iI4);
-    dI4 = reinterpret_cast<float*>(I4);
-    dsIdx = reinterpret_cast<int*>(sIdx4);
-    dsPar = reinterpret_cast<float*>(sPar4);
-    dbIdx = reinterpret_cast<int*>(bIdx4);
-    dbPar = reinterpret_cast<float*>(bPar4);
-    dbRest = reinterpret_cast<float*>(bRest4);
     d.invMass = dInvMass;
-    d.invInertia4 = dII4;
-    d.inertia4 = dI4;
-    d.sIdx = dsIdx;
-    d.sPar = dsPar;
-    d.bIdx = dbIdx;
-    d.bPar = dbPar;
-    d.bRest4 = dbRest;
+    d.invInertia4 = reinterpret_cast<const Real*>(iI4);
+    d.inertia4 = reinterpret_cast<const Real*>(I4);
+    d.sIdx = reinterpret_cast<const int*>(sIdx4);
+    d.sPar = reinterpret_cast<const Real*>(sPar4);
+    d.bIdx = reinterpret_cast<const int*>(bIdx4);
+    d.bPar = reinterpret_cast<const Real*>(bPar4);
+    d.bRest4 = reinterpret_cast<const Real*>(bRest4);
     d.segEnd = dSegEnd;
     d.colStart = dCol;
     d.rods = dRods;
     d.prims = dPrims;
-    *out_batch = b.release();
+    *out_batch = static_cast<rp_batch_impl*>(b.release());
     return RP_OK;
 }
 
-void rp_batch_destroy(void* batch) {
-    if (!batch) return;
-    rp_batch_impl* b = static_cast<rp_batch_impl*>(batch);
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
-    delete b;
-}
-
-static int copy_range(rp_batch_impl& b, int first, int count, float* dev, const float* host, float* hostOut,
-                      const std::vector<int>& startArr, int comps) {
+template <typename T>
+int copy_range(rp_batch_impl& b, int first, int count, T* dev, const T* host, T* hostOut,
+               const std::vector<int>& startArr, int comps) {
     const size_t e0 = size_t(startArr[first]), e1 = size_t(startArr[first + count]);
-    const size_t bytes = (e1 - e0) * comps * sizeof(float);
+    const size_t bytes = (e1 - e0) * comps * sizeof(T);
     if (bytes == 0) return RP_OK;
     if (host) RP_HIP(hipMemcpyAsync(dev + e0 * comps, host, bytes, hipMemcpyHostToDevice, b.stream));
     if (hostOut) RP_HIP(hipMemcpyAsync(hostOut, dev + e0 * comps, bytes, hipMemcpyDeviceToHost, b.stream));
     return RP_OK;
 }
 
-int rp_batch_set_state(void* batch, int first_rod, int num_rods, const float* x, const float* v, const float* q,
-                       const float* omega) {
-    if (!batch) return fail(RP_ERR_INVALID, "null batch");
-    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
-    int rc = check_range(b, first_rod, num_rods);
+template <typename Real>
+int set_state(void* batch, int first_rod, int num_rods, const Real* x, const Real* v, const Real* q,
+              const Real* omega) {
+    BatchT<Real>* bp;
+    int rc = as_precision(batch, bp);
     if (rc) return rc;
-    if ((rc = copy_range(b, first_rod, num_rods, b.d.x, x, nullptr, b.pStart, 3)) ||
-        (rc = copy_range(b, first_rod, num_rods, b.d.v, v, nullptr, b.pStart, 3)) ||
-        (rc = copy_range(b, first_rod, num_rods, b.d.q, q, nullptr, b.fStart, 4)) ||
-        (rc = copy_range(b, first_rod, num_rods, b.d.omega, omega, nullptr, b.fStart, 3)))
+    BatchT<Real>& b = *bp;
+    if ((rc = check_range(b, first_rod, num_rods))) return rc;
+    if ((rc = copy_range<Real>(b, first_rod, num_rods, b.d.x, x, nullptr, b.pStart, 3)) ||
+        (rc = copy_range<Real>(b, first_rod, num_rods, b.d.v, v, nullptr, b.pStart, 3)) ||
+        (rc = copy_range<Real>(b, first_rod, num_rods, b.d.q, q, nullptr, b.fStart, 4)) ||
+        (rc = copy_range<Real>(b, first_rod, num_rods, b.d.omega, omega, nullptr, b.fStart, 3)))
         return rc;
     RP_HIP(hipStreamSynchronize(b.stream));
     return RP_OK;
 }
 
-int rp_batch_set_loads(void* batch, int first_rod, int num_rods, const float* force, const float* kin_vel,
-                       const float* torque, const float* fixed_q) {
-    if (!batch) return fail(RP_ERR_INVALID, "null batch");
-    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
-    int rc = check_range(b, first_rod, num_rods);
+template <typename Real>
+int get_state(void* batch, int first_rod, int num_rods, Real* x, Real* v, Real* q, Real* omega) {
+    BatchT<Real>* bp;
+    int rc = as_precision(batch, bp);
     if (rc) return rc;
-    if ((rc = copy_range(b, first_rod, num_rods, const_cast<float*>(b.d.force), force, nullptr, b.pStart, 3)) ||
-        (rc = copy_range(b, first_rod, num_rods, const_cast<float*>(b.d.kinVel), kin_vel, nullptr, b.pStart, 3)) ||
-        (rc = copy_range(b, first_rod, num_rods, const_cast<float*>(b.d.torque), torque, nullptr, b.fStart, 3)))
+    BatchT<Real>& b = *bp;
+    if ((rc = check_range(b, first_rod, num_rods))) return rc;
+    if ((rc = copy_range<Real>(b, first_rod, num_rods, b.d.x, nullptr, x, b.pStart, 3)) ||
+        (rc = copy_range<Real>(b, first_rod, num_rods, b.d.v, nullptr, v, b.pStart, 3)) ||
+        (rc = copy_range<Real>(b, first_rod, num_rods, b.d.q, nullptr, q, b.fStart, 4)) ||
+        (rc = copy_range<Real>(b, first_rod, num_rods, b.d.omega, nullptr, omega, b.fStart, 3)))
+        return rc;
+    RP_HIP(hipStreamSynchronize(b.stream));
+    return RP_OK;
+}
+
+template <typename Real>
+int set_loads(void* batch, int first_rod, int num_rods, const Real* force, const Real* kin_vel, const Real* torque,
+              const Real* fixed_q) {
+    BatchT<Real>* bp;
+    int rc = as_precision(batch, bp);
+    if (rc) return rc;
+    BatchT<Real>& b = *bp;
+    if ((rc = check_range(b, first_rod, num_rods))) return rc;
+    if ((rc = copy_range<Real>(b, first_rod, num_rods, const_cast<Real*>(b.d.force), force, nullptr, b.pStart, 3)) ||
+        (rc = copy_range<Real>(b, first_rod, num_rods, const_cast<Real*>(b.d.kinVel), kin_vel, nullptr, b.pStart, 3)) ||
+        (rc = copy_range<Real>(b, first_rod, num_rods, const_cast<Real*>(b.d.torque), torque, nullptr, b.fStart, 3)))
         return rc;
     if (fixed_q) {
         const int f0 = b.fStart[first_rod], f1 = b.fStart[first_rod + num_rods];
         if (f1 > f0) {
-            RP_HIP(hipMemcpyAsync(b.fixedStage, fixed_q, size_t(f1 - f0) * 16, hipMemcpyHostToDevice, b.stream));
-            RP_HIP(launch_fixed_frames(b.d.q, b.fixedStage, b.d.invInertia4, f0, f1 - f0, b.stream));
+            RP_HIP(hipMemcpyAsync(b.fixedStage, fixed_q, size_t(f1 - f0) * 4 * sizeof(Real), hipMemcpyHostToDevice,
+                                  b.stream));
+            RP_HIP(Prec<Real>::launch_fixed_frames(b.d.q, b.fixedStage, b.d.invInertia4, f0, f1 - f0, b.stream));
         }
     }
     RP_HIP(hipStreamSynchronize(b.stream));
     return RP_OK;
 }
 
-int rp_batch_set_scales(void* batch, const float* youngs_scale, const float* friction_scale) {
-    if (!batch) return fail(RP_ERR_INVALID, "null batch");
-    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
+template <typename Real>
+int set_scales(void* batch, const Real* youngs_scale, const Real* friction_scale) {
+    BatchT<Real>* bp;
+    int rc = as_precision(batch, bp);
+    if (rc) return rc;
+    BatchT<Real>& b = *bp;
     if (youngs_scale) {
-        std::vector<float> c(b.numRods);
+        std::vector<Real> c(b.numRods);
         for (int r = 0; r < b.numRods; ++r) {
-            if (!(youngs_scale[r] > 0.0f))
+            if (!(youngs_scale[r] > Real(0)))
                 return fail(RP_ERR_INVALID, "youngs_scale[" + std::to_string(r) + "] must be positive");
             // Compliance scales as the inverse of Young's modulus
             // (src/gpu/gpu_batch.cpp:437-439).
-            c[r] = 1.0f / youngs_scale[r];
+            c[r] = Real(1) / youngs_scale[r];
         }
-        RP_HIP(hipMemcpyAsync(b.d.complianceScale, c.data(), b.numRods * 4, hipMemcpyHostToDevice, b.stream));
+        RP_HIP(hipMemcpyAsync(b.d.complianceScale, c.data(), b.numRods * sizeof(Real), hipMemcpyHostToDevice,
+                              b.stream));
         RP_HIP(hipStreamSynchronize(b.stream));
     }
     if (friction_scale) {
-        RP_HIP(hipMemcpyAsync(b.d.frictionScale, friction_scale, b.numRods * 4, hipMemcpyHostToDevice, b.stream));
+        RP_HIP(hipMemcpyAsync(b.d.frictionScale, friction_scale, b.numRods * sizeof(Real), hipMemcpyHostToDevice,
+                              b.stream));
         RP_HIP(hipStreamSynchronize(b.stream));
     }
     return RP_OK;
 }
 
-int rp_batch_step(void* batch, const rp_step_params* params, int num_steps) {
-    if (!batch || !params) return fail(RP_ERR_INVALID, "null argument");
-    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
-    const rp_step_params& p = *params;
+template <typename Real>
+int step(void* batch, const typename Prec<Real>::StepParams* params, int num_steps) {
+    if (!params) return fail(RP_ERR_INVALID, "null argument");
+    BatchT<Real>* bp;
+    int rc = as_precision(batch, bp);
+    if (rc) return rc;
+    BatchT<Real>& b = *bp;
+    const auto& p = *params;
     if (num_steps < 0) return fail(RP_ERR_INVALID, "num_steps must be non-negative");
-    if (p.substeps < 1 || p.iterations < 0 || !(p.dt > 0.0f))
+    if (p.substeps < 1 || p.iterations < 0 || !(p.dt > 0))
         return fail(RP_ERR_INVALID, "need substeps >= 1, iterations >= 0 and dt > 0");
     if (num_steps == 0) return RP_OK;
-    int rc = 0;
     if (b.strategy == RP_STRATEGY_TILED && (rc = resolve_tiles(b, p))) return rc;
     // Steps run in chunks of stepsPerLaunch: one TILED launch, or that many
     // PLAIN launches. Each chunk shape is captured once into a HIP graph
@@ -611,6 +608,108 @@ int rp_batch_step(void* batch, const rp_step_params* params, int num_steps) {
     return RP_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+const char* rp_last_error(void) { return last_error(); }
+
+int rp_tile_config_count(void) { return num_tile_configs(); }
+
+const char* rp_tile_config_name(int id) {
+    if (id < 1 || id > num_tile_configs()) return nullptr;
+    return tile_config(id).name;
+}
+
+int rp_device_query(rp_device_info* out) {
+    if (!out) return fail(RP_ERR_INVALID, "null output");
+    std::memset(out, 0, sizeof(*out));
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return fail(RP_ERR_DEVICE, std::string("no usable HIP device: ") +
+                                       (e != hipSuccess ? hipGetErrorString(e) : "device count is 0"));
+    hipDeviceProp_t prop;
+    RP_HIP(hipGetDeviceProperties(&prop, 0));
+    std::snprintf(out->name, sizeof(out->name), "%s", prop.name);
+    std::snprintf(out->arch, sizeof(out->arch), "%s", prop.gcnArchName);
+    out->compute_units = prop.multiProcessorCount;
+    out->wavefront_size = prop.warpSize;
+    out->lds_per_workgroup = prop.sharedMemPerBlock;
+    out->total_global_mem = prop.totalGlobalMem;
+    out->clock_khz = prop.clockRate;
+    // Longest clamped chain (n + 1 particles, n + 1 frames, n stretch and n
+    // bend constraints, two colours of about n / 2 + 1) some FP32
+    // configuration takes.
+    int best = 0;
+    for (int n2 = 4; n2 <= 4096; ++n2) {
+        const int lds = Prec<float>::tiled_lds_bytes(n2 + 1, n2 + 1, n2, 0, 0, 1, 0);
+        if (size_t(lds) > prop.sharedMemPerBlock) break;
+        for (int id = 1; id <= num_tile_configs(); ++id) {
+            const TileConfig& c = tile_config(id);
+            const int lanes = std::max((n2 + 1 + c.tp - 1) / c.tp, (n2 / 2 + 1 + c.tc - 1) / c.tc);
+            if (lanes <= 64 * c.waves) best = n2;
+        }
+    }
+    out->max_tiled_segments = best;
+    return RP_OK;
+}
+
+int rp_batch_create(const rp_batch_desc* desc, void** out_batch) { return create<float>(desc, out_batch); }
+int rp_batch_create64(const rp_batch_desc64* desc, void** out_batch) { return create<double>(desc, out_batch); }
+
+void rp_batch_destroy(void* batch) {
+    if (!batch) return;
+    rp_batch_impl* b = static_cast<rp_batch_impl*>(batch);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    delete b;
+}
+
+int rp_batch_precision(void* batch) {
+    if (!batch) return fail(RP_ERR_INVALID, "null batch");
+    return static_cast<rp_batch_impl*>(batch)->bits;
+}
+
+int rp_batch_set_state(void* batch, int first_rod, int num_rods, const float* x, const float* v, const float* q,
+                       const float* omega) {
+    return set_state<float>(batch, first_rod, num_rods, x, v, q, omega);
+}
+int rp_batch_set_state64(void* batch, int first_rod, int num_rods, const double* x, const double* v,
+                         const double* q, const double* omega) {
+    return set_state<double>(batch, first_rod, num_rods, x, v, q, omega);
+}
+
+int rp_batch_set_loads(void* batch, int first_rod, int num_rods, const float* force, const float* kin_vel,
+                       const float* torque, const float* fixed_q) {
+    return set_loads<float>(batch, first_rod, num_rods, force, kin_vel, torque, fixed_q);
+}
+int rp_batch_set_loads64(void* batch, int first_rod, int num_rods, const double* force, const double* kin_vel,
+                         const double* torque, const double* fixed_q) {
+    return set_loads<double>(batch, first_rod, num_rods, force, kin_vel, torque, fixed_q);
+}
+
+int rp_batch_set_scales(void* batch, const float* youngs_scale, const float* friction_scale) {
+    return set_scales<float>(batch, youngs_scale, friction_scale);
+}
+int rp_batch_set_scales64(void* batch, const double* youngs_scale, const double* friction_scale) {
+    return set_scales<double>(batch, youngs_scale, friction_scale);
+}
+
+int rp_batch_step(void* batch, const rp_step_params* params, int num_steps) {
+    return step<float>(batch, params, num_steps);
+}
+int rp_batch_step64(void* batch, const rp_step_params64* params, int num_steps) {
+    return step<double>(batch, params, num_steps);
+}
+
+int rp_batch_get_state(void* batch, int first_rod, int num_rods, float* x, float* v, float* q, float* omega) {
+    return get_state<float>(batch, first_rod, num_rods, x, v, q, omega);
+}
+int rp_batch_get_state64(void* batch, int first_rod, int num_rods, double* x, double* v, double* q,
+                         double* omega) {
+    return get_state<double>(batch, first_rod, num_rods, x, v, q, omega);
+}
+
 int rp_batch_set_paths(void* batch, int first_rod, int num_rods, const rp_path_desc* paths) {
     if (!batch) return fail(RP_ERR_INVALID, "null batch");
     rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
@@ -625,7 +724,7 @@ int rp_batch_set_paths(void* batch, int first_rod, int num_rods, const rp_path_d
         }
         RP_HIP(hipMemcpy(b.paths, none.data(), none.size() * sizeof(PathRec), hipMemcpyHostToDevice));
         b.drop_graphs();
-        b.d.paths = b.paths;
+        b.bind_tables();
     }
     std::vector<PathRec> recs(num_rods);
     for (int i = 0; i < num_rods; ++i) {
@@ -634,9 +733,8 @@ int rp_batch_set_paths(void* batch, int first_rod, int num_rods, const rp_path_d
         p.particle = -1;
         if (!paths || paths[i].particle < 0) continue;
         const rp_path_desc& s = paths[i];
-        const RodInfo& ri = b.rods[first_rod + i];
-        if (s.particle >= ri.nP)
-            return fail(RP_ERR_INVALID, "path " + std::to_string(i) + ": particle out of range");
+        const int nP = b.pStart[first_rod + i + 1] - b.pStart[first_rod + i];
+        if (s.particle >= nP) return fail(RP_ERR_INVALID, "path " + std::to_string(i) + ": particle out of range");
         if (s.num_waypoints < 1 || s.num_waypoints > kMaxWaypoints || !s.waypoints || !s.leg_seconds)
             return fail(RP_ERR_INVALID, "path " + std::to_string(i) + ": 1.." + std::to_string(kMaxWaypoints) +
                                             " waypoints with leg durations required");
@@ -700,13 +798,14 @@ int rp_batch_set_observers(void* batch, int first_rod, int num_rods, const rp_ob
         std::vector<ObsRec> none(b.numRods, ObsRec{-1, 1, 0.0, 0.0});
         RP_HIP(hipMemcpy(b.obs, none.data(), none.size() * sizeof(ObsRec), hipMemcpyHostToDevice));
         b.drop_graphs();
-        b.d.obs = b.obs;
+        b.bind_tables();
     }
     std::vector<ObsRec> recs(num_rods, ObsRec{-1, 1, 0.0, 0.0});
     for (int i = 0; observers && i < num_rods; ++i) {
         const rp_observer_desc& o = observers[i];
         if (o.particle < 0) continue;
-        if (o.particle >= b.rods[first_rod + i].nP || o.every < 1)
+        const int nP = b.pStart[first_rod + i + 1] - b.pStart[first_rod + i];
+        if (o.particle >= nP || o.every < 1)
             return fail(RP_ERR_INVALID, "observer " + std::to_string(i) + ": bad particle or interval");
         recs[i] = ObsRec{o.particle, o.every, o.z0, 0.0};
     }
@@ -759,27 +858,13 @@ int rp_batch_sync(void* batch) {
     return RP_OK;
 }
 
-int rp_batch_get_state(void* batch, int first_rod, int num_rods, float* x, float* v, float* q, float* omega) {
-    if (!batch) return fail(RP_ERR_INVALID, "null batch");
-    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
-    int rc = check_range(b, first_rod, num_rods);
-    if (rc) return rc;
-    if ((rc = copy_range(b, first_rod, num_rods, b.d.x, nullptr, x, b.pStart, 3)) ||
-        (rc = copy_range(b, first_rod, num_rods, b.d.v, nullptr, v, b.pStart, 3)) ||
-        (rc = copy_range(b, first_rod, num_rods, b.d.q, nullptr, q, b.fStart, 4)) ||
-        (rc = copy_range(b, first_rod, num_rods, b.d.omega, nullptr, omega, b.fStart, 3)))
-        return rc;
-    RP_HIP(hipStreamSynchronize(b.stream));
-    return RP_OK;
-}
-
 int rp_batch_counters(void* batch, rp_counters* out) {
     if (!batch || !out) return fail(RP_ERR_INVALID, "null argument");
     rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
     std::memset(out, 0, sizeof(*out));
     std::vector<int> ovf(b.numRods);
     RP_HIP(hipStreamSynchronize(b.stream));
-    RP_HIP(hipMemcpy(ovf.data(), b.d.selfOverflow, b.numRods * 4, hipMemcpyDeviceToHost));
+    RP_HIP(hipMemcpy(ovf.data(), b.selfOverflow, b.numRods * 4, hipMemcpyDeviceToHost));
     for (int v : ovf) out->self_contact_overflow += v;
     if (b.timing && b.evPending) {
         RP_HIP(hipEventElapsedTime(&b.lastMs, b.ev0, b.ev1));

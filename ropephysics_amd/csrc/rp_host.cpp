@@ -87,9 +87,17 @@ int num_tile_configs() { return kNumConfigs; }
 
 const TileConfig& tile_config(int id) { return kConfigs[id - 1]; }
 
+template <typename Real>
+bool tile_config_available(int id) {
+    if (id < 1 || id > kNumConfigs) return false;
+    const TileConfig& c = kConfigs[id - 1];
+    return Prec<Real>::tiled_shape_available(c.tp, c.tc, 64 * c.waves);
+}
+
 // Lanes a rod needs under (tp, tc): every particle and frame within tp trips,
 // every colour within tc trips.
-int group_lanes(const RodInfo& r, const std::vector<int>& colStart, int tp, int tc) {
+template <typename Real>
+int group_lanes(const RodInfoT<Real>& r, const std::vector<int>& colStart, int tp, int tc) {
     int g = std::max((r.nP + tp - 1) / tp, (r.nF + tp - 1) / tp);
     const int* cs = colStart.data() + r.colOff;
     for (int c = 0; c < r.nSCol; ++c) g = std::max(g, (cs[c + 1] - cs[c] + tc - 1) / tc);
@@ -98,8 +106,9 @@ int group_lanes(const RodInfo& r, const std::vector<int>& colStart, int tp, int 
     return std::max(g, 1);
 }
 
-int rod_tiled_lds(const RodInfo& r) {
-    return tiled_lds_bytes(r.nP, r.nF, r.nS, r.nPrim, r.self, r.table, r.selfCap);
+template <typename Real>
+int rod_tiled_lds(const RodInfoT<Real>& r) {
+    return Prec<Real>::tiled_lds_bytes(r.nP, r.nF, r.nS, r.nPrim, r.self, r.table, r.selfCap);
 }
 
 int lds_limits(LdsLimits& out) {
@@ -108,9 +117,10 @@ int lds_limits(LdsLimits& out) {
     // Workgroups may use the whole LDS of a compute unit; where the runtime
     // refuses the larger dynamic allocation, stay within 64 KiB.
     out.perCU = prop.sharedMemPerBlock;
-    int granted = 0;
-    if (allow_dynamic_lds(int(prop.sharedMemPerBlock), &granted) == hipSuccess) {
-        out.perWorkgroup = size_t(granted);
+    int granted32 = 0, granted64 = 0;
+    if (Prec<float>::allow_dynamic_lds(int(prop.sharedMemPerBlock), &granted32) == hipSuccess &&
+        Prec<double>::allow_dynamic_lds(int(prop.sharedMemPerBlock), &granted64) == hipSuccess) {
+        out.perWorkgroup = size_t(std::min(granted32, granted64));
     } else {
         (void)hipGetLastError();
         out.perCU = out.perWorkgroup = 65536 - 1024;
@@ -125,7 +135,8 @@ int lds_budget(const LdsLimits& l, int perCU) {
 
 // Best fit over open workgroups, rods taken widest first; a workgroup closes
 // when neither lanes nor LDS are left for the narrowest remaining rod.
-int pack_rods(const std::vector<RodInfo>& rods, const std::vector<int>& ids, const std::vector<int>& colStart,
+template <typename Real>
+int pack_rods(const std::vector<RodInfoT<Real>>& rods, const std::vector<int>& ids, const std::vector<int>& colStart,
               int configId, int ldsCap, Packing& out) {
     const TileConfig& cfg = tile_config(configId);
     const int laneCap = 64 * cfg.waves;
@@ -133,7 +144,7 @@ int pack_rods(const std::vector<RodInfo>& rods, const std::vector<int>& ids, con
     std::vector<int> lanes(rods.size()), lds(rods.size()), order(n);
     for (int i = 0; i < n; ++i) {
         const int r = ids[i];
-        const RodInfo& ri = rods[r];
+        const RodInfoT<Real>& ri = rods[r];
         if (ri.nSCol > kTiledColours || ri.nBCol > kTiledColours) return RP_ERR_CAPACITY;
         lanes[r] = group_lanes(ri, colStart, cfg.tp, cfg.tc);
         lds[r] = rod_tiled_lds(ri);
@@ -192,10 +203,13 @@ int pack_rods(const std::vector<RodInfo>& rods, const std::vector<int>& ids, con
     return RP_OK;
 }
 
-StepArgs step_args(const rp_step_params& p, int steps) {
-    StepArgs a;
-    a.h = p.dt / float(p.substeps);
-    a.invH2 = 1.0f / (a.h * a.h);
+// h = dt / substeps and the velocity decays exp(-damping h) in the working
+// precision, as the reference's step forms them (src/core/solver.cpp:218-251).
+template <typename Real>
+StepArgsT<Real> step_args(const typename Prec<Real>::StepParams& p, int steps) {
+    StepArgsT<Real> a;
+    a.h = p.dt / Real(p.substeps);
+    a.invH2 = Real(1) / (a.h * a.h);
     a.gx = p.gravity[0];
     a.gy = p.gravity[1];
     a.gz = p.gravity[2];
@@ -208,7 +222,10 @@ StepArgs step_args(const rp_step_params& p, int steps) {
     return a;
 }
 
-int build_records(const rp_batch_desc* desc, Records& out) {
+template <typename Real>
+int build_records(const typename Prec<Real>::BatchDesc* desc, RecordsT<Real>& out) {
+    using Vec4 = typename Prec<Real>::Vec4;
+    const auto make4 = &Prec<Real>::make4;
     if (!desc) return fail(RP_ERR_INVALID, "null argument");
     const int R = desc->num_rods;
     if (R <= 0) return fail(RP_ERR_INVALID, "num_rods must be positive");
@@ -241,7 +258,7 @@ int build_records(const rp_batch_desc* desc, Records& out) {
     if (desc->num_worlds < 0 || (desc->num_worlds > 0 && !desc->worlds))
         return fail(RP_ERR_INVALID, "bad world table");
     for (int w = 0; w < desc->num_worlds; ++w) {
-        const rp_world_desc& wd = desc->worlds[w];
+        const auto& wd = desc->worlds[w];
         if (wd.num_prims < 0 || wd.num_prims > kMaxPrims || (wd.num_prims > 0 && !wd.prims))
             return fail(RP_ERR_INVALID, "world " + std::to_string(w) + ": 0.." + std::to_string(kMaxPrims) +
                                             " primitives supported");
@@ -255,27 +272,27 @@ int build_records(const rp_batch_desc* desc, Records& out) {
     // ---- per-rod colouring, constraint reordering and rod info
     std::vector<int4>& sIdx = out.sIdx;
     std::vector<int4>& bIdx = out.bIdx;
-    std::vector<float4>& sPar = out.sPar;
-    std::vector<float4>& bPar = out.bPar;
-    std::vector<float4>& bRest4 = out.bRest4;
+    std::vector<Vec4>& sPar = out.sPar;
+    std::vector<Vec4>& bPar = out.bPar;
+    std::vector<Vec4>& bRest4 = out.bRest4;
     std::vector<int>& segEnd = out.segEnd;
     sIdx.assign(S, int4{});
     bIdx.assign(B, int4{});
-    sPar.assign(S, float4{});
-    bPar.assign(B, float4{});
-    bRest4.assign(B, float4{});
+    sPar.assign(S, Vec4{});
+    bPar.assign(B, Vec4{});
+    bRest4.assign(B, Vec4{});
     segEnd.assign(2 * size_t(S), 0);
     std::vector<int>& colStart = out.colStart;
     colStart.clear();
     out.rods.resize(R);
     out.colourFit.assign(R, 1);
-    std::vector<Prim>& prims = out.prims;
+    std::vector<PrimT<Real>>& prims = out.prims;
     std::vector<int> primOffOfWorld(std::max(desc->num_worlds, 1), 0);
     for (int w = 0; w < desc->num_worlds; ++w) {
         primOffOfWorld[w] = int(prims.size());
         for (int k = 0; k < desc->worlds[w].num_prims; ++k) {
-            const rp_primitive& s = desc->worlds[w].prims[k];
-            Prim p;
+            const auto& s = desc->worlds[w].prims[k];
+            PrimT<Real> p;
             p.type = s.type;
             for (int c = 0; c < 3; ++c) {
                 p.a[c] = s.a[c];
@@ -293,7 +310,7 @@ int build_records(const rp_batch_desc* desc, Records& out) {
     std::vector<std::array<int, 3>> res;
     int maxPlainLds = 0;
     for (int r = 0; r < R; ++r) {
-        RodInfo& ri = out.rods[r];
+        RodInfoT<Real>& ri = out.rods[r];
         std::memset(&ri, 0, sizeof(ri));
         ri.pStart = out.pStart[r];
         ri.nP = out.pStart[r + 1] - out.pStart[r];
@@ -325,7 +342,7 @@ int build_records(const rp_batch_desc* desc, Records& out) {
         for (int k = 0; k < ri.nS; ++k) {
             const int src = ri.sStart + order[k], dst = ri.sStart + k;
             sIdx[dst] = make_int4(desc->stretch_p0[src], desc->stretch_p1[src], desc->stretch_seg[src], order[k]);
-            sPar[dst] = make_float4(desc->stretch_rest[src], desc->stretch_compliance[3 * size_t(src)],
+            sPar[dst] = make4(desc->stretch_rest[src], desc->stretch_compliance[3 * size_t(src)],
                                     desc->stretch_compliance[3 * size_t(src) + 1],
                                     desc->stretch_compliance[3 * size_t(src) + 2]);
         }
@@ -344,12 +361,11 @@ int build_records(const rp_batch_desc* desc, Records& out) {
         for (int k = 0; k < ri.nB; ++k) {
             const int src = ri.bStart + order[k], dst = ri.bStart + k;
             bIdx[dst] = make_int4(desc->bend_a[src], desc->bend_b[src], order[k], 0);
-            bPar[dst] = make_float4(desc->bend_rest[src], desc->bend_compliance[3 * size_t(src)],
+            bPar[dst] = make4(desc->bend_rest[src], desc->bend_compliance[3 * size_t(src)],
                                     desc->bend_compliance[3 * size_t(src) + 1],
                                     desc->bend_compliance[3 * size_t(src) + 2]);
-            bRest4[dst] = make_float4(desc->bend_rest_darboux[3 * size_t(src)],
-                                      desc->bend_rest_darboux[3 * size_t(src) + 1],
-                                      desc->bend_rest_darboux[3 * size_t(src) + 2], 0.0f);
+            bRest4[dst] = make4(desc->bend_rest_darboux[3 * size_t(src)], desc->bend_rest_darboux[3 * size_t(src) + 1],
+                                desc->bend_rest_darboux[3 * size_t(src) + 2], Real(0));
         }
         colStart.insert(colStart.end(), start.begin(), start.end());
         if (ri.nSCol > kMaxColours || ri.nBCol > kMaxColours)
@@ -358,19 +374,21 @@ int build_records(const rp_batch_desc* desc, Records& out) {
         if (ri.nSCol > kTiledColours || ri.nBCol > kTiledColours) out.colourFit[r] = 0;
         // Contacts.
         if (w >= 0) {
-            const rp_world_desc& wd = desc->worlds[w];
+            const auto& wd = desc->worlds[w];
             ri.nPrim = wd.num_prims;
             ri.primOff = primOffOfWorld[w];
-            // contactMarginRadii * radius, in FP32 as the CPU path forms it.
+            // contactMarginRadii * radius, in the working precision as the CPU
+            // path forms it.
             ri.margin = wd.contact_margin_radii * ri.radius;
             ri.self = wd.self_collision ? 1 : 0;
             ri.selfFriction = wd.self_friction;
             ri.table = wd.self_collision ? wd.hash_table_size : 1;
             if (ri.self) {
-                // selfCollisionIndexGap (src/core/collision.cpp:67-74), in FP32.
-                float lMin = ri.nS > 0 ? desc->stretch_rest[ri.sStart] : 1.0f;
+                // selfCollisionIndexGap (src/core/collision.cpp:67-74), in the
+                // working precision.
+                Real lMin = ri.nS > 0 ? desc->stretch_rest[ri.sStart] : Real(1);
                 for (int k = 0; k < ri.nS; ++k) lMin = std::min(lMin, desc->stretch_rest[ri.sStart + k]);
-                const int geometric = int(std::floor(2.0f * ri.radius / lMin)) + 2;
+                const int geometric = int(std::floor(Real(2) * ri.radius / lMin)) + 2;
                 ri.selfGap = std::max(wd.self_skip, geometric);
                 ri.selfCap = wd.self_capacity > 0 ? wd.self_capacity : std::max(16, ri.nS / 2);
             }
@@ -378,7 +396,7 @@ int build_records(const rp_batch_desc* desc, Records& out) {
             ri.table = 1;
         }
         maxPlainLds =
-            std::max(maxPlainLds, plain_lds_bytes(ri.nP, ri.nF, ri.nS, ri.nB, ri.nPrim, ri.self, ri.table, ri.selfCap));
+            std::max(maxPlainLds, Prec<Real>::plain_lds_bytes(ri.nP, ri.nF, ri.nS, ri.nB, ri.nPrim, ri.self, ri.table, ri.selfCap));
     }
     for (int r = 0; r < R; ++r) out.anySelf = out.anySelf || out.rods[r].self != 0;
     out.maxPlainLds = maxPlainLds;
@@ -386,13 +404,25 @@ int build_records(const rp_batch_desc* desc, Records& out) {
     out.iI4.resize(F);
     out.I4.resize(F);
     for (int j = 0; j < F; ++j) {
-        out.iI4[j] = make_float4(desc->inv_inertia[3 * size_t(j)], desc->inv_inertia[3 * size_t(j) + 1],
-                                 desc->inv_inertia[3 * size_t(j) + 2], 0.0f);
-        out.I4[j] = make_float4(desc->inertia[3 * size_t(j)], desc->inertia[3 * size_t(j) + 1],
-                                desc->inertia[3 * size_t(j) + 2], 0.0f);
+        out.iI4[j] = make4(desc->inv_inertia[3 * size_t(j)], desc->inv_inertia[3 * size_t(j) + 1],
+                           desc->inv_inertia[3 * size_t(j) + 2], Real(0));
+        out.I4[j] = make4(desc->inertia[3 * size_t(j)], desc->inertia[3 * size_t(j) + 1],
+                          desc->inertia[3 * size_t(j) + 2], Real(0));
     }
     return RP_OK;
 }
+
+#define RP_HOST_INSTANTIATE(Real)                                                                              \
+    template bool tile_config_available<Real>(int);                                                            \
+    template int group_lanes<Real>(const RodInfoT<Real>&, const std::vector<int>&, int, int);                  \
+    template int rod_tiled_lds<Real>(const RodInfoT<Real>&);                                                   \
+    template int pack_rods<Real>(const std::vector<RodInfoT<Real>>&, const std::vector<int>&,                  \
+                                 const std::vector<int>&, int, int, Packing&);                                 \
+    template StepArgsT<Real> step_args<Real>(const Prec<Real>::StepParams&, int);                              \
+    template int build_records<Real>(const Prec<Real>::BatchDesc*, RecordsT<Real>&);
+RP_HOST_INSTANTIATE(float)
+RP_HOST_INSTANTIATE(double)
+#undef RP_HOST_INSTANTIATE
 
 }  // namespace host
 }  // namespace rp

@@ -24,6 +24,7 @@ namespace rp {
     hipError_t launch_tiled(int tp, int tc, bool self, const DeviceArraysT<R>& d, const StepArgsT<R>& sa,      \
                             const Tile* tiles, const TileSlot* slots, int numTiles, int threads, size_t lds,   \
                             hipStream_t s);                                                                    \
+    bool tiled_shape_available(int tp, int tc, int threads);                                                   \
     hipError_t launch_fixed_frames(R* q, const R* src, const R* invInertia4, int f0, int count, hipStream_t s);
 namespace f32 {
 RP_KERNEL_API(float)
@@ -33,17 +34,46 @@ RP_KERNEL_API(double)
 }
 #undef RP_KERNEL_API
 
-// The host side below works on FP32 records.
-using f32::DeviceArrays;
-using f32::Prim;
-using f32::RodInfo;
-using f32::StepArgs;
-using f32::allow_dynamic_lds;
-using f32::launch_fixed_frames;
-using f32::launch_plain;
-using f32::launch_tiled;
-using f32::plain_lds_bytes;
-using f32::tiled_lds_bytes;
+// One precision's kernel entry points and ABI types, for host code written
+// once over the scalar.
+template <typename Real>
+struct Prec;
+
+template <>
+struct Prec<float> {
+    static constexpr int bits = 32;
+    using Vec4 = float4;
+    using BatchDesc = rp_batch_desc;
+    using WorldDesc = rp_world_desc;
+    using Primitive = rp_primitive;
+    using StepParams = rp_step_params;
+    static constexpr auto plain_lds_bytes = &f32::plain_lds_bytes;
+    static constexpr auto tiled_lds_bytes = &f32::tiled_lds_bytes;
+    static constexpr auto allow_dynamic_lds = &f32::allow_dynamic_lds;
+    static constexpr auto launch_plain = &f32::launch_plain;
+    static constexpr auto launch_tiled = &f32::launch_tiled;
+    static constexpr auto tiled_shape_available = &f32::tiled_shape_available;
+    static constexpr auto launch_fixed_frames = &f32::launch_fixed_frames;
+    static Vec4 make4(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
+};
+
+template <>
+struct Prec<double> {
+    static constexpr int bits = 64;
+    using Vec4 = double4;
+    using BatchDesc = rp_batch_desc64;
+    using WorldDesc = rp_world_desc64;
+    using Primitive = rp_primitive64;
+    using StepParams = rp_step_params64;
+    static constexpr auto plain_lds_bytes = &f64::plain_lds_bytes;
+    static constexpr auto tiled_lds_bytes = &f64::tiled_lds_bytes;
+    static constexpr auto allow_dynamic_lds = &f64::allow_dynamic_lds;
+    static constexpr auto launch_plain = &f64::launch_plain;
+    static constexpr auto launch_tiled = &f64::launch_tiled;
+    static constexpr auto tiled_shape_available = &f64::tiled_shape_available;
+    static constexpr auto launch_fixed_frames = &f64::launch_fixed_frames;
+    static Vec4 make4(double a, double b, double c, double d) { return make_double4(a, b, c, d); }
+};
 
 namespace host {
 
@@ -68,26 +98,37 @@ struct TileConfig {
 int num_tile_configs();
 const TileConfig& tile_config(int id);  // id in 1 .. num_tile_configs()
 
+// Whether tile shape `id` runs in precision Real (FP64 has no two-trip and
+// no 512-lane shapes).
+template <typename Real>
+bool tile_config_available(int id);
+
 // A batch description validated, coloured and laid out as the kernels read
-// it: rod records with offsets from 0, constraints in colour order, float4
+// it: rod records with offsets from 0, constraints in colour order, 4-vector
 // inertia, primitives of every world one after another.
-struct Records {
+template <typename Real>
+struct RecordsT {
+    using Vec4 = typename Prec<Real>::Vec4;
     int R = 0, P = 0, F = 0, S = 0, B = 0;
     std::vector<int> pStart, fStart, sStart, bStart;  // CSR, R + 1 entries
-    std::vector<RodInfo> rods;
+    std::vector<RodInfoT<Real>> rods;
     std::vector<unsigned char> colourFit;  // at most two colours per family
     std::vector<int4> sIdx, bIdx;
-    std::vector<float4> sPar, bPar, bRest4, iI4, I4;
-    std::vector<float> invMass;
+    std::vector<Vec4> sPar, bPar, bRest4, iI4, I4;
+    std::vector<Real> invMass;
     std::vector<int> segEnd, colStart;
-    std::vector<Prim> prims;
+    std::vector<PrimT<Real>> prims;
     int maxPlainLds = 0;
     bool anySelf = false;
 };
-int build_records(const rp_batch_desc* desc, Records& out);
+using Records = RecordsT<float>;
+
+template <typename Real>
+int build_records(const typename Prec<Real>::BatchDesc* desc, RecordsT<Real>& out);
 
 // LDS a workgroup may take (dynamic part) and the LDS of a compute unit;
-// raises the stepping kernels' dynamic-LDS limit on first use.
+// raises the stepping kernels' dynamic-LDS limit (both precisions) on first
+// use.
 struct LdsLimits {
     size_t perWorkgroup = 65536;
     size_t perCU = 65536;
@@ -96,8 +137,10 @@ int lds_limits(LdsLimits& out);
 // Dynamic LDS budget of a workgroup when `perCU` of them share a CU.
 int lds_budget(const LdsLimits& l, int perCU);
 
-int group_lanes(const RodInfo& r, const std::vector<int>& colStart, int tp, int tc);
-int rod_tiled_lds(const RodInfo& r);
+template <typename Real>
+int group_lanes(const RodInfoT<Real>& r, const std::vector<int>& colStart, int tp, int tc);
+template <typename Real>
+int rod_tiled_lds(const RodInfoT<Real>& r);
 
 // Rods `ids` of `rods` packed into workgroups of shape `configId`, each within
 // `ldsCap` bytes of LDS (best fit, widest rods first).
@@ -108,10 +151,12 @@ struct Packing {
     int maxLds = 0;
     int lanesBusy = 0;
 };
-int pack_rods(const std::vector<RodInfo>& rods, const std::vector<int>& ids, const std::vector<int>& colStart,
+template <typename Real>
+int pack_rods(const std::vector<RodInfoT<Real>>& rods, const std::vector<int>& ids, const std::vector<int>& colStart,
               int configId, int ldsCap, Packing& out);
 
-StepArgs step_args(const rp_step_params& p, int steps);
+template <typename Real>
+StepArgsT<Real> step_args(const typename Prec<Real>::StepParams& p, int steps);
 
 }  // namespace host
 }  // namespace rp

@@ -672,20 +672,32 @@ hipError_t allow_dynamic_lds(int bytes, int* granted) {
     if (e == hipSuccess) e = allow_tiled_lds<1, 1>(bytes, granted);
     if (e == hipSuccess) e = allow_tiled_lds<2, 1>(bytes, granted);
     if (e == hipSuccess) e = allow_tiled_lds<3, 1>(bytes, granted);
+#if !RP_FP64
     if (e == hipSuccess) e = allow_tiled_lds<3, 2>(bytes, granted);
     if (e == hipSuccess) e = allow_tiled_lds<5, 2>(bytes, granted);
+#endif
     return e;
 }
 
-// Trip shapes compiled into the library; (TP, TC) pairs.
+// Trip shapes compiled into the library; (TP, TC) pairs. In FP64 the
+// two-trip shapes are left out: their state does not fit the register file.
 hipError_t launch_tiled(int tp, int tc, bool self, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
                         const TileSlot* slots, int numTiles, int threads, size_t lds, hipStream_t s) {
+    if (threads < 64 || threads % 64 != 0 || threads > tiled_max_lanes(tp)) return hipErrorInvalidValue;
     if (tp == 1 && tc == 1) return launch_tiled_t<1, 1>(self, d, sa, tiles, slots, numTiles, threads, lds, s);
     if (tp == 2 && tc == 1) return launch_tiled_t<2, 1>(self, d, sa, tiles, slots, numTiles, threads, lds, s);
     if (tp == 3 && tc == 1) return launch_tiled_t<3, 1>(self, d, sa, tiles, slots, numTiles, threads, lds, s);
+#if !RP_FP64
     if (tp == 3 && tc == 2) return launch_tiled_t<3, 2>(self, d, sa, tiles, slots, numTiles, threads, lds, s);
     if (tp == 5 && tc == 2) return launch_tiled_t<5, 2>(self, d, sa, tiles, slots, numTiles, threads, lds, s);
+#endif
     return hipErrorInvalidValue;
+}
+
+bool tiled_shape_available(int tp, int tc, int threads) {
+    if (threads < 64 || threads % 64 != 0 || threads > tiled_max_lanes(tp)) return false;
+    if (tc == 1) return tp >= 1 && tp <= 3;
+    return !RP_FP64 && tc == 2 && (tp == 3 || tp == 5);
 }
 
 hipError_t launch_fixed_frames(real* q, const real* src, const real* invInertia4, int f0, int count,

@@ -151,7 +151,8 @@ __device__ inline void path_point(const PathRec& p, double t, double& x, double&
 }
 
 // Velocity of the driven particle during the rod's step `step`: constant over
-// each control tick, (P(t_{k+1}) - P(t_k)) / period, rounded to FP32.
+// each control tick, (P(t_{k+1}) - P(t_k)) / period, rounded to the working
+// precision.
 __device__ inline v3 path_velocity(const PathRec& p, int step) {
     const int tick = step / p.stepsPerControl;
     double x1, y1, x0, y0;

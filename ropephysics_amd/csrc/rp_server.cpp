@@ -24,6 +24,7 @@ namespace {
 
 using namespace rp;
 using namespace rp::host;
+using namespace rp::f32;  // the server steps FP32 requests
 
 // First-fit allocator of index ranges in [0, capacity), coalescing on free.
 class RangePool {
@@ -405,7 +406,7 @@ int rp_server_step(void* server, int* num_finished) {
         RP_HIP(hipStreamSynchronize(s.stream));
         s.dirty = false;
     }
-    const StepArgs a = step_args(s.params, s.stepsPerLaunch);
+    const StepArgs a = step_args<float>(s.params, s.stepsPerLaunch);
     if (!self.empty()) {
         // Uploads and tiles are on the main stream: the second stream starts
         // after them, and the main stream continues after its launch.
