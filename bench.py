@@ -11,8 +11,12 @@ during the warm-up); the PLAIN one-rod-per-workgroup launch (the reference's
 launch shape) is timed over the same steps of the same workload beside it; the
 reference CPU path runs a bounded sample of it.
 
-Secondary: the uniform batch of 4096 rods x 64 segments over a floor plane with
-per-rod stiffness and friction, 1000 steps, TILED and PLAIN; and the serving
+Secondary: the routing workload again as an FP64 batch (the reference CPU
+solver's precision on the GPU), its routed verdicts checked against the
+reference CPU path in colour order; the uniform batch of 4096 rods x 64
+segments over a floor plane with per-rod stiffness and friction, 1000 steps,
+TILED and PLAIN; the hold/slide sweep of 3600 draped cables (8 s) in FP32 and
+FP64 with its verdicts spot-checked against the reference CPU path; and the
 stream (requests of 1-256 rods of 32-256 segments, 0.5-8 s horizons, random
 contact shapes, self-collision in half) through the continuous-batching
 server: its capacity with every request arriving at once, then p50/p99 latency
@@ -104,8 +108,8 @@ def roofline(flops_per_unit: float, kernel_rate: float, kernel_ms: float, units_
 
 # ------------------------------------------------------------------ ragged routing (headline)
 
-def make_ragged_batch(w: scenes.Workload, strategy: str, tile_config: int = 0) -> rp.Batch:
-    b = rp.Batch(w.rods, strategy=strategy, world=w.world, tile_config=tile_config)
+def make_ragged_batch(w: scenes.Workload, strategy: str, tile_config: int = 0, precision: int = 32) -> rp.Batch:
+    b = rp.Batch(w.rods, strategy=strategy, world=w.world, tile_config=tile_config, precision=precision)
     b.set_material_scales(w.youngs_scale, w.friction_scale)
     b.set_paths(0, w.info["paths"])
     return b
@@ -155,11 +159,39 @@ def routing_leg(w: scenes.Workload, steps: int, plain_steps: int, warmup: int, t
     return out
 
 
-def cpu_baseline_routing(w: scenes.Workload, stride: int, steps: int, threads: int) -> dict:
+def routing_fp64_leg(w: scenes.Workload, steps: int, warmup: int) -> dict:
+    """The routing workload in an FP64 batch (the reference CPU solver's own
+    precision on the GPU): throughput and routed verdicts."""
+    b = make_ragged_batch(w, "tiled", precision=64)
+    reset_ragged(b, w)
+    b.step(w.params, warmup)
+    b.synchronize()
+    reset_ragged(b, w)
+    b.set_timing(True)
+    secs, c = time_steps(b, w.params, steps)
+    work = float(w.segments_total) * w.params.substeps * steps
+    launches = max(1, c["last_step_launches"])
+    ent = {"value": work / secs, "unit": "segment-substeps/s", "dtype": "f64", "steps": steps, "seconds": secs,
+           "kernel_ms_avg": c["last_step_gpu_ms"] / launches, "launches": launches,
+           "tile_config": c.get("tile_config_name"), "workgroups": c["workgroups"],
+           "self_contact_overflow": c["self_contact_overflow"]}
+    verdicts = None
+    if steps == w.steps:
+        task = w.info["task"]
+        verdicts = [task.evaluate(b.download(i)["x"])["routed"] for i in range(w.num_rods)]
+        ent["routed"] = int(sum(verdicts))
+    log(f"routing FP64 {ent['tile_config']}: {secs:.3f} s for {steps} steps, {ent['value'] / 1e9:.3f} G "
+        f"seg-substeps/s")
+    b.destroy()
+    return ent, verdicts
+
+
+def cpu_baseline_routing(w: scenes.Workload, stride: int, steps: int, threads: int, coloured: bool = False) -> dict:
     """The reference CPU path (its own sources, built into oracle/_ref) on every
     `stride`-th cable of the routing workload over its first `steps` steps, FP64,
     the gripper schedule applied before each step, on `threads` host threads
-    with a static partition over rods."""
+    with a static partition over rods; in the reference's default sequential
+    sweep order, or in colour order (the order the GPU sweeps in)."""
     import oracle  # the CPU checker/baseline; never part of the product path
 
     kind = "ref64" if oracle.available("ref64") else "oracle"
@@ -182,7 +214,7 @@ def cpu_baseline_routing(w: scenes.Workload, stride: int, steps: int, threads: i
                                    self_skip=w.world.self_skip, hash_table_size=w.world.hash_table_size))
     kv = np.array([[paths[r].velocity(s) for s in range(steps)] for r in sel])
     params = oracle.Params(dt=p.dt, substeps=p.substeps, iterations=p.iterations, gravity=p.gravity,
-                           coloured=False)
+                           coloured=coloured)
     t0 = time.perf_counter()
     oracle.step_many_driven(rods, params, worlds, steps, threads, [paths[r].particle for r in sel], kv)
     dt = time.perf_counter() - t0
@@ -195,7 +227,8 @@ def cpu_baseline_routing(w: scenes.Workload, stride: int, steps: int, threads: i
     return {"value": work / dt, "unit": "segment-substeps/s", "cores": threads, "_verdicts": verdicts,
             "kind": "reference" if kind == "ref64" else "port",
             "sample": f"{len(sel)} of {w.num_rods} cables (every {stride}th, {segs} segments), first {steps} of "
-                      f"{w.steps} steps, gripper schedule applied per step, FP64, sequential sweep order, "
+                      f"{w.steps} steps, gripper schedule applied per step, FP64, "
+                      f"{'colour' if coloured else 'sequential'} sweep order, "
                       f"{threads} threads static partition over rods; {dt:.1f} s",
             "seconds": dt}
 
@@ -282,6 +315,93 @@ def uniform_leg(steps: int, warmup: int, no_plain: bool) -> dict:
     return out
 
 
+# ------------------------------------------------------------------ hold/slide sweep (verdicts)
+
+def drape_cpu_verdicts(w: scenes.Workload, sel, steps: int, threads: int) -> dict:
+    """Hold/slide verdicts of the reference CPU path (its own sources, FP64,
+    colour order) for rods `sel` of the drape sweep, each with its stiffness
+    applied to the compliances and its friction to the bar and floor."""
+    import oracle  # the CPU checker; never part of the product path
+
+    kind = "ref64" if oracle.available("ref64") else "oracle"
+    task = w.info["task"]
+    rods, worlds = [], []
+    for r in sel:
+        rod = w.rods[r]
+        o = oracle.Rod.straight(2, 1.0, rod.material.as_tuple(), precision=64, kind=kind)
+        a = rod.arrays()
+        a["sComp"] = a["sComp"] / float(w.youngs_scale[r])
+        a["bComp"] = a["bComp"] / float(w.youngs_scale[r])
+        o.set(a)
+        rods.append(o)
+        prims = [oracle.make_prim(pr.type, pr.a, pr.b, pr.normal, pr.half_extents, pr.rotation, pr.radius,
+                                  pr.friction * float(w.friction_scale[r])) for pr in w.world.primitives]
+        worlds.append(oracle.World(prims=prims, margin_radii=w.world.contact_margin_radii))
+    p = w.params
+    params = oracle.Params(dt=p.dt, substeps=p.substeps, iterations=p.iterations, gravity=p.gravity, coloured=True)
+    t0 = time.perf_counter()
+    oracle.step_many(rods, params, worlds, steps, threads)
+    dt = time.perf_counter() - t0
+    return {"kind": "reference" if kind == "ref64" else "port", "seconds": dt, "threads": threads,
+            "held": {int(r): task.held_from_final(o.get()["x"]) for r, o in zip(sel, rods)}}
+
+
+def drape_leg(steps: int | None, warmup: int, cpu_rods: int, threads: int) -> dict:
+    """The hold/slide sweep: 3600 draped cables stepped as one FP32 batch and
+    as one FP64 batch; held/slid from the final height (the cable's highest
+    particle within 5 cm of the bar's top). FP32 cannot start a resting cable
+    sliding here (gravity's per-substep motion is 0.4 of the FP32 spacing at
+    the bar's height), FP64 can; both verdict sets are compared with the
+    reference CPU path's on a spot sample: half spread over the grid, half the
+    cables nearest the capstan boundary mu = ln(ratio) / pi."""
+    w = scenes.drape_sweep(steps=steps)
+    steps = w.steps
+    task = w.info["task"]
+    meta = w.info["meta"]
+    out = {"config": {"workload": w.name, **{k: v for k, v in w.info.items() if k not in ("task", "meta")}},
+           "float_motion_margin": task.float_motion_margin()}
+    held = {}
+    for prec in (32, 64):
+        b = rp.Batch(w.rods, strategy="tiled", world=w.world, precision=prec)
+        b.set_material_scales(w.youngs_scale, w.friction_scale)
+        b.step(w.params, warmup)
+        b.synchronize()
+        b.upload_all(w.rods)
+        b.set_timing(True)
+        secs, c = time_steps(b, w.params, steps)
+        work = float(w.segments_total) * w.params.substeps * steps
+        launches = max(1, c["last_step_launches"])
+        x = b.download_positions()
+        ps = b.particle_start
+        held[prec] = [task.held_from_final(x[ps[i]:ps[i + 1]]) for i in range(w.num_rods)] \
+            if steps == task.steps() else None
+        out[f"fp{prec}"] = {"value": work / secs, "unit": "segment-substeps/s", "steps": steps, "seconds": secs,
+                            "kernel_ms_avg": c["last_step_gpu_ms"] / launches, "launches": launches,
+                            "tile_config": c.get("tile_config_name"),
+                            "held": None if held[prec] is None else int(sum(held[prec]))}
+        log(f"drape FP{prec} {c.get('tile_config_name')}: {secs:.3f} s for {steps} steps, "
+            f"{work / secs / 1e9:.3f} G seg-substeps/s, held {out[f'fp{prec}']['held']} of {w.num_rods}")
+        b.destroy()
+    if held[32] is None:
+        return out
+    out["fp32_vs_fp64_agree"] = int(sum(1 for a, c in zip(held[32], held[64]) if a == c))
+    if cpu_rods > 0:
+        half = cpu_rods // 2
+        spread = list(range(0, w.num_rods, max(1, w.num_rods // max(1, half))))[:half]
+        gap = np.array([abs(mu - task.capstan_friction(r)) for r, mu, _ in meta])
+        near = [int(i) for i in np.argsort(gap, kind="stable") if int(i) not in spread][:cpu_rods - len(spread)]
+        sel = spread + near
+        cpu = drape_cpu_verdicts(w, sel, steps, threads)
+        out["cpu_spot_check"] = {
+            "rods": len(sel), "kind": cpu["kind"], "seconds": cpu["seconds"], "threads": threads,
+            "order": "colour", "held_cpu": int(sum(cpu["held"].values())),
+            "agree_fp64": int(sum(1 for r, v in cpu["held"].items() if held[64][r] == v)),
+            "agree_fp32": int(sum(1 for r, v in cpu["held"].items() if held[32][r] == v))}
+        log(f"drape verdicts vs reference CPU FP64 on {len(sel)} cables: FP64 GPU "
+            f"{out['cpu_spot_check']['agree_fp64']} agree, FP32 GPU {out['cpu_spot_check']['agree_fp32']} agree")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,6 +416,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6000, help="routing steps of the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-routing", type=int, default=1, help="also run the routing workload in FP64 (0: skip)")
+    ap.add_argument("--drape-steps", type=int, default=None, help="steps of the hold/slide sweep (default: the "
+                                                                   "whole 8 s = 16000; 0: skip)")
+    ap.add_argument("--drape-cpu-rods", type=int, default=32, help="cables of the CPU verdict spot check")
     ap.add_argument("--serve-requests", type=int, default=40, help="requests of the serving leg (0: skip)")
     ap.add_argument("--serve-load", type=float, default=0.8, help="Poisson rate as a fraction of capacity")
     ap.add_argument("--serve-budget-gib", type=float, default=4.0, help="server HBM budget")
@@ -336,13 +460,17 @@ def main():
     roof = roofline(FLOPS_PER_SEGMENT_SUBSTEP_ROUTING, kernel_rate, t["kernel_ms_avg"], work / t["launches"],
                     "routing" if not args.tile_config else None)
 
+    r64, v64 = routing_fp64_leg(w, K, W) if args.fp64_routing else (None, None)
     uniform = uniform_leg(args.uniform_steps, W, args.no_plain) if args.uniform_steps > 0 else None
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    drape = None
+    if args.drape_steps != 0:
+        drape = drape_leg(args.drape_steps, W, 0 if args.no_cpu else args.drape_cpu_rods, threads)
     serving_out = serving_leg(args.serve_requests, args.serve_load, args.serve_budget_gib) \
         if args.serve_requests > 0 else None
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline_routing(w, args.cpu_stride, min(args.cpu_steps, w.steps), threads)
         log(f"CPU baseline: {cpu['value'] / 1e6:.2f} M seg-substeps/s ({cpu['sample']})")
         cv, gv = cpu.pop("_verdicts"), leg.get("_verdicts")
@@ -352,6 +480,16 @@ def main():
                                               "routed_gpu": sum(1 for r in cv if gv[r]),
                                               "routed_cpu": sum(1 for v in cv.values() if v)}
             log(f"routed verdicts, GPU FP32 vs reference CPU FP64 on {len(cv)} cables: {agree} agree")
+        if v64 is not None and K == w.steps:
+            col = cpu_baseline_routing(w, args.cpu_stride, w.steps, threads, coloured=True)
+            cc = col.pop("_verdicts")
+            r64["verdicts_vs_reference_cpu_colour_order"] = {
+                "sampled_cables": len(cc), "agree_fp64": sum(1 for r, v in cc.items() if v64[r] == v),
+                "agree_fp32": sum(1 for r, v in cc.items() if gv is not None and gv[r] == v),
+                "routed_cpu": sum(1 for v in cc.values() if v), "cpu_seconds": col["seconds"]}
+            log(f"routed verdicts vs reference CPU FP64 in colour order on {len(cc)} cables: FP64 GPU "
+                f"{r64['verdicts_vs_reference_cpu_colour_order']['agree_fp64']} agree, FP32 GPU "
+                f"{r64['verdicts_vs_reference_cpu_colour_order']['agree_fp32']} agree")
 
     plain = leg.get("plain")
     leg.pop("_verdicts", None)
@@ -367,7 +505,9 @@ def main():
         "speedup_vs_plain": (t["value"] / plain["value"]) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "routing_fp64": r64,
         "uniform": uniform,
+        "drape": drape,
         "serving": serving_out,
         "p50_latency_s": serving_out["poisson"]["p50_latency_s"] if serving_out else None,
         "p99_latency_s": serving_out["poisson"]["p99_latency_s"] if serving_out else None,
