@@ -352,12 +352,12 @@ def drape_leg(steps: int | None, warmup: int, cpu_rods: int, threads: int) -> di
     particle within 5 cm of the bar's top). FP32 cannot start a resting cable
     sliding here (gravity's per-substep motion is 0.4 of the FP32 spacing at
     the bar's height), FP64 can; both verdict sets are compared with the
-    reference CPU path's on a spot sample: half spread over the grid, half the
-    cables nearest the capstan boundary mu = ln(ratio) / pi."""
+    reference CPU path's on a spot sample: a quarter the cables FP64 holds, a
+    quarter those FP32 holds and FP64 lets slide, the rest spread over the
+    grid (each group drawn evenly from its list)."""
     w = scenes.drape_sweep(steps=steps)
     steps = w.steps
     task = w.info["task"]
-    meta = w.info["meta"]
     out = {"config": {"workload": w.name, **{k: v for k, v in w.info.items() if k not in ("task", "meta")}},
            "float_motion_margin": task.float_motion_margin()}
     held = {}
@@ -386,11 +386,15 @@ def drape_leg(steps: int | None, warmup: int, cpu_rods: int, threads: int) -> di
         return out
     out["fp32_vs_fp64_agree"] = int(sum(1 for a, c in zip(held[32], held[64]) if a == c))
     if cpu_rods > 0:
-        half = cpu_rods // 2
-        spread = list(range(0, w.num_rods, max(1, w.num_rods // max(1, half))))[:half]
-        gap = np.array([abs(mu - task.capstan_friction(r)) for r, mu, _ in meta])
-        near = [int(i) for i in np.argsort(gap, kind="stable") if int(i) not in spread][:cpu_rods - len(spread)]
-        sel = spread + near
+        def even(ids, k):
+            return [ids[int(j)] for j in np.linspace(0, len(ids) - 1, min(k, len(ids)))] if ids and k > 0 else []
+
+        held64 = [i for i in range(w.num_rods) if held[64][i]]
+        only32 = [i for i in range(w.num_rods) if held[32][i] and not held[64][i]]
+        sel = even(held64, cpu_rods // 4)
+        sel += [i for i in even(only32, cpu_rods // 4) if i not in sel]
+        rest = [i for i in range(w.num_rods) if i not in sel]
+        sel += even(rest, cpu_rods - len(sel))
         cpu = drape_cpu_verdicts(w, sel, steps, threads)
         out["cpu_spot_check"] = {
             "rods": len(sel), "kind": cpu["kind"], "seconds": cpu["seconds"], "threads": threads,

@@ -78,8 +78,10 @@ typedef struct rp_world_desc {
     float self_friction;
     int self_skip;         /* selfSkip; the gap used is max(self_skip, floor(2r / l_min) + 2) */
     int hash_table_size;   /* spatial-hash buckets (reference default 4096) */
-    int self_capacity;     /* self contacts stored per rod, 0 for max(16, segments / 2);
-                              more are counted as overflow */
+    int self_capacity;     /* self contacts stored per rod, 0 for the rule below; more are
+                              counted as overflow (and dropped, where the CPU path keeps them) */
+    float self_capacity_per_segment; /* with self_capacity 0: max(16, ceil(this * segments)) slots
+                                        when positive, else the reference's max(16, segments / 2) */
 } rp_world_desc;
 
 /* A ragged batch: rods may differ in every size. Element arrays are indexed
@@ -293,6 +295,7 @@ typedef struct rp_world_desc64 {
     int self_skip;
     int hash_table_size;
     int self_capacity;
+    double self_capacity_per_segment;
 } rp_world_desc64;
 
 typedef struct rp_batch_desc64 {

@@ -311,6 +311,7 @@ class Batch {
         wd.self_skip = w.selfSkip;
         wd.hash_table_size = w.hashTableSize;
         wd.self_capacity = 0;
+        wd.self_capacity_per_segment = 0.0f;
     }
     static void check(int rc) {
         if (rc != RP_OK) throw std::runtime_error(std::string("ropephysics: ") + rp_last_error());

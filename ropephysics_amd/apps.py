@@ -104,6 +104,11 @@ class RoutingTask:
     cable_density: float = 1100.0
     self_friction: float = 0.3
     hash_table_size: int = 256
+    # Self-contact pool per cable: 1.5 slots per segment. A cable lying in
+    # coils holds up to about 1.13 contacts per segment (measured over the
+    # routing workload); the reference's max(16, n / 2) drops contacts there,
+    # where the CPU path keeps every one.
+    self_capacity_per_segment: float = 1.5
     slack: float = 0.015
     leg_seconds: tuple = (1.2, 0.6, 0.5, 0.5, 0.5, 0.6, 0.6, 0.5)
     hold_seconds: float = 1.0
@@ -125,7 +130,8 @@ class RoutingTask:
     def world(self) -> CollisionWorld:
         mu = self.board_friction
         w = CollisionWorld(self_collision=True, self_friction=self.self_friction,
-                           hash_table_size=self.hash_table_size)
+                           hash_table_size=self.hash_table_size,
+                           self_capacity_per_segment=self.self_capacity_per_segment)
         w.primitives.append(Primitive.plane((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), mu))
         for peg in (self.peg_a, self.peg_b):
             w.primitives.append(Primitive.capsule(peg, (peg[0], peg[1], peg[2] + self.peg_height), self.peg_radius, mu))

@@ -267,6 +267,8 @@ int build_records(const typename Prec<Real>::BatchDesc* desc, RecordsT<Real>& ou
                 return fail(RP_ERR_INVALID, "world " + std::to_string(w) + ": unknown primitive type");
         if (wd.self_collision && wd.hash_table_size < 1)
             return fail(RP_ERR_INVALID, "world " + std::to_string(w) + ": hash_table_size must be positive");
+        if (wd.self_capacity < 0 || !(wd.self_capacity_per_segment >= 0) || wd.self_capacity_per_segment > 64)
+            return fail(RP_ERR_INVALID, "world " + std::to_string(w) + ": self capacity out of range");
     }
 
     // ---- per-rod colouring, constraint reordering and rod info
@@ -390,7 +392,12 @@ int build_records(const typename Prec<Real>::BatchDesc* desc, RecordsT<Real>& ou
                 for (int k = 0; k < ri.nS; ++k) lMin = std::min(lMin, desc->stretch_rest[ri.sStart + k]);
                 const int geometric = int(std::floor(Real(2) * ri.radius / lMin)) + 2;
                 ri.selfGap = std::max(wd.self_skip, geometric);
-                ri.selfCap = wd.self_capacity > 0 ? wd.self_capacity : std::max(16, ri.nS / 2);
+                if (wd.self_capacity > 0)
+                    ri.selfCap = wd.self_capacity;
+                else if (wd.self_capacity_per_segment > 0)
+                    ri.selfCap = std::max(16, int(std::ceil(double(wd.self_capacity_per_segment) * ri.nS)));
+                else
+                    ri.selfCap = std::max(16, ri.nS / 2);  // src/gpu/gpu_batch.cpp:169
             }
         } else {
             ri.table = 1;

@@ -426,9 +426,12 @@ class CollisionWorld:
     self_skip: int = 2
     contact_margin_radii: float = 0.25
     hash_table_size: int = 4096
-    # Self contacts stored per rod by the batch (0: max(16, segments / 2), the
-    # reference's pool); contacts beyond it are counted as overflow.
+    # Self contacts stored per rod by the batch: self_capacity when positive,
+    # else max(16, ceil(self_capacity_per_segment * segments)) when that is
+    # positive, else max(16, segments / 2), the reference's pool. Contacts
+    # beyond it are counted as overflow.
     self_capacity: int = 0
+    self_capacity_per_segment: float = 0.0
 
 
 @dataclass

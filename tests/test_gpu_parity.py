@@ -202,6 +202,32 @@ def test_self_collision_overflow_is_counted():
         assert tb.self_contact_overflow() > 0
 
 
+def test_self_capacity_per_segment():
+    """A pool of ceil(k * segments) slots (at least 16) is the pool of that
+    many fixed slots: same bits, same overflow count; out-of-range values are
+    refused."""
+    from ropephysics_amd import _abi
+
+    rope = helix_rope()  # 60 segments
+    base = dict(primitives=SELF_WORLD.primitives, self_collision=True, self_friction=0.3, hash_table_size=256)
+    for fixed, per_seg in ((510, 8.5), (16, 0.2)):
+        runs = []
+        for world in (rp.CollisionWorld(**base, self_capacity=fixed),
+                      rp.CollisionWorld(**base, self_capacity_per_segment=per_seg)):
+            b = rp.Batch([rope], strategy="tiled", world=world)
+            b.step(SELF_PARAMS, 20)
+            runs.append((b.get_state(), b.counters()))
+        (sa, ca), (sb, cb) = runs
+        for k in sa:
+            assert np.array_equal(bits(sa[k]), bits(sb[k])), (fixed, k)
+        assert ca["self_contact_overflow"] == cb["self_contact_overflow"]
+        assert ca["lds_bytes_per_workgroup"] == cb["lds_bytes_per_workgroup"]
+        assert (ca["self_contact_overflow"] > 0) == (fixed == 16)
+    with pytest.raises(_abi.RopePhysicsError) as e:
+        rp.Batch([rope], world=rp.CollisionWorld(**base, self_capacity_per_segment=-1.0))
+    assert e.value.code == _abi.RP_ERR_INVALID
+
+
 def pinned_coil(turns=5, per_turn=16, pitch=0.008, radius=0.025):
     """A helix whose pitch (8 mm) is below the rope's diameter (10 mm), both
     ends pinned: the turns keep pressing on each other."""
