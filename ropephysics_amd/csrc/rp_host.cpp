@@ -392,6 +392,14 @@ int build_records(const typename Prec<Real>::BatchDesc* desc, RecordsT<Real>& ou
                 for (int k = 0; k < ri.nS; ++k) lMin = std::min(lMin, desc->stretch_rest[ri.sStart + k]);
                 const int geometric = int(std::floor(Real(2) * ri.radius / lMin)) + 2;
                 ri.selfGap = std::max(wd.self_skip, geometric);
+                // Whether consecutive segments (index order) share a particle.
+                ri.selfChain = 1;
+                for (int k = 0; k + 1 < ri.nS; ++k) {
+                    const int g = ri.sStart + k;
+                    const int a = desc->stretch_p0[g], b = desc->stretch_p1[g];
+                    const int c = desc->stretch_p0[g + 1], e = desc->stretch_p1[g + 1];
+                    if (a != c && a != e && b != c && b != e) ri.selfChain = 0;
+                }
                 if (wd.self_capacity > 0)
                     ri.selfCap = wd.self_capacity;
                 else if (wd.self_capacity_per_segment > 0)

@@ -486,7 +486,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
             if (anySelf) {
                 if (regen) {
                     self_regen_group<TP>(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
-                                     ri.selfFriction * fscale, ri.self != 0 && live, l, G);
+                                         ri.selfFriction * fscale, ri.selfChain != 0, ri.self != 0 && live, l, G);
                 } else if (ri.self && l == 0 && live) {
                     self_reset(sl);
                 }

@@ -38,7 +38,8 @@ struct RodInfoT {
     R radius;
     R margin;  // contact margin (margin radii * radius)
     R selfFriction;
-    int pad[3];
+    int selfChain;  // consecutive segments (stretch constraints in index order) share a particle
+    int pad[2];
 };
 
 template <typename R>

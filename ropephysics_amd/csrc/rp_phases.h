@@ -175,18 +175,20 @@ struct SelfLds {
     int* cellStart;  // table + 1
     real* pool;     // cap * kSelfFloats
     int* misc;       // [0] live, [1] overflow
-    rbits* rmisc;    // [0] cell size, [1] longest segment, as bits (non-negative: ordered as integers)
-    int* scan;       // kScanLanes, scratch of the group-parallel scans
+    rbits* rmisc;    // [0] cell size, [1] longest segment, [2] largest |coordinate| of a centre, as bits
+                     // (non-negative: ordered as integers)
+    int* scan;       // kScanWaves, scratch of the group-parallel scans
 };
 
-constexpr int kScanLanes = 256;
+// Wavefronts a lane group may span (512 lanes).
+constexpr int kScanWaves = 8;
 
 // misc then rmisc, padded so the int arrays after them stay 16-byte aligned.
-constexpr int kMiscBytes = RP_FP64 ? 32 : 16;
+constexpr int kMiscBytes = 32;
 
 __host__ __device__ inline int self_lds_bytes(int nS, int table, int cap) {
     return nS * int(sizeof(real4)) + 4 * nS * 4 + (table + 1) * 4 + cap * kSelfFloats * int(sizeof(real)) +
-           kMiscBytes + kScanLanes * 4;
+           kMiscBytes + kScanWaves * 4;
 }
 
 __device__ inline SelfLds self_carve(char* base, int nS, int table, int cap) {
@@ -297,38 +299,49 @@ __device__ inline void self_query(const SelfLds& s, const real4* x4, const int* 
 
 // ------------------------------------------------------------------ group-parallel broadphase
 //
-// The same hash, candidate order and pool layout as self_build_hash /
-// self_allocate, built by the G lanes of a rod's lane group: the longest
-// segment by an atomic max on the (non-negative) real bits, bucket counts by
-// atomic adds, prefix sums by a chunked scan, and each bucket's items placed
-// in ascending segment order by counting the earlier segments in the same
-// bucket. Every step is exact integer work or an order-free max, so the
-// tables equal the serial build's. Every lane of the workgroup calls these
-// (they contain barriers); `on` is false for lanes whose rod has no
-// self-collision.
+// The contact list of self_build_hash / self_query / self_allocate, built by
+// the G lanes of a rod's lane group. Counting needs no hash tables: each lane
+// scans its segment's later segments in index order and weights every contact
+// by the number of times the bucketed query would reach it. The tables are
+// built only where some rod of the workgroup has contacts, for the fill pass
+// that writes them in the CPU's order. Every step is exact integer work, an
+// order-free max, or a cull that can drop only non-contacts, so the list equals
+// the serial build's. Every lane of the workgroup calls these (they contain
+// barriers); `on` is false for lanes whose rod has no self-collision.
 
-// Exclusive prefix sum of in[0, n) into out[0, n) by the first
-// min(G, kScanLanes) lanes of the group (each summing a chunk); returns the
-// total. Fixed barrier count whatever G, n or `on`; lanes past kScanLanes only
-// take part in the barriers.
+// Exclusive prefix sum of in[0, n) into out[0, n) by the G lanes of a rod's
+// lane group, each summing a chunk of ceil(n / G) entries: a scan of the chunk
+// sums within each wavefront (lane - off is in the same wavefront when its
+// position in the wavefront is at least off, and in the same group -- a
+// group's lanes are consecutive -- when l >= off), then across the wavefronts
+// the group spans through `scratch`, one entry per wavefront. Returns the
+// total. Two barriers, whatever G, n or `on`.
 __device__ inline int group_scan(const int* in, int* out, int n, int* scratch, int l, int G, bool on) {
-    const int P = min(G, kScanLanes);
-    const bool part = on && l < P;
-    const int chunk = (n + P - 1) / max(P, 1);
-    const int lo = part ? min(n, l * chunk) : 0;
-    const int hi = part ? min(n, lo + chunk) : 0;
+    const int chunk = on ? (n + G - 1) / G : 0;
+    const int lo = on ? min(n, l * chunk) : 0;
+    const int hi = on ? min(n, lo + chunk) : 0;
     int sum = 0;
     for (int i = lo; i < hi; ++i) sum += in[i];
-    if (part) scratch[l] = sum;
-    __syncthreads();
-    for (int off = 1; off < kScanLanes; off <<= 1) {
-        const int v = (part && l >= off) ? scratch[l - off] : 0;
-        __syncthreads();
-        if (part) scratch[l] += v;
-        __syncthreads();
+    int x = sum;
+    const int lane = int(threadIdx.x);
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(x, off, 64);
+        if (l >= off && (lane & 63) >= off) x += v;
     }
-    const int total = on ? scratch[P - 1] : 0;
-    int run = part ? scratch[l] - sum : 0;
+    const int g0 = lane - l;
+    const int w = (lane >> 6) - (g0 >> 6);
+    if (on && ((lane & 63) == 63 || l == G - 1)) scratch[w] = x;
+    __syncthreads();
+    int before = 0, total = 0;
+    if (on) {
+        const int wLast = ((g0 + G - 1) >> 6) - (g0 >> 6);
+        for (int i = 0; i <= wLast; ++i) {
+            const int t = scratch[i];
+            before += i < w ? t : 0;
+            total += t;
+        }
+    }
+    int run = before + x - sum;
     for (int i = lo; i < hi; ++i) {
         const int v = in[i];
         out[i] = run;
@@ -343,20 +356,25 @@ __device__ __forceinline__ int count_eq4(int4 c, int h) {
     return (c.x == h ? 1 : 0) + (c.y == h ? 1 : 0) + (c.z == h ? 1 : 0) + (c.w == h ? 1 : 0);
 }
 
-// self_query for the lane group: segment j's centre and length (cj4, held by
-// the lane) and candidates' centres stored in bucket order (s.cen[e] belongs
-// to s.sorted[e]), so a candidate costs one LDS round trip; the three buckets
-// of a (dz, dy) row are fetched together. Same candidates in the same order as
-// self_query, with one addition: a candidate whose centre is farther from j's
-// than the two half-lengths plus 2r (with a relative margin of 1e-3 and an
-// absolute one of 1e-6 times the coordinate scale, far above the FP32
-// rounding of centres and lengths) is dropped before the closest-point test.
-// Every point of a segment lies within half its length of its centre, so such
-// a pair's computed separation is positive and it would not be a contact: the
-// contact list is unchanged. Returns the number of contacts.
-template <bool kFill>
-__device__ inline int self_query_rows(const SelfLds& s, const real4* x4, const int* segEnd, int table, int gap,
-                                      int cap, real radius, real friction, real cell, int j, real4 cj4,
+// Bucket of cell (cx + dx, cy + dy, cz + dz) for dx, dy, dz in -1..1, with the
+// row term hzy = (cz + dz) p3 ^ (cy + dy) p2 given: cell_hash's arithmetic.
+__device__ __forceinline__ int row_bucket(int cx, int dx, unsigned hzy, unsigned tbl, bool pow2) {
+    const unsigned hh = (static_cast<unsigned>(cx + dx) * 73856093u) ^ hzy;
+    return static_cast<int>(pow2 ? (hh & (tbl - 1u)) : hh % tbl);
+}
+
+// The fill pass of self_query for the lane group: segment j's contacts, in
+// the CPU's candidate order (27 cells dz, dy, dx; bucket items ascending; a
+// bucket reached twice is visited twice), written from slotBase. The three
+// buckets of a (dz, dy) row are fetched together and their candidates tested
+// four at a time; a candidate whose centre is farther from j's than the two
+// half-lengths plus 2r (with a relative margin of 1e-3 and an absolute one of
+// 1e-6 times the coordinate scale, far above the rounding of centres and
+// lengths) cannot be a contact and skips the closest-point test. Every point of
+// a segment lies within half its length of its centre, so the list is
+// unchanged.
+__device__ inline void self_fill_rows(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
+                                      int gap, int cap, real radius, real friction, real cell, int j, real4 cj4,
                                       int slotBase) {
     const v3 cj = f4v(cj4);
     const real reachJ = real(0.5) * cj4.w + real(2.0) * radius;
@@ -371,22 +389,15 @@ __device__ inline int self_query_rows(const SelfLds& s, const real4* x4, const i
     int count = 0;
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy) {
-            // cell_hash's XOR of the three products, the x term varying fastest.
             const unsigned hzy =
                 (static_cast<unsigned>(cz + dz) * 83492791u) ^ (static_cast<unsigned>(cy + dy) * 19349663u);
             int lo[3], hi[3];
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
-                const unsigned hh = (static_cast<unsigned>(cx + dx - 1) * 73856093u) ^ hzy;
-                const int h = static_cast<int>(pow2 ? (hh & (tbl - 1u)) : hh % tbl);
+                const int h = row_bucket(cx, dx - 1, hzy, tbl, pow2);
                 lo[dx] = s.cellStart[h];
                 hi[dx] = s.cellStart[h + 1];
             }
-            // The row's candidates, buckets dx = -1, 0, 1 concatenated, four
-            // at a time: their loads go out together (indices past the end
-            // repeat the last candidate, so every load is in range and none
-            // waits on a test), the culls run side by side without branches,
-            // and survivors reach the narrow phase in candidate order.
             const int n0 = hi[0] - lo[0], n01 = n0 + hi[1] - lo[1], total = n01 + hi[2] - lo[2];
             for (int base = 0; base < total; base += 4) {
                 int kq[4];
@@ -395,15 +406,18 @@ __device__ inline int self_query_rows(const SelfLds& s, const real4* x4, const i
                 for (int u = 0; u < 4; ++u) {
                     const int q = min(base + u, total - 1);
                     const int e = q < n0 ? lo[0] + q : (q < n01 ? lo[1] + (q - n0) : lo[2] + (q - n01));
-                    kq[u] = s.sorted[e];
-                    cq[u] = s.cen[e];
+                    // Bucket items are segment indices below nS; the clamp keeps
+                    // every load in range whatever the tables hold.
+                    kq[u] = min(max(s.sorted[e], 0), nS - 1);
                 }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) cq[u] = s.cen[kq[u]];
                 unsigned pass = 0u;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const v3 ck = f4v(cq[u]);
-                    const real lim =
-                        real(1.001) * (reachJ + real(0.5) * cq[u].w) + real(1e-6) * (real(1.0) + r_max(scaleJ, abs_max3(ck)));
+                    const real lim = real(1.001) * (reachJ + real(0.5) * cq[u].w) +
+                                     real(1e-6) * (real(1.0) + r_max(scaleJ, abs_max3(ck)));
                     const bool keep = (base + u < total) & (kq[u] - j >= gap) & !(norm2(sub(ck, cj)) > lim * lim);
                     pass |= keep ? (1u << u) : 0u;
                 }
@@ -411,24 +425,87 @@ __device__ inline int self_query_rows(const SelfLds& s, const real4* x4, const i
                     const int u = __builtin_ctz(pass);
                     pass &= pass - 1u;
                     const int k = u == 0 ? kq[0] : u == 1 ? kq[1] : u == 2 ? kq[2] : kq[3];
-                    self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * k], segEnd[2 * k + 1], radius, friction, kFill,
+                    self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * k], segEnd[2 * k + 1], radius, friction, true,
                               slotBase, cap, count);
                 }
             }
         }
+}
+
+// Number of contacts segment j has in the list: its later segments k >= j +
+// gap scanned in index order, each contact counted once for every one of j's
+// 27 cells whose bucket is k's (the times the bucketed query reaches k). The
+// fill pass's cull, with k's own half-length and coordinate scale, keeps every
+// contact; `lim` is that cull radius at the longest segment and the largest
+// coordinate, a bound for any k. When consecutive segments share a particle
+// (`chain`), consecutive centres are at most the longest segment apart, so a
+// segment at distance D > lim also rules out the next floor((D - lim) / step)
+// segments, step being that bound with a 1e-3 margin: far parts of the rod
+// cost one test per stretch of segments.
+__device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
+                                      int gap, real radius, real cell, real lenMax, real scaleMax, bool chain,
+                                      int j, real4 cj4) {
+    const v3 cj = f4v(cj4);
+    const real reachJ = real(0.5) * cj4.w + real(2.0) * radius;
+    const real scaleJ = abs_max3(cj);
+    const real lim = real(1.001) * (reachJ + real(0.5) * lenMax) + real(1e-6) * (real(1.0) + scaleMax);
+    const real lim2 = lim * lim;
+    const float limF = float(lim);
+    const float invStep = chain && lenMax > real(0.0) ? 0.999f / (1.001f * float(lenMax)) : 0.0f;
+    const unsigned tbl = static_cast<unsigned>(table);
+    const bool pow2 = __all((tbl & (tbl - 1u)) == 0u) != 0;
+    int j0 = -1, j1 = 0;
+    v3 a0 = mk(real(0.), real(0.), real(0.)), a1 = a0;
+    int count = 0;
+    int k = j + gap;
+    while (k < nS) {
+        const real4 ck4 = s.cen[k];
+        const v3 ck = f4v(ck4);
+        const real D2 = norm2(sub(ck, cj));
+        if (D2 > lim2) {
+            const float sk = (__builtin_amdgcn_sqrtf(float(D2)) - limF) * invStep;
+            k += 1 + (sk > 0.0f ? int(fminf(sk, float(nS))) : 0);
+            continue;
+        }
+        const real limK =
+            real(1.001) * (reachJ + real(0.5) * ck4.w) + real(1e-6) * (real(1.0) + r_max(scaleJ, abs_max3(ck)));
+        if (!(D2 > limK * limK)) {
+            if (j0 < 0) {
+                j0 = segEnd[2 * j];
+                j1 = segEnd[2 * j + 1];
+                a0 = f4v(x4[j0]);
+                a1 = f4v(x4[j1]);
+            }
+            int c = 0;
+            self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * k], segEnd[2 * k + 1], radius, real(0.0), false, 0, 0, c);
+            if (c > 0) {
+                const int cx = cell_coord(cj.x, cell), cy = cell_coord(cj.y, cell), cz = cell_coord(cj.z, cell);
+                const int hk = s.cellOf[k];
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const unsigned hzy = (static_cast<unsigned>(cz + dz) * 83492791u) ^
+                                             (static_cast<unsigned>(cy + dy) * 19349663u);
+#pragma unroll
+                        for (int dx = -1; dx <= 1; ++dx) count += row_bucket(cx, dx, hzy, tbl, pow2) == hk ? 1 : 0;
+                    }
+            }
+        }
+        ++k;
+    }
     return count;
 }
 
 // Broadphase and contact generation by the G lanes of a rod's lane group; lane
 // l owns segments j = l + G t, t < TP (the host guarantees nS <= TP G), and
-// keeps their centres in registers. s.cen ends up in bucket order.
+// keeps their centres in registers; s.cen holds them in segment order.
 template <int TP>
 __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
-                                        int gap, int cap, real radius, real friction, bool on, int l, int G) {
+                                        int gap, int cap, real radius, real friction, bool chain, bool on, int l,
+                                        int G) {
     real4 cj[TP];
-    if (on) {
-        for (int t = l; t <= table; t += G) s.cellStart[t] = 0;
-        if (l == 0) s.rmisc[1] = 0;
+    if (on && l == 0) {
+        s.rmisc[1] = 0;
+        s.rmisc[2] = 0;
     }
     __syncthreads();
 #pragma unroll
@@ -438,33 +515,63 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
         if (on && j < nS) {
             const v3 a = f4v(x4[segEnd[2 * j]]), b = f4v(x4[segEnd[2 * j + 1]]);
             const real len = norm(sub(b, a));
-            cj[t] = vf4(scale(add(a, b), real(0.5)), len);
+            const v3 c = scale(add(a, b), real(0.5));
+            cj[t] = vf4(c, len);
+            s.cen[j] = cj[t];
             atomicMax(&s.rmisc[1], real_bits(len));
+            atomicMax(&s.rmisc[2], real_bits(abs_max3(c)));
         }
     }
     __syncthreads();
-    const real cell = on ? bits_real(s.rmisc[1]) + real(2.0) * radius : real(0.0);
+    const real lenMax = on ? bits_real(s.rmisc[1]) : real(0.0);
+    const real scaleMax = on ? bits_real(s.rmisc[2]) : real(0.0);
+    const real cell = lenMax + real(2.0) * radius;
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
         if (on && j < nS) {
             const v3 c = f4v(cj[t]);
-            const int h = int(cell_hash(cell_coord(c.x, cell), cell_coord(c.y, cell), cell_coord(c.z, cell), table));
-            s.cellOf[j] = h;
-            atomicAdd(&s.cellStart[h], 1);
+            s.cellOf[j] = int(cell_hash(cell_coord(c.x, cell), cell_coord(c.y, cell), cell_coord(c.z, cell), table));
         }
     }
     if (on && l == 0) s.rmisc[0] = real_bits(cell);
     __syncthreads();
-    const int items = group_scan(s.cellStart, s.cellStart, on ? table : 0, s.scan, l, G, on);
-    if (on && l == 0) s.cellStart[table] = items;
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+        const int j = l + G * t;
+        if (on && j < nS)
+            s.segCount[j] = nS > gap ? self_count_scan(s, x4, segEnd, nS, table, gap, radius, cell, lenMax, scaleMax,
+                                                       chain, j, cj[t])
+                                     : 0;
+    }
+    __syncthreads();
+    const int total = group_scan(s.segCount, s.segStart, on ? nS : 0, s.scan, l, G, on);
+    if (on && l == 0) {
+        s.misc[0] = total < cap ? total : cap;
+        if (total > cap) s.misc[1] += total - cap;
+    }
+    // The fill pass needs the bucket tables; they are built only where some
+    // rod of the workgroup has contacts.
+    if (__syncthreads_or(on && total > 0) == 0) return;
+    const bool build = on && total > 0;
+    if (build)
+        for (int t = l; t <= table; t += G) s.cellStart[t] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+        const int j = l + G * t;
+        if (build && j < nS) atomicAdd(&s.cellStart[s.cellOf[j]], 1);
+    }
+    __syncthreads();
+    const int items = group_scan(s.cellStart, s.cellStart, build ? table : 0, s.scan, l, G, build);
+    if (build && l == 0) s.cellStart[table] = items;
     // Stable placement: j goes after the earlier segments of its bucket.
     // cellOf is 16-byte aligned, so four buckets per LDS read.
     const int4* cellOf4 = reinterpret_cast<const int4*>(s.cellOf);
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
-        if (on && j < nS) {
+        if (build && j < nS) {
             const int h = s.cellOf[j];
             int rank = 0, i = 0;
             for (; i + 16 <= j; i += 16) {
@@ -474,32 +581,15 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
             }
             for (; i + 4 <= j; i += 4) rank += count_eq4(cellOf4[i >> 2], h);
             for (; i < j; ++i) rank += s.cellOf[i] == h ? 1 : 0;
-            const int e = s.cellStart[h] + rank;
-            s.sorted[e] = j;
-            s.cen[e] = cj[t];
+            s.sorted[s.cellStart[h] + rank] = j;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
-        if (on && j < nS)
-            s.segCount[j] = nS > gap ? self_query_rows<false>(s, x4, segEnd, table, gap, cap, radius, friction,
-                                                              cell, j, cj[t], 0)
-                                     : 0;
-    }
-    __syncthreads();
-    const int total = group_scan(s.segCount, s.segStart, on ? nS : 0, s.scan, l, G, on);
-    if (on && l == 0) {
-        s.misc[0] = total < cap ? total : cap;
-        if (total > cap) s.misc[1] += total - cap;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < TP; ++t) {
-        const int j = l + G * t;
-        if (on && j < nS && s.segCount[j] > 0)
-            self_query_rows<true>(s, x4, segEnd, table, gap, cap, radius, friction, cell, j, cj[t], s.segStart[j]);
+        if (build && j < nS && s.segCount[j] > 0)
+            self_fill_rows(s, x4, segEnd, nS, table, gap, cap, radius, friction, cell, j, cj[t], s.segStart[j]);
     }
 }
 

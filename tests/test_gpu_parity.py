@@ -262,6 +262,21 @@ def test_self_collision_pinned_coil_hash_tables(strategy, table):
     assert seen >= 10  # the coil really is in contact
 
 
+@pytest.mark.parametrize("tile_config", [9, 11, 12, 13])
+def test_self_collision_lane_groups_across_wavefronts(tile_config):
+    """Self-colliding rods of 61-81 particles at one or two particles per lane,
+    packed side by side: lane groups start mid-wavefront and span wavefront
+    boundaries, which the group-parallel scans and broadphase must handle.
+    Bitwise equal to the oracle, every shape."""
+    world = rp.CollisionWorld(primitives=[rp.Primitive.plane((0, 0, 0), (0, 0, 1), 0.6)], self_collision=True,
+                              self_friction=0.3, hash_table_size=64, self_capacity=700)
+    rods = [benchmark_rod(24), pinned_coil(), helix_rope(), pinned_coil(), benchmark_rod(40)]
+    diffs, _, b = run_pair(rods, world, SELF_PARAMS, 12, "tiled", checkpoints=[2, 12], tile_config=tile_config)
+    assert b.counters()["tile_config"] == tile_config
+    assert b.self_contact_overflow() == 0
+    assert_bitwise(diffs)
+
+
 @pytest.mark.parametrize("strategy", STRATEGIES)
 def test_ragged_batch_and_alone_equals_mixed(strategy):
     sizes = [7, 20, 33, 64, 97, 130, 160, 48]
