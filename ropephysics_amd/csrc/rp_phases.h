@@ -174,17 +174,19 @@ struct SelfLds {
     int* segStart;   // nS
     int* cellStart;  // table + 1
     real* pool;     // cap * kSelfFloats
-    int* misc;       // [0] live, [1] overflow
-    rbits* rmisc;    // [0] cell size, [1] longest segment, [2] largest |coordinate| of a centre, as bits
-                     // (non-negative: ordered as integers)
+    int* misc;       // [0] live, [1] overflow, [2] long segments, [3..6] the first four of them
+    rbits* rmisc;    // [0] cell size, [1] longest segment, [2] largest |coordinate| of a centre, [3] shortest
+                     // segment, as bits (non-negative: ordered as integers)
     int* scan;       // kScanWaves, scratch of the group-parallel scans
 };
 
 // Wavefronts a lane group may span (512 lanes).
 constexpr int kScanWaves = 8;
 
-// misc then rmisc, padded so the int arrays after them stay 16-byte aligned.
-constexpr int kMiscBytes = 32;
+// misc (8 ints) then rmisc (4 reals), padded so the int arrays after them stay
+// 16-byte aligned.
+constexpr int kMiscBytes = 64;
+constexpr int kLongSlots = 4;
 
 __host__ __device__ inline int self_lds_bytes(int nS, int table, int cap) {
     return nS * int(sizeof(real4)) + 4 * nS * 4 + (table + 1) * 4 + cap * kSelfFloats * int(sizeof(real)) +
@@ -196,7 +198,7 @@ __device__ inline SelfLds self_carve(char* base, int nS, int table, int cap) {
     s.cen = reinterpret_cast<real4*>(base);
     s.pool = reinterpret_cast<real*>(base + nS * int(sizeof(real4)));
     s.misc = reinterpret_cast<int*>(base + nS * int(sizeof(real4)) + cap * kSelfFloats * int(sizeof(real)));
-    s.rmisc = reinterpret_cast<rbits*>(s.misc + 2);
+    s.rmisc = reinterpret_cast<rbits*>(s.misc + 8);
     s.cellOf = s.misc + kMiscBytes / 4;
     s.sorted = s.cellOf + nS;
     s.segCount = s.sorted + nS;
@@ -436,46 +438,67 @@ __device__ inline void self_fill_rows(const SelfLds& s, const real4* x4, const i
 // gap scanned in index order, each contact counted once for every one of j's
 // 27 cells whose bucket is k's (the times the bucketed query reaches k). The
 // fill pass's cull, with k's own half-length and coordinate scale, keeps every
-// contact; `lim` is that cull radius at the longest segment and the largest
-// coordinate, a bound for any k. When consecutive segments share a particle
-// (`chain`), consecutive centres are at most the longest segment apart, so a
-// segment at distance D > lim also rules out the next floor((D - lim) / step)
-// segments, step being that bound with a 1e-3 margin: far parts of the rod
-// cost one test per stretch of segments.
+// contact. When consecutive segments share a particle (`chain`), consecutive
+// centres are at most the longer of the two segments apart, so a segment k at
+// distance D also rules out the following segments k' while D stays above the
+// cull radius at their bound: floor((D - lim) / step) of them, lim being the
+// cull radius at a length bound and step that bound with a 1e-3 margin. The
+// bound is lenCap over a stretch with no segment longer than lenCap (the rod's
+// "long" segments, listed in `longs`, end a stretch) and the longest segment
+// from a long one. Far parts of the rod cost one test per stretch of segments.
+// Before the closest-point test, a pair is also dropped when segment j is
+// farther from k's centre than k's half-length plus 2r (same margins): every
+// point of segment k lies within its half-length of its centre.
 __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
-                                      int gap, real radius, real cell, real lenMax, real scaleMax, bool chain,
-                                      int j, real4 cj4) {
+                                      int gap, real radius, real cell, real lenMax, real lenCap, const int* longs,
+                                      real scaleMax, bool chain, int j, real4 cj4) {
     const v3 cj = f4v(cj4);
     const real reachJ = real(0.5) * cj4.w + real(2.0) * radius;
     const real scaleJ = abs_max3(cj);
-    const real lim = real(1.001) * (reachJ + real(0.5) * lenMax) + real(1e-6) * (real(1.0) + scaleMax);
-    const real lim2 = lim * lim;
-    const float limF = float(lim);
-    const float invStep = chain && lenMax > real(0.0) ? 0.999f / (1.001f * float(lenMax)) : 0.0f;
+    const real eps = real(1e-6) * (real(1.0) + scaleMax);
+    const float limCap = float(real(1.001) * (reachJ + real(0.5) * lenCap) + eps);
+    const float limMax = float(real(1.001) * (reachJ + real(0.5) * lenMax) + eps);
+    const float invCap = chain && lenCap > real(0.0) ? 0.999f / (1.001f * float(lenCap)) : 0.0f;
+    const float invMax = chain && lenMax > real(0.0) ? 0.999f / (1.001f * float(lenMax)) : 0.0f;
     const unsigned tbl = static_cast<unsigned>(table);
     const bool pow2 = __all((tbl & (tbl - 1u)) == 0u) != 0;
     int j0 = -1, j1 = 0;
-    v3 a0 = mk(real(0.), real(0.), real(0.)), a1 = a0;
+    v3 a0 = mk(real(0.), real(0.), real(0.)), a1 = a0, dj = a0;
+    real invA = real(0.0);
     int count = 0;
     int k = j + gap;
     while (k < nS) {
         const real4 ck4 = s.cen[k];
         const v3 ck = f4v(ck4);
         const real D2 = norm2(sub(ck, cj));
-        if (D2 > lim2) {
-            const float sk = (__builtin_amdgcn_sqrtf(float(D2)) - limF) * invStep;
-            k += 1 + (sk > 0.0f ? int(fminf(sk, float(nS))) : 0);
+        const real limK = real(1.001) * (reachJ + real(0.5) * ck4.w) + real(1e-6) * (real(1.0) + r_max(scaleJ, abs_max3(ck)));
+        if (D2 > limK * limK) {
+            // The first long segment at or after k ends the stretch.
+            int lf = nS;
+#pragma unroll
+            for (int i = 0; i < kLongSlots; ++i) lf = longs[i] >= k ? min(lf, longs[i]) : lf;
+            const float D = __builtin_amdgcn_sqrtf(float(D2));
+            const float sk = lf > k ? (D - limCap) * invCap : (D - limMax) * invMax;
+            int next = k + 1 + (sk > 0.0f ? int(fminf(sk, float(nS))) : 0);
+            k = lf > k ? min(next, lf) : next;
             continue;
         }
-        const real limK =
-            real(1.001) * (reachJ + real(0.5) * ck4.w) + real(1e-6) * (real(1.0) + r_max(scaleJ, abs_max3(ck)));
-        if (!(D2 > limK * limK)) {
-            if (j0 < 0) {
-                j0 = segEnd[2 * j];
-                j1 = segEnd[2 * j + 1];
-                a0 = f4v(x4[j0]);
-                a1 = f4v(x4[j1]);
-            }
+        if (j0 < 0) {
+            j0 = segEnd[2 * j];
+            j1 = segEnd[2 * j + 1];
+            a0 = f4v(x4[j0]);
+            a1 = f4v(x4[j1]);
+            dj = sub(a1, a0);
+            const real a = norm2(dj);
+            invA = a > real(1e-30) ? real(1.0) / a : real(0.0);
+        }
+        // Distance from segment j to k's centre (the nearest point of j up to
+        // rounding, far inside the margin).
+        const real tq = r_max(real(0.0), min_ab(real(1.0), dot(sub(ck, a0), dj) * invA));
+        const v3 off = sub(add(a0, scale(dj, tq)), ck);
+        const real limP = real(1.001) * (real(0.5) * ck4.w + real(2.0) * radius) +
+                          real(1e-6) * (real(1.0) + r_max(scaleJ, abs_max3(ck)));
+        if (!(norm2(off) > limP * limP)) {
             int c = 0;
             self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * k], segEnd[2 * k + 1], radius, real(0.0), false, 0, 0, c);
             if (c > 0) {
@@ -504,8 +527,10 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
                                         int G) {
     real4 cj[TP];
     if (on && l == 0) {
+        s.misc[2] = 0;
         s.rmisc[1] = 0;
         s.rmisc[2] = 0;
+        s.rmisc[3] = real_bits(real(1e30));
     }
     __syncthreads();
 #pragma unroll
@@ -520,11 +545,15 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
             s.cen[j] = cj[t];
             atomicMax(&s.rmisc[1], real_bits(len));
             atomicMax(&s.rmisc[2], real_bits(abs_max3(c)));
+            atomicMin(&s.rmisc[3], real_bits(len));
         }
     }
     __syncthreads();
     const real lenMax = on ? bits_real(s.rmisc[1]) : real(0.0);
     const real scaleMax = on ? bits_real(s.rmisc[2]) : real(0.0);
+    // Segments longer than twice the shortest are "long"; the first few are
+    // listed, and skips stop at them.
+    const real lenLong = on ? real(2.0) * bits_real(s.rmisc[3]) : real(0.0);
     const real cell = lenMax + real(2.0) * radius;
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
@@ -532,16 +561,28 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
         if (on && j < nS) {
             const v3 c = f4v(cj[t]);
             s.cellOf[j] = int(cell_hash(cell_coord(c.x, cell), cell_coord(c.y, cell), cell_coord(c.z, cell), table));
+            if (cj[t].w > lenLong) {
+                const int slot = atomicAdd(&s.misc[2], 1);
+                if (slot < kLongSlots) s.misc[3 + slot] = j;
+            }
         }
     }
     if (on && l == 0) s.rmisc[0] = real_bits(cell);
     __syncthreads();
+    // With more long segments than slots, the longest segment bounds every
+    // stretch.
+    int longs[kLongSlots];
+    const int numLong = on ? s.misc[2] : 0;
+    const bool listed = numLong <= kLongSlots && lenLong > real(0.0);
+    const real lenCap = listed ? lenLong : lenMax;
+#pragma unroll
+    for (int i = 0; i < kLongSlots; ++i) longs[i] = listed && i < numLong ? s.misc[3 + i] : nS;
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
         if (on && j < nS)
-            s.segCount[j] = nS > gap ? self_count_scan(s, x4, segEnd, nS, table, gap, radius, cell, lenMax, scaleMax,
-                                                       chain, j, cj[t])
+            s.segCount[j] = nS > gap ? self_count_scan(s, x4, segEnd, nS, table, gap, radius, cell, lenMax, lenCap,
+                                                       longs, scaleMax, chain, j, cj[t])
                                      : 0;
     }
     __syncthreads();
