@@ -194,6 +194,7 @@ struct prim {
     real radius, friction;
 };
 
+RP_HD real abs_max3v(v3 a) { return r_max(r_abs(a.x), r_max(r_abs(a.y), r_abs(a.z))); }
 RP_HD real clamp01(real v) { return v < real(0.0) ? real(0.0) : (v > real(1.0) ? real(1.0) : v); }
 RP_HD real absf_(real v) { return v < real(0.0) ? -v : v; }
 RP_HD real clampf_(real v, real lo, real hi) { return v < lo ? lo : (v > hi ? hi : v); }

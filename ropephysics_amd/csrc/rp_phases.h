@@ -88,7 +88,34 @@ __device__ __forceinline__ void stage_prim(real* s, const Prim& g) {
     for (int c = 0; c < 4; ++c) s[13 + c] = g.rot[c];
     s[17] = g.radius;
     s[18] = g.friction;
-    s[19] = real(0.0);
+    // Bounding radius about the centre (a, or the capsule's mid-axis): every
+    // point of the shape lies within it; -1 for planes.
+    real bound = real(-1.0);
+    if (g.type == kSphere) {
+        bound = g.radius;
+    } else if (g.type == kCapsule) {
+        const v3 ab = mk(g.b[0] - g.a[0], g.b[1] - g.a[1], g.b[2] - g.a[2]);
+        bound = real(0.5) * norm(ab) + g.radius;
+    } else if (g.type == kBox) {
+        bound = norm(mk(g.half[0], g.half[1], g.half[2]));
+    }
+    s[19] = bound;
+}
+
+// A particle farther from a convex primitive's bounding sphere than radius +
+// margin (with a 1e-3 relative and 1e-6 * coordinate-scale absolute allowance,
+// far above the rounding of the signed distance) has no contact with it: the
+// signed distance of a convex shape is at least the distance to its centre
+// less the bounding radius.
+__device__ __forceinline__ bool prim_far(const real* s, v3 x, real radius, real margin) {
+    const real bound = s[19];
+    if (bound < real(0.0)) return false;
+    const int type = slot_int(s[0]);
+    v3 c = mk(s[1], s[2], s[3]);
+    if (type == kCapsule) c = scale(add(c, mk(s[4], s[5], s[6])), real(0.5));
+    const real lim = real(1.001) * (bound + radius + r_max(margin, real(0.0))) +
+                     real(1e-6) * (real(1.0) + r_max(abs_max3v(x), abs_max3v(c)));
+    return norm2(sub(x, c)) > lim * lim;
 }
 
 // World contacts of one particle: regenerate (or just reset) its P slots.
@@ -100,7 +127,10 @@ __device__ __forceinline__ void world_contacts_begin(real* slots, const real* pr
         s[6] = real(0.0);  // applied
         if (!regen) continue;
         wcontact c;
-        world_contact_generate(c, load_prim(prims + k * kPrimFloats), x, w, radius, margin, fscale);
+        if (prim_far(prims + k * kPrimFloats, x, radius, margin))
+            c.active = 0;
+        else
+            world_contact_generate(c, load_prim(prims + k * kPrimFloats), x, w, radius, margin, fscale);
         s[7] = int_slot(c.active);
         if (c.active) {
             s[0] = c.n.x;
@@ -437,8 +467,8 @@ __device__ inline void self_fill_rows(const SelfLds& s, const real4* x4, const i
 // Number of contacts segment j has in the list: its later segments k >= j +
 // gap scanned in index order, each contact counted once for every one of j's
 // 27 cells whose bucket is k's (the times the bucketed query reaches k). The
-// fill pass's cull, with k's own half-length and coordinate scale, keeps every
-// contact. When consecutive segments share a particle (`chain`), consecutive
+// fill pass's cull, with k's own half-length and the rod's largest coordinate
+// for the scale, keeps every contact. When consecutive segments share a particle (`chain`), consecutive
 // centres are at most the longer of the two segments apart, so a segment k at
 // distance D also rules out the following segments k' while D stays above the
 // cull radius at their bound: floor((D - lim) / step) of them, lim being the
@@ -454,7 +484,6 @@ __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const i
                                       real scaleMax, bool chain, int j, real4 cj4) {
     const v3 cj = f4v(cj4);
     const real reachJ = real(0.5) * cj4.w + real(2.0) * radius;
-    const real scaleJ = abs_max3(cj);
     const real eps = real(1e-6) * (real(1.0) + scaleMax);
     const float limCap = float(real(1.001) * (reachJ + real(0.5) * lenCap) + eps);
     const float limMax = float(real(1.001) * (reachJ + real(0.5) * lenMax) + eps);
@@ -471,7 +500,7 @@ __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const i
         const real4 ck4 = s.cen[k];
         const v3 ck = f4v(ck4);
         const real D2 = norm2(sub(ck, cj));
-        const real limK = real(1.001) * (reachJ + real(0.5) * ck4.w) + real(1e-6) * (real(1.0) + r_max(scaleJ, abs_max3(ck)));
+        const real limK = real(1.001) * (reachJ + real(0.5) * ck4.w) + eps;
         if (D2 > limK * limK) {
             // The first long segment at or after k ends the stretch.
             int lf = nS;
@@ -496,8 +525,7 @@ __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const i
         // rounding, far inside the margin).
         const real tq = r_max(real(0.0), min_ab(real(1.0), dot(sub(ck, a0), dj) * invA));
         const v3 off = sub(add(a0, scale(dj, tq)), ck);
-        const real limP = real(1.001) * (real(0.5) * ck4.w + real(2.0) * radius) +
-                          real(1e-6) * (real(1.0) + r_max(scaleJ, abs_max3(ck)));
+        const real limP = real(1.001) * (real(0.5) * ck4.w + real(2.0) * radius) + eps;
         if (!(norm2(off) > limP * limP)) {
             int c = 0;
             self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * k], segEnd[2 * k + 1], radius, real(0.0), false, 0, 0, c);
