@@ -468,32 +468,26 @@ __device__ inline void self_fill_rows(const SelfLds& s, const real4* x4, const i
 // gap scanned in index order, each contact counted once for every one of j's
 // 27 cells whose bucket is k's (the times the bucketed query reaches k). The
 // fill pass's cull, with k's own half-length and the rod's largest coordinate
-// for the scale, keeps every contact. When consecutive segments share a particle (`chain`), consecutive
-// centres are at most the longer of the two segments apart, so a segment k at
-// distance D also rules out the following segments k' while D stays above the
-// cull radius at their bound: floor((D - lim) / step) of them, lim being the
-// cull radius at a length bound and step that bound with a 1e-3 margin. The
-// bound is lenCap over a stretch with no segment longer than lenCap (the rod's
-// "long" segments, listed in `longs`, end a stretch) and the longest segment
-// from a long one. Far parts of the rod cost one test per stretch of segments.
+// for the scale, keeps every contact. When consecutive segments share a
+// particle (`chain`), consecutive centres are at most the longer of the two
+// segments apart, so a segment k at distance D also rules out the following
+// segments k' while D stays above the cull radius at their bound:
+// floor((D - lim) / step) of them, lim being the cull radius at the bound
+// lenCap and step lenCap with a 1e-3 margin. No segment of such a stretch may
+// be longer than lenCap: nextLong[k] is the first segment at or after k that
+// is (nS if none), and a stretch ends there; from such a segment the scan
+// steps by one. Far parts of the rod cost one test per stretch of segments.
 // Before the closest-point test, a pair is also dropped when segment j is
 // farther from k's centre than k's half-length plus 2r (same margins): every
 // point of segment k lies within its half-length of its centre.
-__device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
-                                      int gap, real radius, real cell, real lenMax, real lenCap, const int* longs,
-                                      real scaleMax, bool chain, int j, real4 cj4) {
+__device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const int* segEnd, const int* nextLong,
+                                      int nS, int table, int gap, real radius, real cell, real lenCap, real scaleMax,
+                                      bool chain, int j, real4 cj4) {
     const v3 cj = f4v(cj4);
     const real reachJ = real(0.5) * cj4.w + real(2.0) * radius;
     const real eps = real(1e-6) * (real(1.0) + scaleMax);
     const float limCap = float(real(1.001) * (reachJ + real(0.5) * lenCap) + eps);
-    const float limMax = float(real(1.001) * (reachJ + real(0.5) * lenMax) + eps);
     const float invCap = chain && lenCap > real(0.0) ? 0.999f / (1.001f * float(lenCap)) : 0.0f;
-    const float invMax = chain && lenMax > real(0.0) ? 0.999f / (1.001f * float(lenMax)) : 0.0f;
-    const unsigned tbl = static_cast<unsigned>(table);
-    const bool pow2 = __all((tbl & (tbl - 1u)) == 0u) != 0;
-    int j0 = -1, j1 = 0;
-    v3 a0 = mk(real(0.), real(0.), real(0.)), a1 = a0, dj = a0;
-    real invA = real(0.0);
     int count = 0;
     int k = j + gap;
     while (k < nS) {
@@ -502,34 +496,27 @@ __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const i
         const real D2 = norm2(sub(ck, cj));
         const real limK = real(1.001) * (reachJ + real(0.5) * ck4.w) + eps;
         if (D2 > limK * limK) {
-            // The first long segment at or after k ends the stretch.
-            int lf = nS;
-#pragma unroll
-            for (int i = 0; i < kLongSlots; ++i) lf = longs[i] >= k ? min(lf, longs[i]) : lf;
-            const float D = __builtin_amdgcn_sqrtf(float(D2));
-            const float sk = lf > k ? (D - limCap) * invCap : (D - limMax) * invMax;
-            int next = k + 1 + (sk > 0.0f ? int(fminf(sk, float(nS))) : 0);
-            k = lf > k ? min(next, lf) : next;
+            const int lf = nextLong[k];
+            const float sk = (__builtin_amdgcn_sqrtf(float(D2)) - limCap) * invCap;
+            const int next = k + 1 + (sk > 0.0f ? int(fminf(sk, float(nS))) : 0);
+            k = lf > k ? min(next, lf) : k + 1;
             continue;
         }
-        if (j0 < 0) {
-            j0 = segEnd[2 * j];
-            j1 = segEnd[2 * j + 1];
-            a0 = f4v(x4[j0]);
-            a1 = f4v(x4[j1]);
-            dj = sub(a1, a0);
-            const real a = norm2(dj);
-            invA = a > real(1e-30) ? real(1.0) / a : real(0.0);
-        }
+        const int j0 = segEnd[2 * j], j1 = segEnd[2 * j + 1];
+        const v3 a0 = f4v(x4[j0]), a1 = f4v(x4[j1]);
+        const v3 dj = sub(a1, a0);
+        const real a = norm2(dj);
         // Distance from segment j to k's centre (the nearest point of j up to
         // rounding, far inside the margin).
-        const real tq = r_max(real(0.0), min_ab(real(1.0), dot(sub(ck, a0), dj) * invA));
+        const real tq = a > real(1e-30) ? r_max(real(0.0), min_ab(real(1.0), dot(sub(ck, a0), dj) / a)) : real(0.0);
         const v3 off = sub(add(a0, scale(dj, tq)), ck);
         const real limP = real(1.001) * (real(0.5) * ck4.w + real(2.0) * radius) + eps;
         if (!(norm2(off) > limP * limP)) {
             int c = 0;
             self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * k], segEnd[2 * k + 1], radius, real(0.0), false, 0, 0, c);
             if (c > 0) {
+                const unsigned tbl = static_cast<unsigned>(table);
+                const bool pow2 = (tbl & (tbl - 1u)) == 0u;
                 const int cx = cell_coord(cj.x, cell), cy = cell_coord(cj.y, cell), cz = cell_coord(cj.z, cell);
                 const int hk = s.cellOf[k];
                 for (int dz = -1; dz <= 1; ++dz)
@@ -597,20 +584,31 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
     }
     if (on && l == 0) s.rmisc[0] = real_bits(cell);
     __syncthreads();
-    // With more long segments than slots, the longest segment bounds every
-    // stretch.
-    int longs[kLongSlots];
+    // First long segment at or after each segment, in s.sorted until the fill
+    // pass needs it. With more long segments than slots, the longest segment
+    // bounds every stretch and none ends one.
     const int numLong = on ? s.misc[2] : 0;
     const bool listed = numLong <= kLongSlots && lenLong > real(0.0);
     const real lenCap = listed ? lenLong : lenMax;
 #pragma unroll
-    for (int i = 0; i < kLongSlots; ++i) longs[i] = listed && i < numLong ? s.misc[3 + i] : nS;
+    for (int t = 0; t < TP; ++t) {
+        const int j = l + G * t;
+        if (on && j < nS) {
+            int lf = nS;
+            for (int i = 0; listed && i < numLong; ++i) {
+                const int L = s.misc[3 + i];
+                lf = L >= j && L < lf ? L : lf;
+            }
+            s.sorted[j] = lf;
+        }
+    }
+    __syncthreads();
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
         if (on && j < nS)
-            s.segCount[j] = nS > gap ? self_count_scan(s, x4, segEnd, nS, table, gap, radius, cell, lenMax, lenCap,
-                                                       longs, scaleMax, chain, j, cj[t])
+            s.segCount[j] = nS > gap ? self_count_scan(s, x4, segEnd, s.sorted, nS, table, gap, radius, cell, lenCap,
+                                                       scaleMax, chain, j, cj[t])
                                      : 0;
     }
     __syncthreads();
