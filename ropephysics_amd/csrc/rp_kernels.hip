@@ -99,7 +99,7 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
     real4* I4 = reinterpret_cast<real4*>(smem + L.I);
     real4* lamS = reinterpret_cast<real4*>(smem + L.lamS);
     real4* lamB = reinterpret_cast<real4*>(smem + L.lamB);
-    real* cont = reinterpret_cast<real*>(smem + L.cont);
+    real4* cont4 = reinterpret_cast<real4*>(smem + L.cont);
     real* prims = reinterpret_cast<real*>(smem + L.prims);
     SelfLds sl;
     if (ri.self) sl = self_carve(smem + L.self, nS, ri.table, ri.selfCap);
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
             for (int k = tid; k < nB; k += nt) lamB[k] = make_real4(real(0.), real(0.), real(0.), real(0.));
             if (nPrim > 0)
                 for (int i = tid; i < nP; i += nt)
-                    world_contacts_begin(cont + i * nPrim * kContactFloats, prims, nPrim, f4v(x4[i]), x4[i].w,
+                    world_contacts_begin(cont4 + i, nP, prims, nPrim, f4v(x4[i]), x4[i].w,
                                          ri.radius, ri.margin, fscale, regen);
             if (ri.self) {
                 if (regen) {
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
                         v3 x = f4v(x4[i]);
                         const real w = x4[i].w;
                         if (w <= real(0.0)) continue;
-                        world_contacts_project(cont + i * nPrim * kContactFloats, nPrim, x, f4v(xp4[i]), w);
+                        world_contacts_project(cont4 + i, nP, nPrim, x, f4v(xp4[i]), w);
                         x4[i] = vf4(x, w);
                     }
                     __syncthreads();
@@ -356,7 +356,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     real4* f4 = reinterpret_cast<real4*>(base + L.f);
     real4* kv4 = reinterpret_cast<real4*>(base + L.kv);
     real4* tq4 = reinterpret_cast<real4*>(base + L.tq);
-    real* cont = reinterpret_cast<real*>(base + L.cont);
+    real4* cont4 = reinterpret_cast<real4*>(base + L.cont);
     real* prims = reinterpret_cast<real*>(base + L.prims);
     real4* xp4 = reinterpret_cast<real4*>(base + L.xp);
     SelfLds sl;
@@ -479,7 +479,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                 for (int t = 0; t < TP; ++t) {
                     const int i = l + G * t;
                     if (live && i < nP)
-                        world_contacts_begin(cont + i * nPrim * kContactFloats, prims, nPrim, f4v(x4[i]), wP[t],
+                        world_contacts_begin(cont4 + i, nP, prims, nPrim, f4v(x4[i]), wP[t],
                                              ri.radius, ri.margin, fscale, regen);
                 }
             }
@@ -552,7 +552,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                             const int i = l + G * t;
                             if (live && i < nP && wP[t] > real(0.0)) {
                                 v3 x = f4v(x4[i]);
-                                world_contacts_project(cont + i * nPrim * kContactFloats, nPrim, x, xpv[t], wP[t]);
+                                world_contacts_project(cont4 + i, nP, nPrim, x, xpv[t], wP[t]);
                                 x4[i] = vf4(x, wP[t]);
                             }
                         }

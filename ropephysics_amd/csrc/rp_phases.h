@@ -118,44 +118,56 @@ __device__ __forceinline__ bool prim_far(const real* s, v3 x, real radius, real 
     return norm2(sub(x, c)) > lim * lim;
 }
 
-// World contacts of one particle: regenerate (or just reset) its P slots.
-__device__ __forceinline__ void world_contacts_begin(real* slots, const real* prims, int nPrim, v3 x, real w,
-                                                     real radius, real margin, real fscale, bool regen) {
+// World contacts of one particle: its slot for primitive k is two real4
+// words, c4[2k * stride] = (normal, offset) and c4[(2k + 1) * stride] =
+// (friction, lamN, applied, active), `stride` being the rod's particle count:
+// lanes holding consecutive particles touch consecutive 16-byte words.
+// Regenerate (or just reset) the particle's slots.
+__device__ __forceinline__ void world_contacts_begin(real4* c4, int stride, const real* prims, int nPrim, v3 x,
+                                                     real w, real radius, real margin, real fscale, bool regen) {
     for (int k = 0; k < nPrim; ++k) {
-        real* s = slots + k * kContactFloats;
-        s[5] = real(0.0);  // lamN
-        s[6] = real(0.0);  // applied
-        if (!regen) continue;
+        real4* A = c4 + 2 * k * stride;
+        real4* B = A + stride;
+        if (!regen) {
+            real4 b = *B;
+            b.y = real(0.0);  // lamN
+            b.z = real(0.0);  // applied
+            *B = b;
+            continue;
+        }
         wcontact c;
         if (prim_far(prims + k * kPrimFloats, x, radius, margin))
             c.active = 0;
         else
             world_contact_generate(c, load_prim(prims + k * kPrimFloats), x, w, radius, margin, fscale);
-        s[7] = int_slot(c.active);
         if (c.active) {
-            s[0] = c.n.x;
-            s[1] = c.n.y;
-            s[2] = c.n.z;
-            s[3] = c.offset;
-            s[4] = c.friction;
+            *A = make_real4(c.n.x, c.n.y, c.n.z, c.offset);
+            *B = make_real4(c.friction, real(0.0), real(0.0), int_slot(1));
+        } else {
+            // An inactive slot's other fields are never read.
+            *B = make_real4(real(0.0), real(0.0), real(0.0), int_slot(0));
         }
     }
 }
 
-__device__ __forceinline__ void world_contacts_project(real* slots, int nPrim, v3& x, v3 xPrev, real w) {
+__device__ __forceinline__ void world_contacts_project(real4* c4, int stride, int nPrim, v3& x, v3 xPrev, real w) {
     for (int k = 0; k < nPrim; ++k) {
-        real* s = slots + k * kContactFloats;
+        const real4* A = c4 + 2 * k * stride;
+        real4* B = c4 + (2 * k + 1) * stride;
+        real4 b = *B;
+        if (!slot_int(b.w)) continue;
+        const real4 a = *A;
         wcontact c;
-        c.active = slot_int(s[7]);
-        if (!c.active) continue;
-        c.n = mk(s[0], s[1], s[2]);
-        c.offset = s[3];
-        c.friction = s[4];
-        c.lamN = s[5];
-        c.applied = s[6];
+        c.active = 1;
+        c.n = mk(a.x, a.y, a.z);
+        c.offset = a.w;
+        c.friction = b.x;
+        c.lamN = b.y;
+        c.applied = b.z;
         world_contact_project(c, x, xPrev, w);
-        s[5] = c.lamN;
-        s[6] = c.applied;
+        b.y = c.lamN;
+        b.z = c.applied;
+        *B = b;
     }
 }
 
