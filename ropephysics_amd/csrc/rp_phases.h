@@ -418,13 +418,12 @@ __device__ __forceinline__ int row_bucket(int cx, int dx, unsigned hzy, unsigned
 // a segment lies within half its length of its centre, so the list is
 // unchanged.
 __device__ inline void self_fill_rows(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
-                                      int gap, int cap, real radius, real friction, real cell, int j, real4 cj4,
-                                      int slotBase) {
+                                      int gap, int cap, real radius, real friction, real cell, int j, int j0, int j1,
+                                      real4 cj4, int slotBase) {
     const v3 cj = f4v(cj4);
     const real reachJ = real(0.5) * cj4.w + real(2.0) * radius;
     const real scaleJ = abs_max3(cj);
     const int cx = cell_coord(cj.x, cell), cy = cell_coord(cj.y, cell), cz = cell_coord(cj.z, cell);
-    const int j0 = segEnd[2 * j], j1 = segEnd[2 * j + 1];
     const v3 a0 = f4v(x4[j0]), a1 = f4v(x4[j1]);
     const unsigned tbl = static_cast<unsigned>(table);
     // Usually every lane group of the wave has a power-of-two table: then the
@@ -494,7 +493,7 @@ __device__ inline void self_fill_rows(const SelfLds& s, const real4* x4, const i
 // point of segment k lies within its half-length of its centre.
 __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const int* segEnd, const int* nextLong,
                                       int nS, int table, int gap, real radius, real cell, real lenCap, real scaleMax,
-                                      bool chain, int j, real4 cj4) {
+                                      bool chain, int j, int j0, int j1, real4 cj4) {
     const v3 cj = f4v(cj4);
     const real reachJ = real(0.5) * cj4.w + real(2.0) * radius;
     const real eps = real(1e-6) * (real(1.0) + scaleMax);
@@ -514,7 +513,6 @@ __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const i
             k = lf > k ? min(next, lf) : k + 1;
             continue;
         }
-        const int j0 = segEnd[2 * j], j1 = segEnd[2 * j + 1];
         const v3 a0 = f4v(x4[j0]), a1 = f4v(x4[j1]);
         const v3 dj = sub(a1, a0);
         const real a = norm2(dj);
@@ -548,10 +546,27 @@ __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const i
 // Broadphase and contact generation by the G lanes of a rod's lane group; lane
 // l owns segments j = l + G t, t < TP (the host guarantees nS <= TP G), and
 // keeps their centres in registers; s.cen holds them in segment order.
+// Inclusive max (kMin: min) of v over the lanes of this lane's group in its
+// wavefront, in wavefront order; the group's last lane in the wavefront ends
+// with the value of the whole stretch.
+template <bool kMin>
+__device__ __forceinline__ rbits group_wave_extreme(rbits v, int l) {
+    const int lane = int(threadIdx.x);
+    for (int off = 1; off < 64; off <<= 1) {
+        const rbits u = __shfl_up(v, off, 64);
+        if (l >= off && (lane & 63) >= off) v = kMin ? (u < v ? u : v) : (u > v ? u : v);
+    }
+    return v;
+}
+
+// Broadphase and contact generation by the G lanes of a rod's lane group; lane
+// l owns segments j = l + G t, t < TP (the host guarantees nS <= TP G), with
+// particles e0[t], e1[t], and keeps their centres in registers; s.cen holds
+// them in segment order.
 template <int TP>
-__device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
-                                        int gap, int cap, real radius, real friction, bool chain, bool on, int l,
-                                        int G) {
+__device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const int* segEnd, const int* e0,
+                                        const int* e1, int nS, int table, int gap, int cap, real radius,
+                                        real friction, bool chain, bool on, int l, int G) {
     real4 cj[TP];
     if (on && l == 0) {
         s.misc[2] = 0;
@@ -559,21 +574,33 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
         s.rmisc[2] = 0;
         s.rmisc[3] = real_bits(real(1e30));
     }
-    __syncthreads();
+    // The longest and shortest segment and the largest coordinate: per lane,
+    // per wavefront stretch of the group, then one atomic per stretch.
+    rbits lenHi = 0, lenLo = real_bits(real(1e30)), scaleHi = 0;
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
         cj[t] = make_real4(real(0.), real(0.), real(0.), real(0.));
         if (on && j < nS) {
-            const v3 a = f4v(x4[segEnd[2 * j]]), b = f4v(x4[segEnd[2 * j + 1]]);
+            const v3 a = f4v(x4[e0[t]]), b = f4v(x4[e1[t]]);
             const real len = norm(sub(b, a));
             const v3 c = scale(add(a, b), real(0.5));
             cj[t] = vf4(c, len);
             s.cen[j] = cj[t];
-            atomicMax(&s.rmisc[1], real_bits(len));
-            atomicMax(&s.rmisc[2], real_bits(abs_max3(c)));
-            atomicMin(&s.rmisc[3], real_bits(len));
+            const rbits lb = real_bits(len), sb = real_bits(abs_max3(c));
+            lenHi = lb > lenHi ? lb : lenHi;
+            lenLo = lb < lenLo ? lb : lenLo;
+            scaleHi = sb > scaleHi ? sb : scaleHi;
         }
+    }
+    lenHi = group_wave_extreme<false>(lenHi, l);
+    scaleHi = group_wave_extreme<false>(scaleHi, l);
+    lenLo = group_wave_extreme<true>(lenLo, l);
+    __syncthreads();
+    if (on && ((threadIdx.x & 63) == 63 || l == G - 1)) {
+        atomicMax(&s.rmisc[1], lenHi);
+        atomicMax(&s.rmisc[2], scaleHi);
+        atomicMin(&s.rmisc[3], lenLo);
     }
     __syncthreads();
     const real lenMax = on ? bits_real(s.rmisc[1]) : real(0.0);
@@ -620,7 +647,7 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
         const int j = l + G * t;
         if (on && j < nS)
             s.segCount[j] = nS > gap ? self_count_scan(s, x4, segEnd, s.sorted, nS, table, gap, radius, cell, lenCap,
-                                                       scaleMax, chain, j, cj[t])
+                                                       scaleMax, chain, j, e0[t], e1[t], cj[t])
                                      : 0;
     }
     __syncthreads();
@@ -668,7 +695,8 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
         if (build && j < nS && s.segCount[j] > 0)
-            self_fill_rows(s, x4, segEnd, nS, table, gap, cap, radius, friction, cell, j, cj[t], s.segStart[j]);
+            self_fill_rows(s, x4, segEnd, nS, table, gap, cap, radius, friction, cell, j, e0[t], e1[t], cj[t],
+                           s.segStart[j]);
     }
 }
 
