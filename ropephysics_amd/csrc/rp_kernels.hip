@@ -445,15 +445,6 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     const real cscale = active ? d.complianceScale[rod] : real(1.0);
     const real fscale = active ? d.frictionScale[rod] : real(1.0);
     const int* segEnd = d.segEnd + 2 * ri.sStart;
-    // Particles of the lane's own segments, for the self-collision broadphase.
-    int sE0[TP], sE1[TP];
-#pragma unroll
-    for (int t = 0; t < TP; ++t) {
-        const int j = l + G * t;
-        const bool mine = SELF && ri.self && j < nS;
-        sE0[t] = mine ? segEnd[2 * j] : 0;
-        sE1[t] = mine ? segEnd[2 * j + 1] : 0;
-    }
     if (SELF && ri.self && l == 0) {
         sl.misc[0] = 0;
         sl.misc[1] = 0;
@@ -494,7 +485,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
             }
             if (anySelf) {
                 if (regen) {
-                    self_regen_group<TP>(sl, x4, segEnd, sE0, sE1, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
+                    self_regen_group<TP>(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
                                          ri.selfFriction * fscale, ri.selfChain != 0, ri.self != 0 && live, l, G);
                 } else if (ri.self && l == 0 && live) {
                     self_reset(sl);

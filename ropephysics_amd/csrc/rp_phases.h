@@ -560,13 +560,12 @@ __device__ __forceinline__ rbits group_wave_extreme(rbits v, int l) {
 }
 
 // Broadphase and contact generation by the G lanes of a rod's lane group; lane
-// l owns segments j = l + G t, t < TP (the host guarantees nS <= TP G), with
-// particles e0[t], e1[t], and keeps their centres in registers; s.cen holds
-// them in segment order.
+// l owns segments j = l + G t, t < TP (the host guarantees nS <= TP G), and
+// keeps their centres in registers; s.cen holds them in segment order.
 template <int TP>
-__device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const int* segEnd, const int* e0,
-                                        const int* e1, int nS, int table, int gap, int cap, real radius,
-                                        real friction, bool chain, bool on, int l, int G) {
+__device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
+                                        int gap, int cap, real radius, real friction, bool chain, bool on, int l,
+                                        int G) {
     real4 cj[TP];
     if (on && l == 0) {
         s.misc[2] = 0;
@@ -582,7 +581,7 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
         const int j = l + G * t;
         cj[t] = make_real4(real(0.), real(0.), real(0.), real(0.));
         if (on && j < nS) {
-            const v3 a = f4v(x4[e0[t]]), b = f4v(x4[e1[t]]);
+            const v3 a = f4v(x4[segEnd[2 * j]]), b = f4v(x4[segEnd[2 * j + 1]]);
             const real len = norm(sub(b, a));
             const v3 c = scale(add(a, b), real(0.5));
             cj[t] = vf4(c, len);
@@ -647,7 +646,7 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
         const int j = l + G * t;
         if (on && j < nS)
             s.segCount[j] = nS > gap ? self_count_scan(s, x4, segEnd, s.sorted, nS, table, gap, radius, cell, lenCap,
-                                                       scaleMax, chain, j, e0[t], e1[t], cj[t])
+                                                       scaleMax, chain, j, segEnd[2 * j], segEnd[2 * j + 1], cj[t])
                                      : 0;
     }
     __syncthreads();
@@ -695,8 +694,8 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
         if (build && j < nS && s.segCount[j] > 0)
-            self_fill_rows(s, x4, segEnd, nS, table, gap, cap, radius, friction, cell, j, e0[t], e1[t], cj[t],
-                           s.segStart[j]);
+            self_fill_rows(s, x4, segEnd, nS, table, gap, cap, radius, friction, cell, j, segEnd[2 * j],
+                           segEnd[2 * j + 1], cj[t], s.segStart[j]);
     }
 }
 
