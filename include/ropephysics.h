@@ -374,6 +374,7 @@ typedef struct rp_server_desc {
     int tile_config;             /* 0: tp3-tc1-w4 for rods without self-collision, tp1-tc1-w8 for
                                     rods with it; else this shape for both */
     int steps_per_launch;        /* steps per rp_server_step (0: 64) */
+    int self_tile_config;        /* when positive: the shape for rods with self-collision */
 } rp_server_desc;
 
 /* A request's initial state and loads, packed rod after rod like the

@@ -184,13 +184,15 @@ int rp_server_create(const rp_server_desc* desc, void** out_server) {
     if (desc->hbm_budget_bytes <= 0) return fail(RP_ERR_INVALID, "hbm_budget_bytes must be positive");
     if (desc->params.dt <= 0.0f || desc->params.substeps < 1 || desc->params.iterations < 1)
         return fail(RP_ERR_INVALID, "dt, substeps and iterations must be positive");
-    if (desc->tile_config < 0 || desc->tile_config > num_tile_configs())
+    if (desc->tile_config < 0 || desc->tile_config > num_tile_configs() || desc->self_tile_config < 0 ||
+        desc->self_tile_config > num_tile_configs())
         return fail(RP_ERR_INVALID, "unknown tile configuration");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(RP_ERR_DEVICE, "no usable HIP device");
     auto s = std::make_unique<rp_server_impl>();
     s->params = desc->params;
     if (desc->tile_config > 0) s->configLean = s->configSelf = desc->tile_config;
+    if (desc->self_tile_config > 0) s->configSelf = desc->self_tile_config;
     if (desc->steps_per_launch > 0) s->stepsPerLaunch = desc->steps_per_launch;
     int rc = lds_limits(s->lim);
     if (rc) return rc;
