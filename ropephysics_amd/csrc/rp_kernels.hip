@@ -463,6 +463,9 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     __syncthreads();
 
     for (int step = 0; step < sa.steps; ++step) {
+        // Addresses that depend on the rod (its path and observer records) are
+        // formed where they are used, not held in registers across the loop.
+        asm volatile("" : "+v"(rod));
         const bool live = active && clk < limit;
         if (driver && live) kv4[drivenP] = vf4(path_velocity(d.paths[rod], clk), real(0.0));
         for (int sub = 0; sub < sa.substeps; ++sub) {
@@ -584,17 +587,20 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     }
 
     __syncthreads();
+    // The write-back addresses are formed here, after the loop.
+    int p0e = p0, f0e = f0;
+    asm volatile("" : "+v"(p0e), "+v"(f0e), "+v"(rod));
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int i = l + G * t;
         if (active && i < nP) {
-            st3(d.x, p0 + i, f4v(x4[i]));
-            st3(d.v, p0 + i, vel[t]);
+            st3(d.x, p0e + i, f4v(x4[i]));
+            st3(d.v, p0e + i, vel[t]);
         }
         const int j = l + G * t;
         if (active && j < nF) {
-            st4q(d.q, f0 + j, f4q(q4s[j]));
-            st3(d.omega, f0 + j, om[t]);
+            st4q(d.q, f0e + j, f4q(q4s[j]));
+            st3(d.omega, f0e + j, om[t]);
         }
     }
     if (SELF && ri.self && l == 0) d.selfOverflow[rod] += sl.misc[1];
