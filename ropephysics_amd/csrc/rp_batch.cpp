@@ -353,15 +353,27 @@ int create(const typename Prec<Real>::BatchDesc* desc, void** out_batch) {
     if ((rc = lds_limits(lim))) return rc;
     b->ldsLimit = lim.perWorkgroup;
     b->cuLds = lim.perCU;
+    // A rod runs TILED when it fits the LDS budget and some tile shape built
+    // for this precision has lanes enough for it.
     bool allTileable = true;
     for (int r = 0; r < R; ++r) {
         if (b->tileable[r] && size_t(rod_tiled_lds(b->rods[r])) > b->ldsLimit) b->tileable[r] = 0;
+        if (b->tileable[r]) {
+            bool shape = false;
+            for (int id = 1; id <= num_tile_configs() && !shape; ++id) {
+                const TileConfig& c = tile_config(id);
+                shape = tile_config_available<Real>(id) &&
+                        group_lanes(b->rods[r], b->colStart, c.tp, c.tc) <= 64 * c.waves;
+            }
+            b->tileable[r] = shape ? 1 : 0;
+        }
         allTileable = allTileable && b->tileable[r];
     }
     int strategy = desc->strategy;
     if (strategy == RP_STRATEGY_AUTO) strategy = allTileable ? RP_STRATEGY_TILED : RP_STRATEGY_PLAIN;
     if (strategy == RP_STRATEGY_TILED && !allTileable)
-        return fail(RP_ERR_CAPACITY, "a rod does not fit the TILED strategy (more than 2 colours or LDS budget)");
+        return fail(RP_ERR_CAPACITY,
+                    "a rod does not fit the TILED strategy (more than 2 colours, LDS budget or lanes)");
     if (strategy == RP_STRATEGY_PLAIN && size_t(rec.maxPlainLds) > b->ldsLimit)
         return fail(RP_ERR_CAPACITY, "a rod needs " + std::to_string(rec.maxPlainLds) +
                                          " bytes of LDS; the device has " + std::to_string(b->ldsLimit) +
@@ -639,16 +651,16 @@ int rp_device_query(rp_device_info* out) {
     out->total_global_mem = prop.totalGlobalMem;
     out->clock_khz = prop.clockRate;
     // Longest clamped chain (n + 1 particles, n + 1 frames, n stretch and n
-    // bend constraints, two colours of about n / 2 + 1) some FP32
-    // configuration takes.
+    // bend constraints, each family in two colours of at most ceil(n / 2))
+    // some FP32 configuration takes.
     int best = 0;
     for (int n2 = 4; n2 <= 4096; ++n2) {
         const int lds = Prec<float>::tiled_lds_bytes(n2 + 1, n2 + 1, n2, 0, 0, 1, 0);
         if (size_t(lds) > prop.sharedMemPerBlock) break;
         for (int id = 1; id <= num_tile_configs(); ++id) {
             const TileConfig& c = tile_config(id);
-            const int lanes = std::max((n2 + 1 + c.tp - 1) / c.tp, (n2 / 2 + 1 + c.tc - 1) / c.tc);
-            if (lanes <= 64 * c.waves) best = n2;
+            const int lanes = std::max((n2 + 1 + c.tp - 1) / c.tp, ((n2 + 1) / 2 + c.tc - 1) / c.tc);
+            if (tile_config_available<float>(id) && lanes <= 64 * c.waves) best = n2;
         }
     }
     out->max_tiled_segments = best;

@@ -352,6 +352,29 @@ def test_full_size_uniform_batch_sampled_rods_bitwise():
         assert_bitwise(compare_bitwise(got, oracle_state(o)))
 
 
+def test_extreme_rod_sizes():
+    """The smallest rods (1, 2 and 3 segments) and the longest clamped chain
+    the TILED path takes (rp_device_query's max_tiled_segments, as the
+    reference reports maxFusedSegments), packed together: bitwise equal to the
+    oracle under both strategies. One segment more does not fit any tile shape:
+    AUTO runs it PLAIN (still bitwise equal) and TILED refuses it at creation."""
+    from ropephysics_amd import _abi
+
+    nmax = rp.device_info()["max_tiled_segments"]
+    assert nmax >= 255
+    rods = [benchmark_rod(1), benchmark_rod(2), benchmark_rod(3), benchmark_rod(nmax)]
+    for strategy in STRATEGIES:
+        diffs, _, b = run_pair(rods, None, benchmark_params(), 20, strategy, checkpoints=[1, 20])
+        assert_bitwise(diffs)
+    big = [benchmark_rod(nmax + 1)]
+    diffs, _, b = run_pair(big, None, benchmark_params(), 5, "auto")
+    assert b.counters()["strategy"] == "plain"
+    assert_bitwise(diffs)
+    with pytest.raises(rp.RopePhysicsError) as e:
+        rp.Batch(big, strategy="tiled")
+    assert e.value.code == _abi.RP_ERR_CAPACITY
+
+
 def test_invalid_inputs_are_rejected():
     rod = benchmark_rod(8)
     b = rp.Batch(rod, 2)
