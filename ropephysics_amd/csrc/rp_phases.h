@@ -718,52 +718,113 @@ __device__ inline void self_reset(const SelfLds& s) {
     }
 }
 
-// Four-particle contacts, sequentially in pool order (= the CPU's list order).
-// xp4 holds substep-start positions.
-__device__ inline void self_project(const SelfLds& s, real4* x4, const real4* xp4) {
-    const int live = s.misc[0];
-    for (int ci = 0; ci < live; ++ci) {
-        real* c = s.pool + ci * kSelfFloats;
-        int idx[4];
-        real wt[4], im[4];
+// One four-particle contact: unilateral normal projection, then Coulomb
+// friction with the per-substep budget; xp4 holds substep-start positions.
+// Positions are read, corrected and written through LDS in the order the CPU
+// applies them (a particle may appear twice).
+__device__ inline void self_project_one_aliased(const SelfLds& s, real4* x4, const real4* xp4, int ci) {
+    real* c = s.pool + ci * kSelfFloats;
+    int idx[4];
+    real wt[4], im[4];
+    for (int m = 0; m < 4; ++m) {
+        idx[m] = slot_int(c[m]);
+        wt[m] = c[4 + m];
+        im[m] = x4[idx[m]].w;
+    }
+    const v3 n = mk(c[8], c[9], c[10]);
+    real wEff = real(0.0);
+    for (int m = 0; m < 4; ++m) wEff += wt[m] * wt[m] * im[m];
+    if (wEff <= real(0.0)) return;
+    real C = -c[11];
+    for (int m = 0; m < 4; ++m) C += wt[m] * dot(f4v(x4[idx[m]]), n);
+    real dl = -C / wEff;
+    const real lamN0 = c[13];
+    const real cl = max0(lamN0 + dl);
+    dl = cl - lamN0;
+    c[13] = cl;
+    if (dl != real(0.0))
         for (int m = 0; m < 4; ++m) {
-            idx[m] = slot_int(c[m]);
-            wt[m] = c[4 + m];
-            im[m] = x4[idx[m]].w;
-        }
-        const v3 n = mk(c[8], c[9], c[10]);
-        real wEff = real(0.0);
-        for (int m = 0; m < 4; ++m) wEff += wt[m] * wt[m] * im[m];
-        if (wEff <= real(0.0)) continue;
-        real C = -c[11];
-        for (int m = 0; m < 4; ++m) C += wt[m] * dot(f4v(x4[idx[m]]), n);
-        real dl = -C / wEff;
-        const real lamN0 = c[13];
-        const real cl = max0(lamN0 + dl);
-        dl = cl - lamN0;
-        c[13] = cl;
-        if (dl != real(0.0))
-            for (int m = 0; m < 4; ++m) {
-                const v3 xm = add(f4v(x4[idx[m]]), scale(n, im[m] * wt[m] * dl));
-                x4[idx[m]] = vf4(xm, im[m]);
-            }
-        const real friction = c[12];
-        if (friction <= real(0.0) || cl <= real(0.0)) continue;
-        v3 dp = mk(real(0.0), real(0.0), real(0.0));
-        for (int m = 0; m < 4; ++m) dp = add(dp, scale(sub(f4v(x4[idx[m]]), f4v(xp4[idx[m]])), wt[m]));
-        const v3 tang = sub(dp, scale(n, dot(dp, n)));
-        const real slide = norm(tang);
-        if (slide < real(1e-15)) continue;
-        const real budget = friction * cl * wEff - c[14];
-        if (budget <= real(0.0)) continue;
-        const real capped = min_ab(slide, budget);
-        c[14] = c[14] + capped;
-        const v3 corr = scale(tang, -capped / slide);
-        for (int m = 0; m < 4; ++m) {
-            const v3 xm = add(f4v(x4[idx[m]]), scale(corr, im[m] * wt[m] / wEff));
+            const v3 xm = add(f4v(x4[idx[m]]), scale(n, im[m] * wt[m] * dl));
             x4[idx[m]] = vf4(xm, im[m]);
         }
+    const real friction = c[12];
+    if (friction <= real(0.0) || cl <= real(0.0)) return;
+    v3 dp = mk(real(0.0), real(0.0), real(0.0));
+    for (int m = 0; m < 4; ++m) dp = add(dp, scale(sub(f4v(x4[idx[m]]), f4v(xp4[idx[m]])), wt[m]));
+    const v3 tang = sub(dp, scale(n, dot(dp, n)));
+    const real slide = norm(tang);
+    if (slide < real(1e-15)) return;
+    const real budget = friction * cl * wEff - c[14];
+    if (budget <= real(0.0)) return;
+    const real capped = min_ab(slide, budget);
+    c[14] = c[14] + capped;
+    const v3 corr = scale(tang, -capped / slide);
+    for (int m = 0; m < 4; ++m) {
+        const v3 xm = add(f4v(x4[idx[m]]), scale(corr, im[m] * wt[m] / wEff));
+        x4[idx[m]] = vf4(xm, im[m]);
     }
+}
+
+// The same contact when its four particles are distinct (always so for two
+// segments of a chain at least two apart): each particle's position is read
+// once, receives the same corrections in the same order in registers, and is
+// written once.
+__device__ __forceinline__ void self_project_one(const SelfLds& s, real4* x4, const real4* xp4, int ci) {
+    real* c = s.pool + ci * kSelfFloats;
+    const real4* c4 = reinterpret_cast<const real4*>(c);
+    const real4 r0 = c4[0], r1 = c4[1], r2 = c4[2], r3 = c4[3];
+    const int idx[4] = {slot_int(r0.x), slot_int(r0.y), slot_int(r0.z), slot_int(r0.w)};
+    if (idx[0] == idx[1] || idx[0] == idx[2] || idx[0] == idx[3] || idx[1] == idx[2] || idx[1] == idx[3] ||
+        idx[2] == idx[3]) {
+        self_project_one_aliased(s, x4, xp4, ci);
+        return;
+    }
+    const real wt[4] = {r1.x, r1.y, r1.z, r1.w};
+    const v3 n = mk(r2.x, r2.y, r2.z);
+    real4 xv[4];
+    for (int m = 0; m < 4; ++m) xv[m] = x4[idx[m]];
+    real wEff = real(0.0);
+    for (int m = 0; m < 4; ++m) wEff += wt[m] * wt[m] * xv[m].w;
+    if (wEff <= real(0.0)) return;
+    real C = -r2.w;
+    for (int m = 0; m < 4; ++m) C += wt[m] * dot(f4v(xv[m]), n);
+    real dl = -C / wEff;
+    const real lamN0 = r3.y;
+    const real cl = max0(lamN0 + dl);
+    dl = cl - lamN0;
+    c[13] = cl;
+    v3 x[4];
+    for (int m = 0; m < 4; ++m) x[m] = f4v(xv[m]);
+    if (dl != real(0.0))
+        for (int m = 0; m < 4; ++m) x[m] = add(x[m], scale(n, xv[m].w * wt[m] * dl));
+    const real friction = r3.x;
+    bool done = friction <= real(0.0) || cl <= real(0.0);
+    v3 tang = mk(real(0.0), real(0.0), real(0.0));
+    real slide = real(0.0), budget = real(0.0);
+    if (!done) {
+        v3 dp = mk(real(0.0), real(0.0), real(0.0));
+        for (int m = 0; m < 4; ++m) dp = add(dp, scale(sub(x[m], f4v(xp4[idx[m]])), wt[m]));
+        tang = sub(dp, scale(n, dot(dp, n)));
+        slide = norm(tang);
+        done = slide < real(1e-15);
+    }
+    if (!done) {
+        budget = friction * cl * wEff - r3.z;
+        done = budget <= real(0.0);
+    }
+    if (!done) {
+        const real capped = min_ab(slide, budget);
+        c[14] = r3.z + capped;
+        const v3 corr = scale(tang, -capped / slide);
+        for (int m = 0; m < 4; ++m) x[m] = add(x[m], scale(corr, xv[m].w * wt[m] / wEff));
+    }
+    for (int m = 0; m < 4; ++m) x4[idx[m]] = vf4(x[m], xv[m].w);
+}
+
+// Four-particle contacts, sequentially in pool order (= the CPU's list order).
+__device__ inline void self_project(const SelfLds& s, real4* x4, const real4* xp4) {
+    const int live = s.misc[0];
+    for (int ci = 0; ci < live; ++ci) self_project_one(s, x4, xp4, ci);
 }
 
 }  // namespace RP_PREC
