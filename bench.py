@@ -16,7 +16,8 @@ solver's precision on the GPU), its routed verdicts checked against the
 reference CPU path in colour order; the uniform batch of 4096 rods x 64
 segments over a floor plane with per-rod stiffness and friction, 1000 steps,
 TILED and PLAIN; the hold/slide sweep of 3600 draped cables (8 s) in FP32 and
-FP64 with its verdicts spot-checked against the reference CPU path; and the
+FP64 with its verdicts spot-checked against the reference CPU path; the
+cross-entropy harness-learning loop (10 iterations of 1024 rollouts); and the
 stream (requests of 1-256 rods of 32-256 segments, 0.5-8 s horizons, random
 contact shapes, self-collision in half) through the continuous-batching
 server: its capacity with every request arriving at once, then p50/p99 latency
@@ -315,6 +316,65 @@ def uniform_leg(steps: int, warmup: int, no_plain: bool) -> dict:
     return out
 
 
+# ------------------------------------------------------------------ harness learning (application)
+
+def cem_leg(iterations: int, population: int, cpu: bool) -> dict:
+    """The cross-entropy harness-learning loop (runHarnessLearning,
+    application_cases.cpp:353-486) on the routing task: `iterations` batches of
+    `population` rollouts of 6 s, then the learned and the hand-written motion
+    on the same fresh cables; with `cpu`, the softest and stiffest fresh cable
+    under each motion rerun by the reference CPU path (FP64, colour order) and
+    the verdicts compared."""
+    from ropephysics_amd.learning import HarnessLearner
+
+    t0 = time.perf_counter()
+    learner = HarnessLearner(population=population)
+    res = learner.run(iterations)
+    wall = time.perf_counter() - t0
+    out = {"population": population, "iterations": [
+        {"success_rate": i.success_rate, "mean_score": i.mean_score, "seconds": i.seconds}
+        for i in res.iterations], "learn_seconds": res.learn_seconds, "wall_seconds": wall,
+        "learned_rate": res.learned_rate, "hand_rate": res.hand_rate,
+        "learned_waypoints": res.mean.tolist()}
+    if cpu:
+        import oracle
+
+        kind = "ref64" if oracle.available("ref64") else "oracle"
+        task = learner.task
+        ys = res.fresh["youngs"]
+        picks = [int(np.argmin(ys)), int(np.argmax(ys))]
+        agree = compared = 0
+        for r in picks:
+            n = int(res.fresh["segments"][r])
+            for name, wp in (("learned", res.mean), ("hand", task.hand_written_waypoints())):
+                rod = task.build(n)
+                o = oracle.Rod.straight(2, 1.0, rod.material.as_tuple(), precision=64, kind=kind)
+                arr = rod.arrays()
+                arr["sComp"] = arr["sComp"] / float(ys[r])
+                arr["bComp"] = arr["bComp"] / float(ys[r])
+                o.set(arr)
+                fr = float(res.fresh["friction"][r])
+                prims = [oracle.make_prim(pr.type, pr.a, pr.b, pr.normal, pr.half_extents, pr.rotation, pr.radius,
+                                          pr.friction * fr) for pr in task.world().primitives]
+                w = task.world()
+                world = oracle.World(prims=prims, self_collision=True, self_friction=w.self_friction * fr,
+                                     self_skip=w.self_skip, hash_table_size=w.hash_table_size)
+                path = task.path(n, wp)
+                p = task.params()
+                kv = np.array([[path.velocity(s_) for s_ in range(learner.steps)]])
+                oracle.step_many_driven([o], oracle.Params(dt=p.dt, substeps=p.substeps, iterations=p.iterations,
+                                                           gravity=p.gravity, coloured=True), [world],
+                                        learner.steps, 1, [path.particle], kv)
+                agree += int(task.evaluate(o.get()["x"])["routed"] == res.fresh[name][r])
+                compared += 1
+        out["cpu_cross_check"] = {"attempts": compared, "agree": agree, "kind": "reference" if kind == "ref64"
+                                  else "port"}
+    log(f"harness learning: {iterations} x {population} rollouts, {res.learn_seconds:.1f} s on the device; "
+        f"routed per iteration {[round(i.success_rate, 3) for i in res.iterations]}; fresh cables: learned "
+        f"{res.learned_rate:.3f}, hand-written {res.hand_rate:.3f}")
+    return out
+
+
 # ------------------------------------------------------------------ hold/slide sweep (verdicts)
 
 def drape_cpu_verdicts(w: scenes.Workload, sel, steps: int, threads: int) -> dict:
@@ -424,6 +484,8 @@ def main():
     ap.add_argument("--drape-steps", type=int, default=None, help="steps of the hold/slide sweep (default: the "
                                                                    "whole 8 s = 16000; 0: skip)")
     ap.add_argument("--drape-cpu-rods", type=int, default=32, help="cables of the CPU verdict spot check")
+    ap.add_argument("--cem-iterations", type=int, default=10, help="harness-learning iterations (0: skip)")
+    ap.add_argument("--cem-population", type=int, default=1024)
     ap.add_argument("--serve-requests", type=int, default=40, help="requests of the serving leg (0: skip)")
     ap.add_argument("--serve-load", type=float, default=0.8, help="Poisson rate as a fraction of capacity")
     ap.add_argument("--serve-budget-gib", type=float, default=4.0, help="server HBM budget")
@@ -470,6 +532,7 @@ def main():
     drape = None
     if args.drape_steps != 0:
         drape = drape_leg(args.drape_steps, W, 0 if args.no_cpu else args.drape_cpu_rods, threads)
+    cem = cem_leg(args.cem_iterations, args.cem_population, not args.no_cpu) if args.cem_iterations > 0 else None
     serving_out = serving_leg(args.serve_requests, args.serve_load, args.serve_budget_gib) \
         if args.serve_requests > 0 else None
 
@@ -512,6 +575,7 @@ def main():
         "routing_fp64": r64,
         "uniform": uniform,
         "drape": drape,
+        "harness_learning": cem,
         "serving": serving_out,
         "p50_latency_s": serving_out["poisson"]["p50_latency_s"] if serving_out else None,
         "p99_latency_s": serving_out["poisson"]["p99_latency_s"] if serving_out else None,

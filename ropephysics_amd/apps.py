@@ -154,6 +154,18 @@ class RoutingTask:
         return np.array([(0.10, -0.30), (a[0], a[1] - 0.06), (0.33, -0.03), (0.40, 0.0), (0.48, 0.05),
                          (b[0], b[1] + 0.06), (c[0] - 0.10, 0.0), (c[0] + 0.28, -0.03)], dtype=np.float64)
 
+    def hand_written_waypoints(self) -> np.ndarray:
+        """Straight at the targets, as HarnessTask::handWrittenPolicy
+        (src/apps/harness_routing.cpp:62-67) writes it: the base path's first
+        seven waypoints, then an end point on the clip's line 18 cm past it
+        (y = 0) instead of 28 cm past it. The cable is left slack and lies
+        several centimetres off peg A, so it is not routed. This is the
+        starting point of harness learning and the motion that the learned
+        one is compared with."""
+        w = self.base_waypoints()
+        w[-1] = (self.clip[0] + 0.18, 0.0)
+        return w
+
     def wrong_side_waypoints(self) -> np.ndarray:
         """The base path with its second waypoint above peg A instead of
         below: the cable ends on the wrong side of A, by centimetres."""
@@ -195,8 +207,35 @@ class RoutingTask:
             cy = p[1] + (q[1] - p[1]) * u
             miss = min(miss, abs(cy - self.clip[1]))
         through = miss < self.clip_half[1]
+
+        # Smooth score for learning, 3 when routed: each peg 1 when the cable
+        # lies against its correct side, fading to 0.5 further out, 0 on the
+        # wrong side; the clip 1 when threaded, fading with the miss distance
+        # (HarnessTask::evaluate's score, harness_routing.cpp:147-153).
+        def clamp01(v):
+            return max(0.0, min(1.0, v))
+
+        def peg_score(offset):
+            if offset <= 0:
+                return 0.0
+            return 1.0 - 0.5 * clamp01((offset - (touching + self.slack)) / 0.1)
+
+        half = self.clip_half[1]
+        score = peg_score(-a) + peg_score(b) + (1.0 if through else clamp01(1.0 - (miss - half) / 0.1))
         return {"below_a": bool(below_a), "above_b": bool(above_b), "through_clip": bool(through),
-                "routed": bool(below_a and above_b and through), "side_a": a, "side_b": b, "clip_miss": miss}
+                "routed": bool(below_a and above_b and through), "side_a": a, "side_b": b, "clip_miss": miss,
+                "score": score}
+
+    def clamp_reach(self, waypoints: np.ndarray) -> np.ndarray:
+        """Waypoints pulled back to the cable's reach from the connector
+        (toPolicy, application_cases.cpp:303-312)."""
+        w = np.array(waypoints, np.float64).reshape(-1, 2)
+        c = np.array(self.connector[:2], np.float64)
+        for k in range(len(w)):
+            reach = float(np.linalg.norm(w[k] - c))
+            if reach > self.cable_length:
+                w[k] = c + (w[k] - c) * (self.cable_length / reach)
+        return w
 
 
 # ------------------------------------------------------------------ cable hanging

@@ -448,6 +448,8 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     if (SELF && ri.self && l == 0) {
         sl.misc[0] = 0;
         sl.misc[1] = 0;
+        self_regen_reset(sl);
+        self_list_reset(sl, nS);
     }
     const v3 g = mk(sa.gx, sa.gy, sa.gz);
     const real h = sa.h;

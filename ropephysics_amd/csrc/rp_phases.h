@@ -216,7 +216,7 @@ struct SelfLds {
     int* segStart;   // nS
     int* cellStart;  // table + 1
     real* pool;     // cap * kSelfFloats
-    int* misc;       // [0] live, [1] overflow, [2] long segments, [3..6] the first four of them
+    int* misc;       // [0] live, [1] overflow, [2] long segments, [4..7] the first four of them (nS: none)
     rbits* rmisc;    // [0] cell size, [1] longest segment, [2] largest |coordinate| of a centre, [3] shortest
                      // segment, as bits (non-negative: ordered as integers)
     int* scan;       // kScanWaves, scratch of the group-parallel scans
@@ -360,7 +360,8 @@ __device__ inline void self_query(const SelfLds& s, const real4* x4, const int* 
 // group's lanes are consecutive -- when l >= off), then across the wavefronts
 // the group spans through `scratch`, one entry per wavefront. Returns the
 // total. Two barriers, whatever G, n or `on`.
-__device__ inline int group_scan(const int* in, int* out, int n, int* scratch, int l, int G, bool on) {
+template <bool kVote>
+__device__ inline int group_scan_t(const int* in, int* out, int n, int* scratch, int l, int G, bool on, bool& any) {
     const int chunk = on ? (n + G - 1) / G : 0;
     const int lo = on ? min(n, l * chunk) : 0;
     const int hi = on ? min(n, lo + chunk) : 0;
@@ -391,8 +392,23 @@ __device__ inline int group_scan(const int* in, int* out, int n, int* scratch, i
         out[i] = run;
         run += v;
     }
-    __syncthreads();
+    if (kVote)
+        any = __syncthreads_or(on && total > 0) != 0;
+    else
+        __syncthreads();
     return total;
+}
+
+__device__ inline int group_scan(const int* in, int* out, int n, int* scratch, int l, int G, bool on) {
+    bool any = false;
+    return group_scan_t<false>(in, out, n, scratch, l, G, on, any);
+}
+
+// The same, with the final barrier a vote: `any` tells whether some rod of the
+// workgroup has a positive total.
+__device__ inline int group_scan_vote(const int* in, int* out, int n, int* scratch, int l, int G, bool on,
+                                      bool& any) {
+    return group_scan_t<true>(in, out, n, scratch, l, G, on, any);
 }
 
 __device__ __forceinline__ real abs_max3(v3 a) { return r_max(r_abs(a.x), r_max(r_abs(a.y), r_abs(a.z))); }
@@ -485,14 +501,14 @@ __device__ inline void self_fill_rows(const SelfLds& s, const real4* x4, const i
 // segments k' while D stays above the cull radius at their bound:
 // floor((D - lim) / step) of them, lim being the cull radius at the bound
 // lenCap and step lenCap with a 1e-3 margin. No segment of such a stretch may
-// be longer than lenCap: nextLong[k] is the first segment at or after k that
-// is (nS if none), and a stretch ends there; from such a segment the scan
-// steps by one. Far parts of the rod cost one test per stretch of segments.
+// be longer than lenCap: the long segments are listed in misc[4..7] (when
+// `longs`: the rod has some and all are listed), the first one at or after k
+// ends the stretch, and from a long segment the scan steps by one. Far parts of the rod cost one test per stretch of segments.
 // Before the closest-point test, a pair is also dropped when segment j is
 // farther from k's centre than k's half-length plus 2r (same margins): every
 // point of segment k lies within its half-length of its centre.
-__device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const int* segEnd, const int* nextLong,
-                                      int nS, int table, int gap, real radius, real cell, real lenCap, real scaleMax,
+__device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const int* segEnd, bool longs, int nS,
+                                      int table, int gap, real radius, real cell, real lenCap, real scaleMax,
                                       bool chain, int j, int j0, int j1, real4 cj4) {
     const v3 cj = f4v(cj4);
     const real reachJ = real(0.5) * cj4.w + real(2.0) * radius;
@@ -507,7 +523,14 @@ __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const i
         const real D2 = norm2(sub(ck, cj));
         const real limK = real(1.001) * (reachJ + real(0.5) * ck4.w) + eps;
         if (D2 > limK * limK) {
-            const int lf = nextLong[k];
+            int lf = nS;
+            if (longs) {
+                const int4 L = *reinterpret_cast<const int4*>(s.misc + 4);
+                lf = L.x >= k ? min(lf, L.x) : lf;
+                lf = L.y >= k ? min(lf, L.y) : lf;
+                lf = L.z >= k ? min(lf, L.z) : lf;
+                lf = L.w >= k ? min(lf, L.w) : lf;
+            }
             const float sk = (__builtin_amdgcn_sqrtf(float(D2)) - limCap) * invCap;
             const int next = k + 1 + (sk > 0.0f ? int(fminf(sk, float(nS))) : 0);
             k = lf > k ? min(next, lf) : k + 1;
@@ -528,7 +551,8 @@ __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const i
                 const unsigned tbl = static_cast<unsigned>(table);
                 const bool pow2 = (tbl & (tbl - 1u)) == 0u;
                 const int cx = cell_coord(cj.x, cell), cy = cell_coord(cj.y, cell), cz = cell_coord(cj.z, cell);
-                const int hk = s.cellOf[k];
+                const int hk =
+                    int(cell_hash(cell_coord(ck.x, cell), cell_coord(ck.y, cell), cell_coord(ck.z, cell), table));
                 for (int dz = -1; dz <= 1; ++dz)
                     for (int dy = -1; dy <= 1; ++dy) {
                         const unsigned hzy = (static_cast<unsigned>(cz + dz) * 83492791u) ^
@@ -562,19 +586,27 @@ __device__ __forceinline__ rbits group_wave_extreme(rbits v, int l) {
 // Broadphase and contact generation by the G lanes of a rod's lane group; lane
 // l owns segments j = l + G t, t < TP (the host guarantees nS <= TP G), and
 // keeps their centres in registers; s.cen holds them in segment order.
+// Per-regeneration scalars (longest and shortest segment, coordinate scale) set
+// for the next reduction, and the long-segment list emptied; lane 0, when no
+// lane reads them.
+__device__ __forceinline__ void self_regen_reset(const SelfLds& s) {
+    s.rmisc[1] = 0;
+    s.rmisc[2] = 0;
+    s.rmisc[3] = real_bits(real(1e30));
+}
+__device__ __forceinline__ void self_list_reset(const SelfLds& s, int nS) {
+    s.misc[2] = 0;
+    for (int i = 4; i < 8; ++i) s.misc[i] = nS;
+}
+
 template <int TP>
 __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
                                         int gap, int cap, real radius, real friction, bool chain, bool on, int l,
                                         int G) {
     real4 cj[TP];
-    if (on && l == 0) {
-        s.misc[2] = 0;
-        s.rmisc[1] = 0;
-        s.rmisc[2] = 0;
-        s.rmisc[3] = real_bits(real(1e30));
-    }
     // The longest and shortest segment and the largest coordinate: per lane,
-    // per wavefront stretch of the group, then one atomic per stretch.
+    // per wavefront stretch of the group, then one atomic per stretch (on
+    // values reset when the previous regeneration read them).
     rbits lenHi = 0, lenLo = real_bits(real(1e30)), scaleHi = 0;
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
@@ -595,7 +627,6 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
     lenHi = group_wave_extreme<false>(lenHi, l);
     scaleHi = group_wave_extreme<false>(scaleHi, l);
     lenLo = group_wave_extreme<true>(lenLo, l);
-    __syncthreads();
     if (on && ((threadIdx.x & 63) == 63 || l == G - 1)) {
         atomicMax(&s.rmisc[1], lenHi);
         atomicMax(&s.rmisc[2], scaleHi);
@@ -611,56 +642,52 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
-        if (on && j < nS) {
-            const v3 c = f4v(cj[t]);
-            s.cellOf[j] = int(cell_hash(cell_coord(c.x, cell), cell_coord(c.y, cell), cell_coord(c.z, cell), table));
-            if (cj[t].w > lenLong) {
-                const int slot = atomicAdd(&s.misc[2], 1);
-                if (slot < kLongSlots) s.misc[3 + slot] = j;
-            }
+        if (on && j < nS && cj[t].w > lenLong) {
+            const int slot = atomicAdd(&s.misc[2], 1);
+            if (slot < kLongSlots) s.misc[4 + slot] = j;
         }
     }
-    if (on && l == 0) s.rmisc[0] = real_bits(cell);
     __syncthreads();
-    // First long segment at or after each segment, in s.sorted until the fill
-    // pass needs it. With more long segments than slots, the longest segment
-    // bounds every stretch and none ends one.
+    if (on && l == 0) self_regen_reset(s);
+    // With more long segments than slots, the longest segment bounds every
+    // stretch and none ends one.
     const int numLong = on ? s.misc[2] : 0;
     const bool listed = numLong <= kLongSlots && lenLong > real(0.0);
     const real lenCap = listed ? lenLong : lenMax;
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
-        if (on && j < nS) {
-            int lf = nS;
-            for (int i = 0; listed && i < numLong; ++i) {
-                const int L = s.misc[3 + i];
-                lf = L >= j && L < lf ? L : lf;
-            }
-            s.sorted[j] = lf;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < TP; ++t) {
-        const int j = l + G * t;
         if (on && j < nS)
-            s.segCount[j] = nS > gap ? self_count_scan(s, x4, segEnd, s.sorted, nS, table, gap, radius, cell, lenCap,
+            s.segCount[j] = nS > gap ? self_count_scan(s, x4, segEnd, listed && numLong > 0, nS, table, gap, radius,
+                                                       cell, lenCap,
                                                        scaleMax, chain, j, segEnd[2 * j], segEnd[2 * j + 1], cj[t])
                                      : 0;
     }
     __syncthreads();
-    const int total = group_scan(s.segCount, s.segStart, on ? nS : 0, s.scan, l, G, on);
+    // The fill pass needs the bucket tables; they are built only where some
+    // rod of the workgroup has contacts.
+    bool any = false;
+    const int total = group_scan_vote(s.segCount, s.segStart, on ? nS : 0, s.scan, l, G, on, any);
     if (on && l == 0) {
         s.misc[0] = total < cap ? total : cap;
         if (total > cap) s.misc[1] += total - cap;
+        s.rmisc[0] = real_bits(cell);
+        self_list_reset(s, nS);
     }
-    // The fill pass needs the bucket tables; they are built only where some
-    // rod of the workgroup has contacts.
-    if (__syncthreads_or(on && total > 0) == 0) return;
+    if (!any) return;
     const bool build = on && total > 0;
-    if (build)
+    if (build) {
         for (int t = l; t <= table; t += G) s.cellStart[t] = 0;
+#pragma unroll
+        for (int t = 0; t < TP; ++t) {
+            const int j = l + G * t;
+            if (j < nS) {
+                const v3 c = f4v(cj[t]);
+                s.cellOf[j] =
+                    int(cell_hash(cell_coord(c.x, cell), cell_coord(c.y, cell), cell_coord(c.z, cell), table));
+            }
+        }
+    }
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
