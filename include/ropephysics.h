@@ -240,6 +240,27 @@ typedef struct rp_observer_desc {
 int rp_batch_set_observers(void* batch, int first_rod, int num_rods, const rp_observer_desc* observers);
 int rp_batch_get_observations(void* batch, int first_rod, int num_rods, double* max_rise);
 
+/* Trajectory recording on the device. It replaces the reference's frame loop,
+ * which downloads every rod's positions after each frame's steps
+ * (Batch::downloadPositions, src/gpu/gpu_batch.cpp:445-455) and writes them
+ * with TrajectoryWriter::frame (src/validation/scenes.cpp:45-68): here the
+ * stepping kernels store the frames themselves, into a device buffer.
+ *   - rp_batch_record starts recording at every rod's current clock. After
+ *     each step that brings a rod's clock to origin + k * every, for
+ *     k = 1..max_frames, that rod's positions go into frame k - 1.
+ *   - A frame is FP32 x, y, z per particle, rod-major in the batch's particle
+ *     order (rod r at particle_start[r]), for FP32 and FP64 batches alike.
+ *   - The buffer (max_frames frames) is zeroed on each call. every = 0 stops
+ *     recording and frees it.
+ *   - rp_batch_set_clock does not move the origin; call rp_batch_record again
+ *     after it.
+ * rp_batch_recorded_frames gives the number of frames that every rod has
+ * completed. rp_batch_get_frames copies frames [first_frame, first_frame +
+ * num_frames) of the buffer to the host, num_frames * 3 * particles floats. */
+int rp_batch_record(void* batch, int every, int max_frames);
+int rp_batch_recorded_frames(void* batch, int* frames);
+int rp_batch_get_frames(void* batch, int first_frame, int num_frames, float* positions);
+
 /* Record HIP events around every rp_batch_step on the batch's stream
  * (enable = 1) so rp_counters reports its device time. Off by default. */
 int rp_batch_set_timing(void* batch, int enable);

@@ -124,6 +124,9 @@ SIGNATURES = {
     "rp_batch_get_clock": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip]),
     "rp_batch_set_observers": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(RpObserverDesc)]),
     "rp_batch_get_observations": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "rp_batch_record": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "rp_batch_recorded_frames": (C.c_int, [C.c_void_p, _ip]),
+    "rp_batch_get_frames": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _fp]),
     "rp_batch_create64": (C.c_int, [C.POINTER(RpBatchDesc64), C.POINTER(C.c_void_p)]),
     "rp_batch_set_state64": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
     "rp_batch_set_loads64": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),

@@ -313,6 +313,26 @@ class Batch:
                                                         out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def record(self, every: int, max_frames: int):
+        """Record every rod's positions on the device after each `every` steps
+        from the rods' current clocks, up to `max_frames` frames (FP32, rod-major);
+        every = 0 stops recording and frees the frames."""
+        _abi.check(_abi.lib().rp_batch_record(self._h, int(every), int(max_frames)))
+
+    def recorded_frames(self) -> int:
+        """Frames that every rod has completed."""
+        n = np.zeros(1, np.int32)
+        _abi.check(_abi.lib().rp_batch_recorded_frames(self._h, _abi.iptr(n)))
+        return int(n[0])
+
+    def frames(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        """Frames [first, first + count) as float32 (count, particles, 3); count
+        defaults to the frames completed so far."""
+        count = self.recorded_frames() - first if count is None else count
+        out = np.empty((max(count, 0), int(self.particle_start[-1]), 3), np.float32)
+        _abi.check(_abi.lib().rp_batch_get_frames(self._h, int(first), int(count), _abi.fptr(out)))
+        return out
+
     def tune_report(self) -> list:
         """(tile configuration name, trial ms) for each shape the autotuner timed."""
         ids = np.zeros(64, np.int32)

@@ -77,6 +77,12 @@ struct rp_batch_impl {
     int* clock = nullptr;
     PathRec* paths = nullptr;
     ObsRec* obs = nullptr;
+    // Trajectory recording: per-rod records, the FP32 frame buffer and a host
+    // copy of each rod's origin clock.
+    RecRec* rec = nullptr;
+    std::unique_ptr<DevBuf> frameBuf;
+    int recEvery = 0, recMaxFrames = 0;
+    std::vector<int> recOrigin;
 
     std::unique_ptr<PackedTiles> tiles;
     int stepsPerLaunch = 64;
@@ -143,6 +149,9 @@ struct BatchT final : rp_batch_impl {
     void bind_tables() override {
         d.paths = paths;
         d.obs = obs;
+        d.rec = rec;
+        d.frames = frameBuf ? static_cast<float*>(frameBuf->p) : nullptr;
+        d.framePitch = 3LL * pStart[numRods];
     }
 };
 
@@ -255,6 +264,9 @@ int autotune(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int only
     }
     int rc = backup_state(b, false);
     if (rc) return rc;
+    // Trial steps record no frames.
+    const RecRec* recording = b.d.rec;
+    b.d.rec = nullptr;
     double best = 1e300;
     std::unique_ptr<PackedTiles> bestTiles;
     hipEvent_t e0, e1;
@@ -291,6 +303,7 @@ int autotune(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int only
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    b.d.rec = recording;
     rc = backup_state(b, true);
     if (rc) return rc;
     if (!bestTiles) return fail(RP_ERR_CAPACITY, "no tile configuration fits this batch");
@@ -421,6 +434,9 @@ int create(const typename Prec<Real>::BatchDesc* desc, void** out_batch) {
     d.paths = nullptr;
     d.obs = nullptr;
     d.stepLimit = nullptr;
+    d.rec = nullptr;
+    d.frames = nullptr;
+    d.framePitch = 0;
     d.numRods = R;
     using Vec4 = typename Prec<Real>::Vec4;
     Real* dInvMass;
@@ -838,6 +854,86 @@ int rp_batch_get_observations(void* batch, int first_rod, int num_rods, double* 
                           b.stream));
     RP_HIP(hipStreamSynchronize(b.stream));
     for (int i = 0; i < num_rods; ++i) max_rise[i] = recs[i].maxRise;
+    return RP_OK;
+}
+
+int rp_batch_record(void* batch, int every, int max_frames) {
+    if (!batch) return fail(RP_ERR_INVALID, "null batch");
+    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
+    if (every < 0 || max_frames < 0 || (every > 0 && max_frames < 1))
+        return fail(RP_ERR_INVALID, "need every >= 1 and max_frames >= 1, or every = 0 to stop recording");
+    const size_t pitch = 3 * size_t(b.pStart[b.numRods]);
+    if (every > 0 && size_t(max_frames) > (size_t(1) << 62) / (pitch * sizeof(float) + 1))
+        return fail(RP_ERR_INVALID, "frame buffer size overflows");
+    const size_t bytes = every > 0 ? size_t(max_frames) * pitch * sizeof(float) : 0;
+    int rc;
+    if (!b.rec && every > 0) {
+        if ((rc = b.alloc(b.rec, b.numRods))) return rc;
+        b.drop_graphs();
+        b.bind_tables();
+    }
+    if (every == 0 || !b.frameBuf || b.frameBuf->bytes < bytes) {
+        // The kernels' argument block holds the buffer's address: graphs
+        // captured with the old one must go before it is freed.
+        b.drop_graphs();
+        if (b.frameBuf) b.deviceBytes -= b.frameBuf->bytes;
+        b.frameBuf.reset();
+        if (bytes) {
+            auto buf = std::make_unique<DevBuf>();
+            const hipError_t e = hipMalloc(&buf->p, bytes);
+            if (e != hipSuccess)
+                return fail(RP_ERR_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes of frames failed: " +
+                                               hipGetErrorString(e));
+            buf->bytes = bytes;
+            b.deviceBytes += bytes;
+            b.frameBuf = std::move(buf);
+        }
+        b.bind_tables();
+    }
+    if (bytes) RP_HIP(hipMemsetAsync(b.frameBuf->p, 0, bytes, b.stream));
+    b.recOrigin.assign(b.numRods, 0);
+    if (b.rec) {
+        RP_HIP(hipMemcpyAsync(b.recOrigin.data(), b.clock, size_t(b.numRods) * 4, hipMemcpyDeviceToHost, b.stream));
+        RP_HIP(hipStreamSynchronize(b.stream));
+        std::vector<RecRec> recs(b.numRods);
+        for (int r = 0; r < b.numRods; ++r) recs[r] = RecRec{b.recOrigin[r], every, max_frames, 0};
+        RP_HIP(hipMemcpyAsync(b.rec, recs.data(), recs.size() * sizeof(RecRec), hipMemcpyHostToDevice, b.stream));
+    }
+    RP_HIP(hipStreamSynchronize(b.stream));
+    b.recEvery = every;
+    b.recMaxFrames = every > 0 ? max_frames : 0;
+    return RP_OK;
+}
+
+int rp_batch_recorded_frames(void* batch, int* frames) {
+    if (!batch || !frames) return fail(RP_ERR_INVALID, "null argument");
+    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
+    *frames = 0;
+    if (b.recEvery <= 0) return RP_OK;
+    std::vector<int> clk(b.numRods);
+    RP_HIP(hipMemcpyAsync(clk.data(), b.clock, clk.size() * 4, hipMemcpyDeviceToHost, b.stream));
+    RP_HIP(hipStreamSynchronize(b.stream));
+    int n = b.recMaxFrames;
+    for (int r = 0; r < b.numRods; ++r) {
+        const int k = clk[r] - b.recOrigin[r];
+        n = std::min(n, k > 0 ? k / b.recEvery : 0);
+    }
+    *frames = n;
+    return RP_OK;
+}
+
+int rp_batch_get_frames(void* batch, int first_frame, int num_frames, float* positions) {
+    if (!batch || (!positions && num_frames > 0)) return fail(RP_ERR_INVALID, "null argument");
+    rp_batch_impl& b = *static_cast<rp_batch_impl*>(batch);
+    if (b.recEvery <= 0) return fail(RP_ERR_STATE, "the batch is not recording");
+    if (first_frame < 0 || num_frames < 0 || first_frame > b.recMaxFrames - num_frames)
+        return fail(RP_ERR_INVALID, "frames [" + std::to_string(first_frame) + ", +" + std::to_string(num_frames) +
+                                        ") outside the " + std::to_string(b.recMaxFrames) + " recorded");
+    const size_t pitch = 3 * size_t(b.pStart[b.numRods]);
+    if (num_frames == 0) return RP_OK;
+    RP_HIP(hipMemcpyAsync(positions, static_cast<const float*>(b.frameBuf->p) + size_t(first_frame) * pitch,
+                          size_t(num_frames) * pitch * sizeof(float), hipMemcpyDeviceToHost, b.stream));
+    RP_HIP(hipStreamSynchronize(b.stream));
     return RP_OK;
 }
 

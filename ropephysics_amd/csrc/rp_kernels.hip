@@ -29,6 +29,26 @@ namespace RP_PREC {
 constexpr int kR = int(sizeof(real));
 constexpr int kR4 = int(sizeof(real4));
 
+// Where the rod's positions go after the step that brought its clock to
+// `clk`: its particles' slot in the frame this step completes, or null when
+// the step completes no frame (or the rod is not recording, or its frames are
+// all taken). The rod's particle start is read here rather than kept in a
+// register across the step loop.
+__device__ inline float* frame_slot(const DeviceArrays& d, int rod, int clk) {
+    if (!d.rec) return nullptr;
+    const RecRec rr = d.rec[rod];
+    const int k = clk - rr.origin;
+    if (rr.every <= 0 || k <= 0 || k % rr.every != 0 || k / rr.every > rr.maxFrames) return nullptr;
+    return d.frames + (long long)(k / rr.every - 1) * d.framePitch + 3LL * d.rods[rod].pStart;
+}
+
+// One particle's position into a frame, rounded to FP32.
+__device__ inline void frame_store(float* slot, int i, const v3& x) {
+    slot[3 * i + 0] = float(x.x);
+    slot[3 * i + 1] = float(x.y);
+    slot[3 * i + 2] = float(x.z);
+}
+
 // ------------------------------------------------------------------ PLAIN
 
 // LDS carve-up of one rod for the PLAIN kernel, in bytes.
@@ -252,6 +272,8 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
             const double rise = double(x4[obsP].z) - d.obs[r].z0;
             maxRise = maxRise < rise ? rise : maxRise;
         }
+        if (float* slot = frame_slot(d, r, clk))
+            for (int i = tid; i < nP; i += nt) frame_store(slot, i, f4v(x4[i]));
     }
     if (tid == 0) d.clock[r] = clk;
     if (obsP >= 0 && obsP % nt == tid) d.obs[r].maxRise = maxRise;
@@ -585,6 +607,13 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
         if (observer && clk % d.obs[rod].every == 0) {
             const double rise = double(x4[obsP].z) - d.obs[rod].z0;
             maxRise = maxRise < rise ? rise : maxRise;
+        }
+        if (float* slot = frame_slot(d, rod, clk)) {
+#pragma unroll
+            for (int t = 0; t < TP; ++t) {
+                const int i = l + G * t;
+                if (i < nP) frame_store(slot, i, f4v(x4[i]));
+            }
         }
     }
 

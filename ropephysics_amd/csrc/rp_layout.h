@@ -95,6 +95,19 @@ struct ObsRec {
     double maxRise;
 };
 
+// Trajectory recording of one rod: after each step that brings the rod's
+// clock to origin + k * every (k = 1..maxFrames), its positions are stored in
+// FP32 into frame k - 1 of the batch's frame buffer, rod-major at the rod's
+// particle start. This is the frame of the reference's trajectory writer
+// (TrajectoryWriter::frame on downloaded positions,
+// src/validation/scenes.cpp:45-68), written by the stepping kernel itself.
+struct RecRec {
+    int origin;  // clock at which recording started
+    int every;   // steps per frame; 0: not recording
+    int maxFrames;
+    int pad;
+};
+
 template <typename R>
 struct DeviceArraysT {
     // state
@@ -126,6 +139,9 @@ struct DeviceArraysT {
     const PathRec* paths;  // per rod, or null
     ObsRec* obs;           // per rod, or null
     const int* stepLimit;  // per rod: no step once the clock reaches it; null for none
+    const RecRec* rec;     // per rod, or null
+    float* frames;         // maxFrames frames of framePitch floats (3 per particle)
+    long long framePitch;
     int numRods;
 };
 
