@@ -187,6 +187,46 @@ def routing_fp64_leg(w: scenes.Workload, steps: int, warmup: int) -> dict:
     return ent, verdicts
 
 
+def trajectory_leg(w: scenes.Workload, every: int, warmup: int) -> dict:
+    """The routing workload recorded on the device every `every` steps (the
+    reference's trajectory frames, src/validation/scenes.cpp:13-21, taken by the
+    stepping kernel instead of a download per frame): the same full rollout
+    timed without and with recording, the frames' size and download time, and
+    the last frame checked bitwise against the final positions."""
+    from ropephysics_amd import trajectory
+
+    b = make_ragged_batch(w, "tiled")
+    reset_ragged(b, w)
+    b.step(w.params, warmup)
+    b.synchronize()
+    steps = w.steps
+    frames = steps // every
+    work = float(w.segments_total) * w.params.substeps * steps
+    reset_ragged(b, w)
+    plain_secs, _ = time_steps(b, w.params, steps)
+    reset_ragged(b, w)
+    b.record(every, frames)
+    rec_secs, c = time_steps(b, w.params, steps)
+    assert b.recorded_frames() == frames
+    t0 = time.perf_counter()
+    fr = b.frames(0, frames)
+    dl = time.perf_counter() - t0
+    last_ok = bool(np.array_equal(fr[-1].view(np.uint32), b.download_positions().astype(np.float32).view(np.uint32)))
+    groups = trajectory.split_by_size(fr[:1], b.particle_start)
+    ent = {"every_steps": every, "fps": 1.0 / (every * w.params.dt), "frames": frames,
+           "frame_bytes": int(fr[0].nbytes), "buffer_bytes": int(fr.nbytes),
+           "value_without": work / plain_secs, "value_with": work / rec_secs,
+           "overhead": rec_secs / plain_secs - 1.0, "download_seconds": dl,
+           "download_GBs": fr.nbytes / dl / 1e9, "last_frame_equals_final_positions": last_ok,
+           "rodtraj_files": len(groups), "unit": "segment-substeps/s"}
+    log(f"trajectory: {frames} frames every {every} steps ({fr.nbytes / 1e6:.0f} MB on the device): "
+        f"{ent['value_with'] / 1e9:.3f} G seg-substeps/s recording vs {ent['value_without'] / 1e9:.3f} G without "
+        f"({100 * ent['overhead']:+.1f}%); download {dl * 1e3:.0f} ms; last frame equals final positions: {last_ok}")
+    b.record(0, 0)
+    b.destroy()
+    return ent
+
+
 def cpu_baseline_routing(w: scenes.Workload, stride: int, steps: int, threads: int, coloured: bool = False) -> dict:
     """The reference CPU path (its own sources, built into oracle/_ref) on every
     `stride`-th cable of the routing workload over its first `steps` steps, FP64,
@@ -484,6 +524,8 @@ def main():
     ap.add_argument("--drape-steps", type=int, default=None, help="steps of the hold/slide sweep (default: the "
                                                                    "whole 8 s = 16000; 0: skip)")
     ap.add_argument("--drape-cpu-rods", type=int, default=32, help="cables of the CPU verdict spot check")
+    ap.add_argument("--record-every", type=int, default=16, help="trajectory leg: steps per recorded frame "
+                    "(0: skip)")
     ap.add_argument("--cem-iterations", type=int, default=10, help="harness-learning iterations (0: skip)")
     ap.add_argument("--cem-population", type=int, default=1024)
     ap.add_argument("--serve-requests", type=int, default=40, help="requests of the serving leg (0: skip)")
@@ -532,6 +574,7 @@ def main():
     drape = None
     if args.drape_steps != 0:
         drape = drape_leg(args.drape_steps, W, 0 if args.no_cpu else args.drape_cpu_rods, threads)
+    traj = trajectory_leg(w, args.record_every, W) if args.record_every > 0 and K == w.steps else None
     cem = cem_leg(args.cem_iterations, args.cem_population, not args.no_cpu) if args.cem_iterations > 0 else None
     serving_out = serving_leg(args.serve_requests, args.serve_load, args.serve_budget_gib) \
         if args.serve_requests > 0 else None
@@ -576,6 +619,7 @@ def main():
         "uniform": uniform,
         "drape": drape,
         "harness_learning": cem,
+        "trajectory": traj,
         "serving": serving_out,
         "p50_latency_s": serving_out["poisson"]["p50_latency_s"] if serving_out else None,
         "p99_latency_s": serving_out["poisson"]["p99_latency_s"] if serving_out else None,
