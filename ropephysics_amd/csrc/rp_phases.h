@@ -650,10 +650,11 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
     __syncthreads();
     if (on && l == 0) self_regen_reset(s);
     // With more long segments than slots, the longest segment bounds every
-    // stretch and none ends one.
+    // stretch and none ends one. With none, every segment is at most lenMax
+    // (itself at most twice the shortest), the tighter bound.
     const int numLong = on ? s.misc[2] : 0;
     const bool listed = numLong <= kLongSlots && lenLong > real(0.0);
-    const real lenCap = listed ? lenLong : lenMax;
+    const real lenCap = listed ? min_ab(lenLong, lenMax) : lenMax;
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
