@@ -491,6 +491,47 @@ __device__ inline void self_fill_rows(const SelfLds& s, const real4* x4, const i
         }
 }
 
+// The fill pass for a segment whose contacts are known: kk[0, nk) are its
+// later segments in contact, ascending, each with a nonzero count. In the
+// bucketed query's order (27 cells dz, dy, dx; within a bucket, ascending
+// segment index), the list is, cell by cell, the known contacts whose bucket
+// is that cell's, ascending -- a contact met at several cells of one bucket is
+// written once per cell. Neither the bucket tables nor the other segments are
+// read.
+template <int M>
+__device__ inline void self_fill_listed(const SelfLds& s, const real4* x4, const int* segEnd, int table, int cap,
+                                        real radius, real friction, real cell, int j0, int j1, real4 cj4,
+                                        int slotBase, const int (&kk)[M], int nk) {
+    const v3 cj = f4v(cj4);
+    const int cx = cell_coord(cj.x, cell), cy = cell_coord(cj.y, cell), cz = cell_coord(cj.z, cell);
+    const v3 a0 = f4v(x4[j0]), a1 = f4v(x4[j1]);
+    const unsigned tbl = static_cast<unsigned>(table);
+    const bool pow2 = (tbl & (tbl - 1u)) == 0u;
+    int hk[M];
+#pragma unroll
+    for (int e = 0; e < M; ++e) {
+        hk[e] = -1;
+        if (e < nk) {
+            const v3 ck = f4v(s.cen[kk[e]]);
+            hk[e] = int(cell_hash(cell_coord(ck.x, cell), cell_coord(ck.y, cell), cell_coord(ck.z, cell), table));
+        }
+    }
+    int count = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy) {
+            const unsigned hzy =
+                (static_cast<unsigned>(cz + dz) * 83492791u) ^ (static_cast<unsigned>(cy + dy) * 19349663u);
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int h = row_bucket(cx, dx, hzy, tbl, pow2);
+#pragma unroll
+                for (int e = 0; e < M; ++e)
+                    if (hk[e] == h)
+                        self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * kk[e]], segEnd[2 * kk[e] + 1], radius, friction,
+                                  true, slotBase, cap, count);
+            }
+        }
+}
+
 // Number of contacts segment j has in the list: its later segments k >= j +
 // gap scanned in index order, each contact counted once for every one of j's
 // 27 cells whose bucket is k's (the times the bucketed query reaches k). The
@@ -507,9 +548,14 @@ __device__ inline void self_fill_rows(const SelfLds& s, const real4* x4, const i
 // Before the closest-point test, a pair is also dropped when segment j is
 // farther from k's centre than k's half-length plus 2r (same margins): every
 // point of segment k lies within its half-length of its centre.
+//
+// The first M contacts reached (those with a nonzero count, in ascending k)
+// are kept in kk[]; nk is the number of them, which may exceed M.
+template <int M>
 __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const int* segEnd, bool longs, int nS,
                                       int table, int gap, real radius, real cell, real lenCap, real scaleMax,
-                                      bool chain, int j, int j0, int j1, real4 cj4) {
+                                      bool chain, int j, int j0, int j1, real4 cj4, int (&kk)[M > 0 ? M : 1],
+                                      int& nk) {
     const v3 cj = f4v(cj4);
     const real reachJ = real(0.5) * cj4.w + real(2.0) * radius;
     const real eps = real(1e-6) * (real(1.0) + scaleMax);
@@ -553,13 +599,21 @@ __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const i
                 const int cx = cell_coord(cj.x, cell), cy = cell_coord(cj.y, cell), cz = cell_coord(cj.z, cell);
                 const int hk =
                     int(cell_hash(cell_coord(ck.x, cell), cell_coord(ck.y, cell), cell_coord(ck.z, cell), table));
+                int mult = 0;
                 for (int dz = -1; dz <= 1; ++dz)
                     for (int dy = -1; dy <= 1; ++dy) {
                         const unsigned hzy = (static_cast<unsigned>(cz + dz) * 83492791u) ^
                                              (static_cast<unsigned>(cy + dy) * 19349663u);
 #pragma unroll
-                        for (int dx = -1; dx <= 1; ++dx) count += row_bucket(cx, dx, hzy, tbl, pow2) == hk ? 1 : 0;
+                        for (int dx = -1; dx <= 1; ++dx) mult += row_bucket(cx, dx, hzy, tbl, pow2) == hk ? 1 : 0;
                     }
+                if (M > 0 && mult > 0) {
+#pragma unroll
+                    for (int m = 0; m < M; ++m)
+                        if (nk == m) kk[m] = k;
+                    ++nk;
+                }
+                count += mult;
             }
         }
         ++k;
@@ -597,6 +651,54 @@ __device__ __forceinline__ void self_regen_reset(const SelfLds& s) {
 __device__ __forceinline__ void self_list_reset(const SelfLds& s, int nS) {
     s.misc[2] = 0;
     for (int i = 4; i < 8; ++i) s.misc[i] = nS;
+}
+
+// The bucket tables of the rods with contacts (`build`): cellOf, cellStart
+// (bucket offsets) and sorted (segments by bucket, ascending within one).
+template <int TP>
+__device__ inline void self_build_tables(const SelfLds& s, int nS, int table, real cell, bool build,
+                                         const real4 (&cj)[TP], int l, int G) {
+    if (build) {
+        for (int t = l; t <= table; t += G) s.cellStart[t] = 0;
+#pragma unroll
+        for (int t = 0; t < TP; ++t) {
+            const int j = l + G * t;
+            if (j < nS) {
+                const v3 c = f4v(cj[t]);
+                s.cellOf[j] =
+                    int(cell_hash(cell_coord(c.x, cell), cell_coord(c.y, cell), cell_coord(c.z, cell), table));
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+        const int j = l + G * t;
+        if (build && j < nS) atomicAdd(&s.cellStart[s.cellOf[j]], 1);
+    }
+    __syncthreads();
+    const int items = group_scan(s.cellStart, s.cellStart, build ? table : 0, s.scan, l, G, build);
+    if (build && l == 0) s.cellStart[table] = items;
+    // Stable placement: j goes after the earlier segments of its bucket.
+    // cellOf is 16-byte aligned, so four buckets per LDS read.
+    const int4* cellOf4 = reinterpret_cast<const int4*>(s.cellOf);
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+        const int j = l + G * t;
+        if (build && j < nS) {
+            const int h = s.cellOf[j];
+            int rank = 0, i = 0;
+            for (; i + 16 <= j; i += 16) {
+                const int4 c0 = cellOf4[(i >> 2) + 0], c1 = cellOf4[(i >> 2) + 1];
+                const int4 c2 = cellOf4[(i >> 2) + 2], c3 = cellOf4[(i >> 2) + 3];
+                rank += count_eq4(c0, h) + count_eq4(c1, h) + count_eq4(c2, h) + count_eq4(c3, h);
+            }
+            for (; i + 4 <= j; i += 4) rank += count_eq4(cellOf4[i >> 2], h);
+            for (; i < j; ++i) rank += s.cellOf[i] == h ? 1 : 0;
+            s.sorted[s.cellStart[h] + rank] = j;
+        }
+    }
+    __syncthreads();
 }
 
 template <int TP>
@@ -655,18 +757,24 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
     const int numLong = on ? s.misc[2] : 0;
     const bool listed = numLong <= kLongSlots && lenLong > real(0.0);
     const real lenCap = listed ? min_ab(lenLong, lenMax) : lenMax;
+    // Contacts kept per segment for the fill: a few registers where a lane
+    // owns one or two segments, none beyond.
+    constexpr int kM = TP == 1 ? 6 : (TP == 2 ? 3 : 0);
+    constexpr int kMr = kM > 0 ? kM : 1;
+    int kk[TP][kMr];
+    int nk[TP];
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
+        nk[t] = 0;
         if (on && j < nS)
-            s.segCount[j] = nS > gap ? self_count_scan(s, x4, segEnd, listed && numLong > 0, nS, table, gap, radius,
-                                                       cell, lenCap,
-                                                       scaleMax, chain, j, segEnd[2 * j], segEnd[2 * j + 1], cj[t])
+            s.segCount[j] = nS > gap ? self_count_scan<kM>(s, x4, segEnd, listed && numLong > 0, nS, table, gap,
+                                                            radius, cell, lenCap, scaleMax, chain, j, segEnd[2 * j],
+                                                            segEnd[2 * j + 1], cj[t], kk[t], nk[t])
                                      : 0;
     }
     __syncthreads();
-    // The fill pass needs the bucket tables; they are built only where some
-    // rod of the workgroup has contacts.
+    // The fill pass runs only where some rod of the workgroup has contacts.
     bool any = false;
     const int total = group_scan_vote(s.segCount, s.segStart, on ? nS : 0, s.scan, l, G, on, any);
     if (on && l == 0) {
@@ -676,54 +784,31 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
         self_list_reset(s, nS);
     }
     if (!any) return;
-    const bool build = on && total > 0;
-    if (build) {
-        for (int t = l; t <= table; t += G) s.cellStart[t] = 0;
+    // The bucket tables are needed only for a segment with more contacts than
+    // it keeps.
+    bool over = kM == 0;
+    if constexpr (kM > 0) {
 #pragma unroll
         for (int t = 0; t < TP; ++t) {
             const int j = l + G * t;
-            if (j < nS) {
-                const v3 c = f4v(cj[t]);
-                s.cellOf[j] =
-                    int(cell_hash(cell_coord(c.x, cell), cell_coord(c.y, cell), cell_coord(c.z, cell), table));
+            over = over || (on && j < nS && nk[t] > kM);
+        }
+        over = __syncthreads_or(over) != 0;
+    }
+    if (over) self_build_tables<TP>(s, nS, table, cell, on && total > 0, cj, l, G);
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+        const int j = l + G * t;
+        if (!(on && j < nS && s.segCount[j] > 0)) continue;
+        if constexpr (kM > 0) {
+            if (nk[t] <= kM) {
+                self_fill_listed<kMr>(s, x4, segEnd, table, cap, radius, friction, cell, segEnd[2 * j],
+                                      segEnd[2 * j + 1], cj[t], s.segStart[j], kk[t], nk[t]);
+                continue;
             }
         }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < TP; ++t) {
-        const int j = l + G * t;
-        if (build && j < nS) atomicAdd(&s.cellStart[s.cellOf[j]], 1);
-    }
-    __syncthreads();
-    const int items = group_scan(s.cellStart, s.cellStart, build ? table : 0, s.scan, l, G, build);
-    if (build && l == 0) s.cellStart[table] = items;
-    // Stable placement: j goes after the earlier segments of its bucket.
-    // cellOf is 16-byte aligned, so four buckets per LDS read.
-    const int4* cellOf4 = reinterpret_cast<const int4*>(s.cellOf);
-#pragma unroll
-    for (int t = 0; t < TP; ++t) {
-        const int j = l + G * t;
-        if (build && j < nS) {
-            const int h = s.cellOf[j];
-            int rank = 0, i = 0;
-            for (; i + 16 <= j; i += 16) {
-                const int4 c0 = cellOf4[(i >> 2) + 0], c1 = cellOf4[(i >> 2) + 1];
-                const int4 c2 = cellOf4[(i >> 2) + 2], c3 = cellOf4[(i >> 2) + 3];
-                rank += count_eq4(c0, h) + count_eq4(c1, h) + count_eq4(c2, h) + count_eq4(c3, h);
-            }
-            for (; i + 4 <= j; i += 4) rank += count_eq4(cellOf4[i >> 2], h);
-            for (; i < j; ++i) rank += s.cellOf[i] == h ? 1 : 0;
-            s.sorted[s.cellStart[h] + rank] = j;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < TP; ++t) {
-        const int j = l + G * t;
-        if (build && j < nS && s.segCount[j] > 0)
-            self_fill_rows(s, x4, segEnd, nS, table, gap, cap, radius, friction, cell, j, segEnd[2 * j],
-                           segEnd[2 * j + 1], cj[t], s.segStart[j]);
+        self_fill_rows(s, x4, segEnd, nS, table, gap, cap, radius, friction, cell, j, segEnd[2 * j],
+                       segEnd[2 * j + 1], cj[t], s.segStart[j]);
     }
 }
 
