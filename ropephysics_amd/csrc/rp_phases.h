@@ -218,16 +218,17 @@ struct SelfLds {
     real* pool;     // cap * kSelfFloats
     int* misc;       // [0] live, [1] overflow, [2] long segments, [4..7] the first four of them (nS: none)
     rbits* rmisc;    // [0] cell size, [1] longest segment, [2] largest |coordinate| of a centre, [3] shortest
-                     // segment, as bits (non-negative: ordered as integers)
+                     // segment, [4] longest segment that is not long, as bits (non-negative: ordered as
+                     // integers)
     int* scan;       // kScanWaves, scratch of the group-parallel scans
 };
 
 // Wavefronts a lane group may span (512 lanes).
 constexpr int kScanWaves = 8;
 
-// misc (8 ints) then rmisc (4 reals), padded so the int arrays after them stay
+// misc (8 ints) then rmisc (5 reals), padded so the int arrays after them stay
 // 16-byte aligned.
-constexpr int kMiscBytes = 64;
+constexpr int kMiscBytes = 80;
 constexpr int kLongSlots = 4;
 
 __host__ __device__ inline int self_lds_bytes(int nS, int table, int cap) {
@@ -651,6 +652,7 @@ __device__ __forceinline__ void self_regen_reset(const SelfLds& s) {
 __device__ __forceinline__ void self_list_reset(const SelfLds& s, int nS) {
     s.misc[2] = 0;
     for (int i = 4; i < 8; ++i) s.misc[i] = nS;
+    s.rmisc[4] = 0;
 }
 
 // The bucket tables of the rods with contacts (`build`): cellOf, cellStart
@@ -741,22 +743,28 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
     // listed, and skips stop at them.
     const real lenLong = on ? real(2.0) * bits_real(s.rmisc[3]) : real(0.0);
     const real cell = lenMax + real(2.0) * radius;
+    rbits shortHi = 0;
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
         if (on && j < nS && cj[t].w > lenLong) {
             const int slot = atomicAdd(&s.misc[2], 1);
             if (slot < kLongSlots) s.misc[4 + slot] = j;
+        } else if (on && j < nS) {
+            const rbits lb = real_bits(cj[t].w);
+            shortHi = lb > shortHi ? lb : shortHi;
         }
     }
+    shortHi = group_wave_extreme<false>(shortHi, l);
+    if (on && ((threadIdx.x & 63) == 63 || l == G - 1)) atomicMax(&s.rmisc[4], shortHi);
     __syncthreads();
     if (on && l == 0) self_regen_reset(s);
     // With more long segments than slots, the longest segment bounds every
-    // stretch and none ends one. With none, every segment is at most lenMax
-    // (itself at most twice the shortest), the tighter bound.
+    // stretch and none ends one. Otherwise a stretch holds no long segment, and
+    // the longest of the others bounds it.
     const int numLong = on ? s.misc[2] : 0;
     const bool listed = numLong <= kLongSlots && lenLong > real(0.0);
-    const real lenCap = listed ? min_ab(lenLong, lenMax) : lenMax;
+    const real lenCap = listed ? (on ? bits_real(s.rmisc[4]) : real(0.0)) : lenMax;
     // Contacts kept per segment for the fill: a few registers where a lane
     // owns one or two segments, none beyond.
     constexpr int kM = TP == 1 ? 6 : (TP == 2 ? 3 : 0);
