@@ -82,6 +82,16 @@ typedef struct rp_world_desc {
                               counted as overflow (and dropped, where the CPU path keeps them) */
     float self_capacity_per_segment; /* with self_capacity 0: max(16, ceil(this * segments)) slots
                                         when positive, else the reference's max(16, segments / 2) */
+    int self_order;        /* 0: self contacts projected one at a time in list order, the
+                              reference's order (projectContacts, src/core/solver.cpp:163-216);
+                              1: coloured order, an opt-in of this library with no counterpart
+                              in the reference -- each regeneration colours the list greedily in
+                              list order (the lowest of 32 colours no earlier contact sharing a
+                              particle holds; a tail when all are taken) and projects colour by
+                              colour, a colour's contacts at once, then the tail in list order.
+                              Results then differ from the reference's, within the solver's own
+                              ordering freedom; they equal a one-at-a-time projection in that
+                              order bit for bit. */
 } rp_world_desc;
 
 /* A ragged batch: rods may differ in every size. Element arrays are indexed
@@ -317,6 +327,7 @@ typedef struct rp_world_desc64 {
     int hash_table_size;
     int self_capacity;
     double self_capacity_per_segment;
+    int self_order;
 } rp_world_desc64;
 
 typedef struct rp_batch_desc64 {

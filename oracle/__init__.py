@@ -53,7 +53,7 @@ class Prim(C.Structure):
 class WorldC(C.Structure):
     _fields_ = [("numPrims", C.c_int), ("prims", C.POINTER(Prim)), ("selfCollision", C.c_int),
                 ("selfFriction", C.c_double), ("selfSkip", C.c_int), ("marginRadii", C.c_double),
-                ("hashTableSize", C.c_int)]
+                ("hashTableSize", C.c_int), ("selfOrder", C.c_int)]
 
 
 class ParamsC(C.Structure):
@@ -121,6 +121,8 @@ def lib(kind: str = "oracle"):
     L.orc_cell_coord.argtypes = [C.c_int, C.c_double, C.c_double]
     L.orc_self_gap.argtypes = [vp, C.POINTER(WorldC)]
     L.orc_colour.argtypes = [vp, C.c_int, _ip, _ip, C.c_int]
+    if hasattr(L, "orc_self_contacts"):
+        L.orc_self_contacts.argtypes = [vp, _ip, _ip, _ip, C.c_int]
     if hasattr(L, "orc_count_ops"):
         L.orc_count_ops.argtypes = [vp, C.POINTER(ParamsC), C.POINTER(WorldC), C.c_int,
                                     C.POINTER(C.c_longlong)]
@@ -158,12 +160,13 @@ class World:
     self_skip: int = 2
     margin_radii: float = 0.25
     hash_table_size: int = 4096
+    self_order: int = 0  # 0: list order (the reference's); 1: coloured order (this build's opt-in)
 
     def c(self) -> WorldC:
         arr = (Prim * max(1, len(self.prims)))(*self.prims)
         w = WorldC(len(self.prims), C.cast(arr, C.POINTER(Prim)), int(self.self_collision),
                    float(self.self_friction), int(self.self_skip), float(self.margin_radii),
-                   int(self.hash_table_size))
+                   int(self.hash_table_size), int(self.self_order))
         w._keep = arr
         return w
 
@@ -277,6 +280,18 @@ class Rod:
         w, s = C.c_int(), C.c_int()
         self.L.orc_contact_counts(self.h, C.byref(w), C.byref(s))
         return w.value, s.value
+
+    def self_contacts(self) -> dict:
+        """The self contacts of the last contact generation, in list order: their
+        particle indices (n, 4), and under the coloured order their colours (32:
+        the sequential tail) and positions in the projection order (-1 otherwise)."""
+        n = self.L.orc_self_contacts(self.h, None, None, None, 0)
+        idx = np.zeros((max(n, 1), 4), np.int32)
+        col = np.zeros(max(n, 1), np.int32)
+        rank = np.zeros(max(n, 1), np.int32)
+        self.L.orc_self_contacts(self.h, idx.ctypes.data_as(_ip), col.ctypes.data_as(_ip),
+                                 rank.ctypes.data_as(_ip), n)
+        return {"idx": idx[:n], "colour": col[:n], "rank": rank[:n]}
 
     def count_ops(self, params: Params, world: World | None = None, steps: int = 1) -> dict:
         """Floating-point operations of `steps` steps of a copy of this FP64 rod."""

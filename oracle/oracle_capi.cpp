@@ -63,6 +63,7 @@ orc::World<T> worldOf(const orc_world& w) {
     r.selfSkip = w.selfSkip;
     r.marginRadii = T(w.marginRadii);
     r.table = w.hashTableSize;
+    r.selfOrder = w.selfOrder;
     return r;
 }
 
@@ -280,6 +281,24 @@ orc::Rod<orc::Cnt> countingRod(const orc::Rod<double>& s) {
 
 }  // namespace
 
+template <typename T>
+int selfContactsOf(const Holder<T>& h, int* idx4, int* colour, int* rank, int capacity) {
+    const auto& set = h.ctx.contacts;
+    const int first = h.ctx.worldContacts, n = int(set.size()) - first;
+    const bool coloured = int(h.ctx.contactColour.size()) == int(set.size()) && !set.empty();
+    std::vector<int> pos(set.size(), -1);
+    if (coloured)
+        for (size_t ii = size_t(first); ii < h.ctx.contactOrder.size(); ++ii)
+            pos[size_t(h.ctx.contactOrder[ii])] = int(ii) - first;
+    for (int i = 0; i < n && i < capacity; ++i) {
+        const auto& c = set[size_t(first + i)];
+        for (int k = 0; k < 4; ++k) idx4[4 * i + k] = k < c.count ? c.idx[k] : -1;
+        colour[i] = coloured ? h.ctx.contactColour[size_t(first + i)] : -1;
+        rank[i] = coloured ? pos[size_t(first + i)] : -1;
+    }
+    return n;
+}
+
 extern "C" {
 
 void* orc_rod_straight(int precision, int n, double length, const double material[4], const double origin[3],
@@ -487,6 +506,12 @@ void orc_seg_seg(int precision, const double p1[3], const double q1[3], const do
 }
 
 unsigned orc_hash_cell(int ix, int iy, int iz, int table) { return orc::cellHash(ix, iy, iz, table); }
+
+int orc_self_contacts(void* rod, int* idx4, int* colour, int* rank, int capacity) {
+    Handle* h = H(rod);
+    return h->precision == 32 ? selfContactsOf(h->f, idx4, colour, rank, capacity)
+                              : selfContactsOf(h->d, idx4, colour, rank, capacity);
+}
 
 int orc_cell_coord(int precision, double v, double cell) {
     return precision == 32 ? orc::cellOf(float(v), float(cell)) : orc::cellOf(v, cell);

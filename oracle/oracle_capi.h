@@ -46,6 +46,7 @@ typedef struct orc_world {
     int selfSkip;
     double marginRadii;
     int hashTableSize;
+    int selfOrder; /* 0: self contacts projected in list order (the reference's); 1: coloured order */
 } orc_world;
 
 typedef struct orc_params {
@@ -92,6 +93,13 @@ double orc_signed_distance(int precision, const orc_prim* prim, const double p[3
 void orc_seg_seg(int precision, const double p1[3], const double q1[3], const double p2[3], const double q2[3],
                  double st[2]);
 unsigned orc_hash_cell(int ix, int iy, int iz, int table);
+
+/* The self contacts of the last contact generation, in list order: their four
+ * particle indices, and (when the world asked for the coloured order) each
+ * one's colour (32: the sequential tail) and its position in the projection
+ * order among the self contacts; -1 otherwise. Returns the number of self
+ * contacts (which may exceed capacity; only capacity are written). */
+int orc_self_contacts(void* rod, int* idx4, int* colour, int* rank, int capacity);
 int orc_cell_coord(int precision, double v, double cell);
 int orc_self_gap(void* rod, const orc_world* world);
 /* Step a copy of an FP64 rod with the operation-counting scalar and report

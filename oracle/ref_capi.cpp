@@ -247,6 +247,7 @@ void orc_rod_free(void* rod) { delete H(rod); }
 
 int orc_step(void* rod, const orc_params* p, const orc_world* world, int steps) {
     if (p->complianceScale != 1.0 || p->frictionScale != 1.0) return -1;
+    if (world && world->selfOrder != 0) return -1;  // the reference projects self contacts in list order only
     Handle* h = H(rod);
     SolverParams sp;
     sp.dt = Real(p->dt);

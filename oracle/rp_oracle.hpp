@@ -653,6 +653,7 @@ struct World {
     int selfSkip = 2;
     T marginRadii = T(0.25);
     int table = 4096;
+    int selfOrder = 0;  // 0: self contacts in list order; 1: coloured order (selfColourOrder)
 };
 
 template <typename T>
@@ -855,11 +856,48 @@ inline void projectBend(Rod<T>& rod, T h, T compScale, const Colouring* col) {
     }
 }
 
-// Unilateral contact plus position-level Coulomb friction capped per substep
-// (reference src/core/solver.cpp:163-216).
+// The coloured projection order of a contact list (an opt-in order of this
+// build; the reference projects in list order): world contacts first, as
+// listed, then the self contacts (from index `first`) coloured greedily in
+// list order -- each takes the lowest of 32 colours that no earlier self
+// contact sharing one of its particles holds, or joins a sequential tail when
+// all 32 are taken -- and ordered by colour, then by list position, the tail
+// last. Contacts of one colour share no particle, so projecting a colour's
+// contacts in any order, or at once, gives the same result.
 template <typename T>
-inline void projectContacts(Rod<T>& rod, std::vector<Contact<T>>& set) {
-    for (Contact<T>& c : set) {
+inline void selfColourOrder(const Rod<T>& rod, const std::vector<Contact<T>>& set, int first,
+                            std::vector<int>& order, std::vector<int>& colour) {
+    const int n = int(set.size());
+    order.clear();
+    for (int i = 0; i < first; ++i) order.push_back(i);
+    std::vector<uint32_t> used(size_t(rod.nP()), 0u);
+    colour.assign(size_t(n), -1);
+    for (int i = first; i < n; ++i) {
+        const Contact<T>& c = set[i];
+        uint32_t mask = 0u;
+        for (int k = 0; k < c.count; ++k) mask |= used[c.idx[k]];
+        int col = 32;
+        for (int b = 0; b < 32; ++b)
+            if (!(mask & (1u << b))) {
+                col = b;
+                break;
+            }
+        colour[i] = col;
+        if (col < 32)
+            for (int k = 0; k < c.count; ++k) used[c.idx[k]] |= 1u << col;
+    }
+    for (int col = 0; col <= 32; ++col)
+        for (int i = first; i < n; ++i)
+            if (colour[i] == col) order.push_back(i);
+}
+
+// Unilateral contact plus position-level Coulomb friction capped per substep
+// (reference src/core/solver.cpp:163-216), in list order or in `order`.
+template <typename T>
+inline void projectContacts(Rod<T>& rod, std::vector<Contact<T>>& set, const std::vector<int>* order = nullptr) {
+    const int n = int(set.size());
+    for (int ii = 0; ii < n; ++ii) {
+        Contact<T>& c = set[order ? (*order)[ii] : ii];
         T wEff = 0;
         for (int k = 0; k < c.count; ++k) wEff += c.w[k] * c.w[k] * rod.invMass[c.idx[k]];
         if (wEff <= T(0)) continue;
@@ -889,7 +927,7 @@ inline void projectContacts(Rod<T>& rod, std::vector<Contact<T>>& set) {
 // One substep of size h (reference src/core/solver.cpp:218-267).
 template <typename T>
 inline void substep(Rod<T>& rod, const Params<T>& p, T h, std::vector<Contact<T>>* contacts,
-                    const Colouring* sc, const Colouring* bc) {
+                    const Colouring* sc, const Colouring* bc, const std::vector<int>* contactOrder = nullptr) {
     const int nP = rod.nP(), nF = rod.nF();
     for (int i = 0; i < nP; ++i) {
         rod.xPrev[i] = rod.x[i];
@@ -911,7 +949,7 @@ inline void substep(Rod<T>& rod, const Params<T>& p, T h, std::vector<Contact<T>
     for (int it = 0; it < p.iterations; ++it) {
         projectStretch(rod, h, p.complianceScale, sc);
         projectBend(rod, h, p.complianceScale, bc);
-        if (contacts) projectContacts(rod, *contacts);
+        if (contacts) projectContacts(rod, *contacts, contactOrder);
     }
     const T linDecay = oExp(-p.linDamp * h);
     const T angDecay = oExp(-p.angDamp * h);
@@ -939,6 +977,8 @@ struct Context {
     Hash<T> hash;
     Colouring sc, bc;
     int worldContacts = 0;
+    std::vector<int> contactOrder;   // the coloured order, when the world asks for it
+    std::vector<int> contactColour;  // per contact: its colour (32: the tail), -1 for world contacts
 };
 
 // One step of p.dt (reference src/core/solver.cpp:269-290). With `world` null
@@ -954,8 +994,11 @@ inline void step(Rod<T>& rod, const Params<T>& p, const World<T>* world, Context
     }
     const int interval = std::max(1, p.contactInterval);
     for (int i = 0; i < p.substeps; ++i) {
-        if (world && i % interval == 0)
+        if (world && i % interval == 0) {
             generateContacts(rod, *world, p.frictionScale, ctx.contacts, ctx.hash, &ctx.worldContacts);
+            if (world->selfOrder)
+                selfColourOrder(rod, ctx.contacts, ctx.worldContacts, ctx.contactOrder, ctx.contactColour);
+        }
         std::fill(rod.sLam.begin(), rod.sLam.end(), V3<T>());
         std::fill(rod.bLam.begin(), rod.bLam.end(), V3<T>());
         if (world)
@@ -963,7 +1006,8 @@ inline void step(Rod<T>& rod, const Params<T>& p, const World<T>* world, Context
                 c.lamN = 0;
                 c.applied = 0;
             }
-        substep(rod, p, h, world ? &ctx.contacts : nullptr, sc, bc);
+        substep(rod, p, h, world ? &ctx.contacts : nullptr, sc, bc,
+                world && world->selfOrder ? &ctx.contactOrder : nullptr);
     }
 }
 

@@ -33,7 +33,7 @@ def _batch_structs(real, suffix):
         _fields_ = [("num_prims", C.c_int), ("prims", C.POINTER(Primitive)), ("contact_margin_radii", real),
                     ("self_collision", C.c_int), ("self_friction", real), ("self_skip", C.c_int),
                     ("hash_table_size", C.c_int), ("self_capacity", C.c_int),
-                    ("self_capacity_per_segment", real)]
+                    ("self_capacity_per_segment", real), ("self_order", C.c_int)]
 
     class BatchDesc(C.Structure):
         _fields_ = [("num_rods", C.c_int), ("particle_start", _ip), ("frame_start", _ip), ("stretch_start", _ip),

@@ -73,7 +73,7 @@ def _world_desc(w: CollisionWorld, precision: int = 32):
     d = pr.WorldDesc(len(w.primitives), C.cast(prims, C.POINTER(pr.Primitive)),
                          float(w.contact_margin_radii), int(w.self_collision), float(w.self_friction),
                          int(w.self_skip), int(w.hash_table_size), int(w.self_capacity),
-                         float(w.self_capacity_per_segment))
+                         float(w.self_capacity_per_segment), int(w.self_order))
     return d, prims
 
 

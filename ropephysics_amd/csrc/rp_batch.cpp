@@ -91,7 +91,8 @@ struct rp_batch_impl {
     std::vector<int> colStart;  // host copy of the colour-start table
     std::vector<std::pair<int, float>> tuneLog;  // (config, ms for the trial) of the last autotune
     bool timing = false;
-    bool anySelf = false;  // some rod collides with itself: run the SELF kernel variant
+    bool anySelf = false;      // some rod collides with itself: run a self-collision kernel variant
+    bool anyColoured = false;  // ... and some asks for the coloured self order: the variant that has it
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool evPending = false;
     int lastLaunches = 0;
@@ -210,7 +211,8 @@ int launch_steps(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int 
     while (left > 0) {
         const int n = std::min(left, b.stepsPerLaunch);
         const StepArgsT<Real> a = step_args<Real>(p, n);
-        RP_HIP(Prec<Real>::launch_tiled(cfg.tp, cfg.tc, b.anySelf, b.d, a, static_cast<const Tile*>(t->dTiles.p),
+        const int selfMode = b.anySelf ? (b.anyColoured ? 2 : 1) : 0;
+        RP_HIP(Prec<Real>::launch_tiled(cfg.tp, cfg.tc, selfMode, b.d, a, static_cast<const Tile*>(t->dTiles.p),
                                         static_cast<const TileSlot*>(t->dSlots.p), int(t->tiles.size()),
                                         64 * cfg.waves, size_t(t->maxLds), b.stream));
         left -= n;
@@ -360,6 +362,7 @@ int create(const typename Prec<Real>::BatchDesc* desc, void** out_batch) {
     b->colStart = rec.colStart;
     b->tileable = rec.colourFit;
     b->anySelf = rec.anySelf;
+    b->anyColoured = rec.anyColoured;
     b->plainLds = rec.maxPlainLds;
 
     LdsLimits lim;

@@ -269,6 +269,8 @@ int build_records(const typename Prec<Real>::BatchDesc* desc, RecordsT<Real>& ou
             return fail(RP_ERR_INVALID, "world " + std::to_string(w) + ": hash_table_size must be positive");
         if (wd.self_capacity < 0 || !(wd.self_capacity_per_segment >= 0) || wd.self_capacity_per_segment > 64)
             return fail(RP_ERR_INVALID, "world " + std::to_string(w) + ": self capacity out of range");
+        if (wd.self_order != 0 && wd.self_order != 1)
+            return fail(RP_ERR_INVALID, "world " + std::to_string(w) + ": self_order is 0 (list) or 1 (coloured)");
     }
 
     // ---- per-rod colouring, constraint reordering and rod info
@@ -406,6 +408,12 @@ int build_records(const typename Prec<Real>::BatchDesc* desc, RecordsT<Real>& ou
                     ri.selfCap = std::max(16, int(std::ceil(double(wd.self_capacity_per_segment) * ri.nS)));
                 else
                     ri.selfCap = std::max(16, ri.nS / 2);  // src/gpu/gpu_batch.cpp:169
+                ri.selfOrder = wd.self_order;
+                // The coloured order keeps a colour mask per particle in the
+                // broadphase's per-segment arrays (4 ints per segment).
+                if (ri.selfOrder && ri.nP > 4 * ri.nS)
+                    return fail(RP_ERR_INVALID, "rod " + std::to_string(r) +
+                                                    ": the coloured self order needs at most 4 particles per segment");
             }
         } else {
             ri.table = 1;
@@ -413,7 +421,10 @@ int build_records(const typename Prec<Real>::BatchDesc* desc, RecordsT<Real>& ou
         maxPlainLds =
             std::max(maxPlainLds, Prec<Real>::plain_lds_bytes(ri.nP, ri.nF, ri.nS, ri.nB, ri.nPrim, ri.self, ri.table, ri.selfCap));
     }
-    for (int r = 0; r < R; ++r) out.anySelf = out.anySelf || out.rods[r].self != 0;
+    for (int r = 0; r < R; ++r) {
+        out.anySelf = out.anySelf || out.rods[r].self != 0;
+        out.anyColoured = out.anyColoured || (out.rods[r].self != 0 && out.rods[r].selfOrder != 0);
+    }
     out.maxPlainLds = maxPlainLds;
     out.invMass.assign(desc->inv_mass, desc->inv_mass + P);
     out.iI4.resize(F);

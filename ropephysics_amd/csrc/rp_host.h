@@ -21,7 +21,7 @@ namespace rp {
     int tiled_lds_bytes(int nP, int nF, int nS, int nPrim, int self, int table, int cap);                      \
     hipError_t launch_plain(const DeviceArraysT<R>& d, const StepArgsT<R>& sa, size_t lds, hipStream_t s);      \
     hipError_t allow_dynamic_lds(int bytes, int* granted);                                                     \
-    hipError_t launch_tiled(int tp, int tc, bool self, const DeviceArraysT<R>& d, const StepArgsT<R>& sa,      \
+    hipError_t launch_tiled(int tp, int tc, int self, const DeviceArraysT<R>& d, const StepArgsT<R>& sa,      \
                             const Tile* tiles, const TileSlot* slots, int numTiles, int threads, size_t lds,   \
                             hipStream_t s);                                                                    \
     bool tiled_shape_available(int tp, int tc, int threads);                                                   \
@@ -120,6 +120,7 @@ struct RecordsT {
     std::vector<PrimT<Real>> prims;
     int maxPlainLds = 0;
     bool anySelf = false;
+    bool anyColoured = false;  // some self-colliding rod asks for the coloured self order
 };
 using Records = RecordsT<float>;
 

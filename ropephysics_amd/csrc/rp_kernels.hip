@@ -188,6 +188,10 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
                     self_reset(sl);
                 }
                 __syncthreads();
+                if (regen && ri.selfOrder) {
+                    if (tid == 0) self_colour(sl, nP);
+                    __syncthreads();
+                }
             }
             for (int i = tid; i < nP; i += nt) {
                 v3 x = f4v(x4[i]), v = f4v(v4[i]), xp;
@@ -252,7 +256,12 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
                     __syncthreads();
                 }
                 if (ri.self) {
-                    if (tid == 0) self_project(sl, x4, xp4);
+                    if (tid == 0) {
+                        if (ri.selfOrder)
+                            self_project_coloured_serial(sl, x4, xp4);
+                        else
+                            self_project(sl, x4, xp4);
+                    }
                     __syncthreads();
                 }
             }
@@ -324,8 +333,11 @@ __host__ __device__ inline TiledLayout tiled_layout(int nP, int nF, int nS, int 
 // TP: trips over particles and frames; TC: trips per colour per family.
 // A lane group of G lanes covers up to TP*G particles and frames and TC*G
 // constraints of each colour; the host sizes G per rod to satisfy that.
-// SELF: compiled with rod self-collision; batches without it run the variant
-// without, whose smaller register footprint keeps two waves per SIMD.
+// SELF: 0, compiled without rod self-collision (batches without it run this
+// variant, whose smaller register footprint keeps two waves per SIMD); 1, with
+// self-collision in the reference's list order; 2, also with the coloured
+// order for the rods that ask for it (batches holding such a rod run this
+// variant, so the others carry none of its code).
 //
 // In FP32, shapes with TP <= 2 are held to 256 registers, so two waves fit
 // on a SIMD (the self-collision variant spills a few bytes to do so), and TP
@@ -335,7 +347,7 @@ __host__ __device__ inline TiledLayout tiled_layout(int nP, int nF, int nS, int 
 constexpr int tiled_max_lanes(int tp) { return (!RP_FP64 && tp == 1) ? 512 : 256; }
 constexpr int tiled_waves_per_simd(int tp) { return (!RP_FP64 && tp <= 2) ? 2 : 1; }
 
-template <int TP, int TC, bool SELF>
+template <int TP, int TC, int SELF>
 __global__ __launch_bounds__(tiled_max_lanes(TP)) __attribute__((amdgpu_waves_per_eu(tiled_waves_per_simd(TP)))) void
 k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slots) {
     extern __shared__ real4 smem4[];
@@ -386,6 +398,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     // Any lane group in the workgroup may run self-collision; the barriers
     // around it must be reached by every lane of the workgroup.
     const bool anySelf = SELF && __syncthreads_or(ri.self) != 0;
+    const bool anyColoured = SELF == 2 && anySelf && __syncthreads_or(ri.self && ri.selfOrder) != 0;
     const bool anyPrim = __syncthreads_or(nPrim > 0) != 0;
 
     const int p0 = ri.pStart, f0 = ri.fStart;
@@ -518,6 +531,10 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                     self_reset(sl);
                 }
                 __syncthreads();
+                if (anyColoured && regen) {
+                    if (ri.self && ri.selfOrder && live && l == 0) self_colour(sl, nP);
+                    __syncthreads();
+                }
             }
             // ---- predict
 #pragma unroll
@@ -587,7 +604,8 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                     __syncthreads();
                 }
                 if (anySelf) {
-                    if (ri.self && l == 0 && live) self_project(sl, x4, xp4);
+                    if (ri.self && !ri.selfOrder && l == 0 && live) self_project(sl, x4, xp4);
+                    if (anyColoured) self_project_coloured(sl, x4, xp4, ri.self && ri.selfOrder && live, l, G);
                     __syncthreads();
                 }
             }
@@ -671,13 +689,16 @@ hipError_t launch_plain(const DeviceArrays& d, const StepArgs& sa, size_t lds, h
 }
 
 template <int TP, int TC>
-static hipError_t launch_tiled_t(bool self, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
+static hipError_t launch_tiled_t(int self, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
                                  const TileSlot* slots, int numTiles, int threads, size_t lds, hipStream_t s) {
-    if (self)
-        hipLaunchKernelGGL((k_tiled_step<TP, TC, true>), dim3(numTiles), dim3(threads), lds, s, d, sa, tiles, slots);
+    if (self == 2)
+        hipLaunchKernelGGL((k_tiled_step<TP, TC, 2>), dim3(numTiles), dim3(threads), lds, s, d, sa, tiles, slots);
+    else if (self == 1)
+        hipLaunchKernelGGL((k_tiled_step<TP, TC, 1>), dim3(numTiles), dim3(threads), lds, s, d, sa, tiles, slots);
+    else if (self == 0)
+        hipLaunchKernelGGL((k_tiled_step<TP, TC, 0>), dim3(numTiles), dim3(threads), lds, s, d, sa, tiles, slots);
     else
-        hipLaunchKernelGGL((k_tiled_step<TP, TC, false>), dim3(numTiles), dim3(threads), lds, s, d, sa, tiles,
-                           slots);
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -695,9 +716,10 @@ static hipError_t allow_lds(const void* kernel, int bytes, int* granted) {
 
 template <int TP, int TC>
 static hipError_t allow_tiled_lds(int bytes, int* granted) {
-    const hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_tiled_step<TP, TC, true>), bytes, granted);
+    hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_tiled_step<TP, TC, 2>), bytes, granted);
+    if (e == hipSuccess) e = allow_lds(reinterpret_cast<const void*>(&k_tiled_step<TP, TC, 1>), bytes, granted);
     if (e != hipSuccess) return e;
-    return allow_lds(reinterpret_cast<const void*>(&k_tiled_step<TP, TC, false>), bytes, granted);
+    return allow_lds(reinterpret_cast<const void*>(&k_tiled_step<TP, TC, 0>), bytes, granted);
 }
 
 // Let every stepping kernel take up to `bytes` of LDS per workgroup, static
@@ -718,7 +740,7 @@ hipError_t allow_dynamic_lds(int bytes, int* granted) {
 
 // Trip shapes compiled into the library; (TP, TC) pairs. In FP64 the
 // two-trip shapes are left out: their state does not fit the register file.
-hipError_t launch_tiled(int tp, int tc, bool self, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
+hipError_t launch_tiled(int tp, int tc, int self, const DeviceArrays& d, const StepArgs& sa, const Tile* tiles,
                         const TileSlot* slots, int numTiles, int threads, size_t lds, hipStream_t s) {
     if (threads < 64 || threads % 64 != 0 || threads > tiled_max_lanes(tp)) return hipErrorInvalidValue;
     if (tp == 1 && tc == 1) return launch_tiled_t<1, 1>(self, d, sa, tiles, slots, numTiles, threads, lds, s);

@@ -39,7 +39,8 @@ struct RodInfoT {
     R margin;  // contact margin (margin radii * radius)
     R selfFriction;
     int selfChain;  // consecutive segments (stretch constraints in index order) share a particle
-    int pad[2];
+    int selfOrder;  // 0: self contacts projected in list order; 1: coloured order
+    int pad;
 };
 
 template <typename R>

@@ -216,7 +216,8 @@ struct SelfLds {
     int* segStart;   // nS
     int* cellStart;  // table + 1
     real* pool;     // cap * kSelfFloats
-    int* misc;       // [0] live, [1] overflow, [2] long segments, [4..7] the first four of them (nS: none)
+    int* misc;       // [0] live, [1] overflow, [2] long segments, [3] colours in use (coloured order),
+                     // [4..7] the first four long segments (nS: none)
     rbits* rmisc;    // [0] cell size, [1] longest segment, [2] largest |coordinate| of a centre, [3] shortest
                      // segment, [4] longest segment that is not long, as bits (non-negative: ordered as
                      // integers)
@@ -946,6 +947,84 @@ __device__ __forceinline__ void self_project_one(const SelfLds& s, real4* x4, co
 __device__ inline void self_project(const SelfLds& s, real4* x4, const real4* xp4) {
     const int live = s.misc[0];
     for (int ci = 0; ci < live; ++ci) self_project_one(s, x4, xp4, ci);
+}
+
+// ------------------------------------------------------------------ coloured self-contact order
+//
+// An opt-in order (rp_world_desc::self_order = 1) with no counterpart in the
+// reference, which projects self contacts one at a time in list order. After
+// each regeneration the list is coloured greedily in list order: a contact
+// takes the lowest of kSelfColours colours that no earlier contact sharing one
+// of its particles holds, or the sequential tail (colour kSelfColours) when all
+// are taken. The contacts are then projected colour by colour, in list order
+// within a colour, the tail last. Contacts of one colour share no particle, so
+// the lanes of a rod's group may project a colour's contacts at once: the result
+// is that of projecting them one at a time, bit for bit.
+
+constexpr int kSelfColours = 32;
+
+__device__ __forceinline__ int self_colour_of(const SelfLds& s, int c) {
+    return slot_int(s.pool[c * kSelfFloats + 15]);
+}
+
+// The colouring, on one lane. A contact's colour goes in its record's last
+// word. misc[3] receives the number of colours in use below the tail, plus 64
+// when the tail is not empty. The per-particle colour masks take the
+// broadphase's per-segment arrays (4 nS ints from cellOf, at least nP; free once
+// the list is built).
+__device__ inline void self_colour(const SelfLds& s, int nP) {
+    unsigned* used = reinterpret_cast<unsigned*>(s.cellOf);
+    for (int p = 0; p < nP; ++p) used[p] = 0u;
+    const int live = s.misc[0];
+    int ncol = 0, tail = 0;
+    for (int c = 0; c < live; ++c) {
+        real* rec = s.pool + c * kSelfFloats;
+        const int i0 = slot_int(rec[0]), i1 = slot_int(rec[1]), i2 = slot_int(rec[2]), i3 = slot_int(rec[3]);
+        const unsigned mask = used[i0] | used[i1] | used[i2] | used[i3];
+        const int col = mask == 0xffffffffu ? kSelfColours : __builtin_ctz(~mask);
+        if (col < kSelfColours) {
+            const unsigned bit = 1u << col;
+            used[i0] |= bit;
+            used[i1] |= bit;
+            used[i2] |= bit;
+            used[i3] |= bit;
+            ncol = col + 1 > ncol ? col + 1 : ncol;
+        } else {
+            tail = 64;
+        }
+        rec[15] = int_slot(col);
+    }
+    s.misc[3] = ncol | tail;
+}
+
+// The coloured projection by the G lanes of a rod's group: one round per
+// colour, lane l taking the colour's contacts at positions l, l + G, ...; then
+// the tail on lane 0, in list order. Every lane of the workgroup calls this (a
+// barrier ends each round); `on`: this lane's rod projects in coloured order.
+__device__ inline void self_project_coloured(const SelfLds& s, real4* x4, const real4* xp4, bool on, int l, int G) {
+    const int info = on ? s.misc[3] : 0;
+    const int ncol = info & 63;
+    const int live = on ? s.misc[0] : 0;
+    for (int col = 0; __syncthreads_or(col < ncol) != 0; ++col) {
+        if (col < ncol)
+            for (int c = l; c < live; c += G)
+                if (self_colour_of(s, c) == col) self_project_one(s, x4, xp4, c);
+    }
+    if ((info & 64) && l == 0)
+        for (int c = 0; c < live; ++c)
+            if (self_colour_of(s, c) == kSelfColours) self_project_one(s, x4, xp4, c);
+}
+
+// The same order on one thread.
+__device__ inline void self_project_coloured_serial(const SelfLds& s, real4* x4, const real4* xp4) {
+    const int info = s.misc[3];
+    const int live = s.misc[0];
+    for (int col = 0; col < (info & 63); ++col)
+        for (int c = 0; c < live; ++c)
+            if (self_colour_of(s, c) == col) self_project_one(s, x4, xp4, c);
+    if (info & 64)
+        for (int c = 0; c < live; ++c)
+            if (self_colour_of(s, c) == kSelfColours) self_project_one(s, x4, xp4, c);
 }
 
 }  // namespace RP_PREC

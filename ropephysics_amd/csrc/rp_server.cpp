@@ -364,7 +364,7 @@ int rp_server_submit(void* server, const rp_batch_desc* req, const rp_request_st
     RP_HIP(hipStreamSynchronize(s.stream));
     for (size_t r = 0; r < R; ++r) {
         s.rods[size_t(r0) + r] = recs[r];
-        s.selfOf[size_t(r0) + r] = recs[r].self ? 1 : 0;
+        s.selfOf[size_t(r0) + r] = recs[r].self ? (recs[r].selfOrder ? 2 : 1) : 0;
     }
     std::copy(rec.colStart.begin(), rec.colStart.end(), s.colStart.begin() + c0);
     q.id = s.nextId++;
@@ -380,11 +380,14 @@ int rp_server_step(void* server, int* num_finished) {
     rp_server_impl& s = *S(server);
     if (num_finished) *num_finished = 0;
     std::vector<int> lean, self;
+    bool coloured = false;  // a running rod asks for the coloured self order
     for (const auto& kv : s.requests) {
         const Request& q = kv.second;
         if (q.stepsDone >= q.steps) continue;
-        for (long long r = q.start[kRods]; r < q.start[kRods] + q.count[kRods]; ++r)
+        for (long long r = q.start[kRods]; r < q.start[kRods] + q.count[kRods]; ++r) {
             (s.selfOf[size_t(r)] ? self : lean).push_back(int(r));
+            coloured = coloured || s.selfOf[size_t(r)] == 2;
+        }
     }
     if (lean.empty() && self.empty()) return RP_OK;
     const long long R = s.pools[kRods].capacity();
@@ -415,14 +418,14 @@ int rp_server_step(void* server, int* num_finished) {
         const TileConfig& cfg = tile_config(s.configSelf);
         RP_HIP(hipEventRecord(s.evReady, s.stream));
         RP_HIP(hipStreamWaitEvent(s.selfStream, s.evReady, 0));
-        RP_HIP(launch_tiled(cfg.tp, cfg.tc, true, s.d, a, s.dTiles + R, s.dSlots + R,
+        RP_HIP(launch_tiled(cfg.tp, cfg.tc, coloured ? 2 : 1, s.d, a, s.dTiles + R, s.dSlots + R,
                             int(s.packSelf.tiles.size()), 64 * cfg.waves, size_t(s.packSelf.maxLds), s.selfStream));
         RP_HIP(hipEventRecord(s.evSelfDone, s.selfStream));
         ++s.launches;
     }
     if (!lean.empty()) {
         const TileConfig& cfg = tile_config(s.configLean);
-        RP_HIP(launch_tiled(cfg.tp, cfg.tc, false, s.d, a, s.dTiles, s.dSlots, int(s.packLean.tiles.size()),
+        RP_HIP(launch_tiled(cfg.tp, cfg.tc, 0, s.d, a, s.dTiles, s.dSlots, int(s.packLean.tiles.size()),
                             64 * cfg.waves, size_t(s.packLean.maxLds), s.stream));
         ++s.launches;
     }

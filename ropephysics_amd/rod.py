@@ -432,6 +432,10 @@ class CollisionWorld:
     # beyond it are counted as overflow.
     self_capacity: int = 0
     self_capacity_per_segment: float = 0.0
+    # Order of the self-contact projection: 0, one at a time in list order (the
+    # reference's); 1, the coloured order (an opt-in with no counterpart in the
+    # reference: greedy colours over the list, a colour's contacts at once).
+    self_order: int = 0
 
 
 @dataclass

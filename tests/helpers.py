@@ -31,7 +31,8 @@ def oracle_world(w: rp.CollisionWorld | None):
     prims = [oracle.make_prim(p.type, p.a, p.b, p.normal, p.half_extents, p.rotation, p.radius, p.friction)
              for p in w.primitives]
     return oracle.World(prims=prims, self_collision=w.self_collision, self_friction=w.self_friction,
-                        self_skip=w.self_skip, margin_radii=w.contact_margin_radii, hash_table_size=w.hash_table_size)
+                        self_skip=w.self_skip, margin_radii=w.contact_margin_radii, hash_table_size=w.hash_table_size,
+                        self_order=w.self_order)
 
 
 def oracle_params(p: rp.SolverParams, coloured=True, compliance_scale=1.0, friction_scale=1.0, precision=32):
