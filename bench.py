@@ -227,6 +227,44 @@ def trajectory_leg(w: scenes.Workload, every: int, warmup: int) -> dict:
     return ent
 
 
+def routing_coloured_leg(w: scenes.Workload, steps: int, warmup: int, gv) -> tuple:
+    """The routing workload with every cable's self contacts in the coloured
+    order (an opt-in of this build; the reference projects them in list
+    order): throughput, the longest launches, and routed verdicts against the
+    list-order FP32 rollout's."""
+    import dataclasses
+
+    world = dataclasses.replace(w.world, self_order=1)
+    b = rp.Batch(w.rods, strategy="tiled", world=world)
+    b.set_material_scales(w.youngs_scale, w.friction_scale)
+    b.set_paths(0, w.info["paths"])
+    reset_ragged(b, w)
+    b.step(w.params, warmup)
+    b.synchronize()
+    reset_ragged(b, w)
+    b.set_timing(True)
+    secs, c = time_steps(b, w.params, steps)
+    work = float(w.segments_total) * w.params.substeps * steps
+    launches = max(1, c["last_step_launches"])
+    ent = {"value": work / secs, "unit": "segment-substeps/s", "dtype": "f32", "steps": steps, "seconds": secs,
+           "kernel_ms_avg": c["last_step_gpu_ms"] / launches, "launches": launches,
+           "tile_config": c.get("tile_config_name"), "self_contact_overflow": c["self_contact_overflow"]}
+    verdicts = None
+    if steps == w.steps:
+        task = w.info["task"]
+        verdicts = [task.evaluate(b.download(i)["x"])["routed"] for i in range(w.num_rods)]
+        ent["routed"] = int(sum(verdicts))
+        if gv is not None:
+            ent["verdicts_vs_list_order_gpu"] = {"cables": w.num_rods,
+                                                 "agree": int(sum(1 for a, g in zip(verdicts, gv) if a == g))}
+    log(f"routing, coloured self order {ent['tile_config']}: {secs:.3f} s for {steps} steps, "
+        f"{ent['value'] / 1e9:.3f} G seg-substeps/s" +
+        (f"; verdicts equal to the list order's on {ent['verdicts_vs_list_order_gpu']['agree']} of {w.num_rods}"
+         if "verdicts_vs_list_order_gpu" in ent else ""))
+    b.destroy()
+    return ent, verdicts
+
+
 def cpu_baseline_routing(w: scenes.Workload, stride: int, steps: int, threads: int, coloured: bool = False) -> dict:
     """The reference CPU path (its own sources, built into oracle/_ref) on every
     `stride`-th cable of the routing workload over its first `steps` steps, FP64,
@@ -521,6 +559,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-routing", type=int, default=1, help="also run the routing workload in FP64 (0: skip)")
+    ap.add_argument("--coloured-routing", type=int, default=1,
+                    help="also run the routing workload with the coloured self-contact order (0: skip)")
     ap.add_argument("--drape-steps", type=int, default=None, help="steps of the hold/slide sweep (default: the "
                                                                    "whole 8 s = 16000; 0: skip)")
     ap.add_argument("--drape-cpu-rods", type=int, default=32, help="cables of the CPU verdict spot check")
@@ -569,6 +609,7 @@ def main():
                     "routing" if not args.tile_config else None)
 
     r64, v64 = routing_fp64_leg(w, K, W) if args.fp64_routing else (None, None)
+    rcol, vcol = routing_coloured_leg(w, K, W, leg.get("_verdicts")) if args.coloured_routing else (None, None)
     uniform = uniform_leg(args.uniform_steps, W, args.no_plain) if args.uniform_steps > 0 else None
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     drape = None
@@ -590,6 +631,11 @@ def main():
                                               "routed_gpu": sum(1 for r in cv if gv[r]),
                                               "routed_cpu": sum(1 for v in cv.values() if v)}
             log(f"routed verdicts, GPU FP32 vs reference CPU FP64 on {len(cv)} cables: {agree} agree")
+            if vcol is not None:
+                rcol["verdicts_vs_reference_cpu"] = {"sampled_cables": len(cv),
+                                                     "agree": sum(1 for r, v in cv.items() if vcol[r] == v)}
+                log(f"routed verdicts, coloured self order vs reference CPU FP64 (list order) on {len(cv)} "
+                    f"cables: {rcol['verdicts_vs_reference_cpu']['agree']} agree")
         if v64 is not None and K == w.steps:
             col = cpu_baseline_routing(w, args.cpu_stride, w.steps, threads, coloured=True)
             cc = col.pop("_verdicts")
@@ -616,6 +662,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "routing_fp64": r64,
+        "routing_coloured_self_order": rcol,
         "uniform": uniform,
         "drape": drape,
         "harness_learning": cem,

@@ -189,7 +189,7 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
                 }
                 __syncthreads();
                 if (regen && ri.selfOrder) {
-                    if (tid == 0) self_colour(sl, nP);
+                    if (tid == 0) self_colour(sl);
                     __syncthreads();
                 }
             }
@@ -489,6 +489,9 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     const v3 g = mk(sa.gx, sa.gy, sa.gz);
     const real h = sa.h;
     int clk = active ? d.clock[rod] : 0;
+    // Whether some rod of the workgroup had self contacts at the last
+    // regeneration (the same on every lane).
+    bool wgContacts = false;
     // A rod whose clock has reached its step limit stops changing; its lanes
     // still take part in every barrier.
     const int limit = (active && d.stepLimit) ? d.stepLimit[rod] : 0x7fffffff;
@@ -525,14 +528,15 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
             }
             if (anySelf) {
                 if (regen) {
-                    self_regen_group<TP>(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap, ri.radius,
-                                         ri.selfFriction * fscale, ri.selfChain != 0, ri.self != 0 && live, l, G);
+                    wgContacts = self_regen_group<TP>(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap,
+                                                      ri.radius, ri.selfFriction * fscale, ri.selfChain != 0,
+                                                      ri.self != 0 && live, l, G);
                 } else if (ri.self && l == 0 && live) {
                     self_reset(sl);
                 }
                 __syncthreads();
-                if (anyColoured && regen) {
-                    if (ri.self && ri.selfOrder && live && l == 0) self_colour(sl, nP);
+                if (anyColoured && regen && wgContacts) {
+                    if (ri.self && ri.selfOrder && live && l == 0) self_colour(sl);
                     __syncthreads();
                 }
             }
@@ -605,7 +609,8 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                 }
                 if (anySelf) {
                     if (ri.self && !ri.selfOrder && l == 0 && live) self_project(sl, x4, xp4);
-                    if (anyColoured) self_project_coloured(sl, x4, xp4, ri.self && ri.selfOrder && live, l, G);
+                    if (anyColoured && wgContacts)
+                        self_project_coloured(sl, x4, xp4, ri.self && ri.selfOrder && live, l, G);
                     __syncthreads();
                 }
             }

@@ -704,8 +704,10 @@ __device__ inline void self_build_tables(const SelfLds& s, int nS, int table, re
     __syncthreads();
 }
 
+// Returns whether some rod of the workgroup has contacts (the same on every
+// lane).
 template <int TP>
-__device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
+__device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
                                         int gap, int cap, real radius, real friction, bool chain, bool on, int l,
                                         int G) {
     real4 cj[TP];
@@ -792,7 +794,7 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
         s.rmisc[0] = real_bits(cell);
         self_list_reset(s, nS);
     }
-    if (!any) return;
+    if (!any) return false;
     // The bucket tables are needed only for a segment with more contacts than
     // it keeps.
     bool over = kM == 0;
@@ -819,6 +821,7 @@ __device__ inline void self_regen_group(const SelfLds& s, const real4* x4, const
         self_fill_rows(s, x4, segEnd, nS, table, gap, cap, radius, friction, cell, j, segEnd[2 * j],
                        segEnd[2 * j + 1], cj[t], s.segStart[j]);
     }
+    return true;
 }
 
 // Counts -> pool offsets; contacts past the capacity are tallied as overflow.
@@ -971,11 +974,14 @@ __device__ __forceinline__ int self_colour_of(const SelfLds& s, int c) {
 // word. misc[3] receives the number of colours in use below the tail, plus 64
 // when the tail is not empty. The per-particle colour masks take the
 // broadphase's per-segment arrays (4 nS ints from cellOf, at least nP; free once
-// the list is built).
-__device__ inline void self_colour(const SelfLds& s, int nP) {
+// the list is built); only the contacts' particles are cleared and read.
+__device__ inline void self_colour(const SelfLds& s) {
     unsigned* used = reinterpret_cast<unsigned*>(s.cellOf);
-    for (int p = 0; p < nP; ++p) used[p] = 0u;
     const int live = s.misc[0];
+    for (int c = 0; c < live; ++c) {
+        const real* rec = s.pool + c * kSelfFloats;
+        for (int m = 0; m < 4; ++m) used[slot_int(rec[m])] = 0u;
+    }
     int ncol = 0, tail = 0;
     for (int c = 0; c < live; ++c) {
         real* rec = s.pool + c * kSelfFloats;
