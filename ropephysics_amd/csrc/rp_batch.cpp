@@ -85,7 +85,13 @@ struct rp_batch_impl {
     std::vector<int> recOrigin;
 
     std::unique_ptr<PackedTiles> tiles;
-    int stepsPerLaunch = 64;
+    // TILED: the most steps one launch runs. A launch lasts as long as its
+    // slowest workgroup, so a call's steps go into as few launches as this
+    // allows: the bursts of work of different rods (self-contact folds, stretched
+    // segments) then overlap instead of each setting one short launch's length.
+    int stepsPerLaunch = 8192;
+    // PLAIN: launches (one step each) captured per graph.
+    int plainStepsPerGraph = 64;
     long long stepsTaken = 0;
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::vector<int> colStart;  // host copy of the colour-start table
@@ -578,10 +584,10 @@ int step(void* batch, const typename Prec<Real>::StepParams* params, int num_ste
         return fail(RP_ERR_INVALID, "need substeps >= 1, iterations >= 0 and dt > 0");
     if (num_steps == 0) return RP_OK;
     if (b.strategy == RP_STRATEGY_TILED && (rc = resolve_tiles(b, p))) return rc;
-    // Steps run in chunks of stepsPerLaunch: one TILED launch, or that many
-    // PLAIN launches. Each chunk shape is captured once into a HIP graph
-    // (cached per parameter set) and replayed.
-    const int chunk = b.stepsPerLaunch;
+    // Steps run in chunks: one TILED launch of at most stepsPerLaunch steps,
+    // or plainStepsPerGraph PLAIN launches. Each chunk shape is captured once
+    // into a HIP graph (cached per parameter set and length) and replayed.
+    const int chunk = b.strategy == RP_STRATEGY_PLAIN ? b.plainStepsPerGraph : b.stepsPerLaunch;
     const int full = num_steps / chunk, rem = num_steps % chunk;
     hipGraphExec_t gFull = nullptr, gRem = nullptr;
     if (b.graphs.size() > 62) {  // bound the cache; a call adds at most two graphs
@@ -633,8 +639,7 @@ int step(void* batch, const typename Prec<Real>::StepParams* params, int num_ste
         RP_HIP(hipEventRecord(b.ev1, b.stream));
         b.evPending = true;
     }
-    b.lastLaunches = b.strategy == RP_STRATEGY_PLAIN ? num_steps
-                                                     : (num_steps + b.stepsPerLaunch - 1) / b.stepsPerLaunch;
+    b.lastLaunches = b.strategy == RP_STRATEGY_PLAIN ? num_steps : full + (rem ? 1 : 0);
     b.stepsTaken += num_steps;
     return RP_OK;
 }

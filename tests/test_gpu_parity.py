@@ -324,6 +324,23 @@ def test_plain_equals_tiled_and_runs_repeat_bitwise():
             assert np.array_equal(bits(o[k]), bits(outs[0][k]))
 
 
+def test_launch_length_does_not_change_the_bits():
+    """TILED runs a call's steps in one launch (up to its steps-per-launch
+    bound): 60 steps in one launch, in calls of 7 and in calls of one step give
+    the same bits, with world contacts and self-collision on."""
+    rods = [helix_rope(), benchmark_rod(24), benchmark_rod(101)]
+    outs = []
+    for calls in ([60], [7] * 8 + [4], [1] * 60):
+        b = rp.Batch(rods, strategy="tiled", world=SELF_WORLD)
+        for k in calls:
+            b.step(SELF_PARAMS, k)
+        assert b.counters()["last_step_launches"] == 1
+        outs.append(b.get_state())
+    for o in outs[1:]:
+        for k in o:
+            assert np.array_equal(bits(o[k]), bits(outs[0][k])), k
+
+
 def test_full_size_uniform_batch_sampled_rods_bitwise():
     """The 4096 x 64 floor workload at full size for 30 steps: a sample of rods
     against the FP32 oracle, bit for bit; every rod finite and above the floor."""
