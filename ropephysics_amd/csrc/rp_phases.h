@@ -362,8 +362,7 @@ __device__ inline void self_query(const SelfLds& s, const real4* x4, const int* 
 // group's lanes are consecutive -- when l >= off), then across the wavefronts
 // the group spans through `scratch`, one entry per wavefront. Returns the
 // total. Two barriers, whatever G, n or `on`.
-template <bool kVote>
-__device__ inline int group_scan_t(const int* in, int* out, int n, int* scratch, int l, int G, bool on, bool& any) {
+__device__ inline int group_scan(const int* in, int* out, int n, int* scratch, int l, int G, bool on) {
     const int chunk = on ? (n + G - 1) / G : 0;
     const int lo = on ? min(n, l * chunk) : 0;
     const int hi = on ? min(n, lo + chunk) : 0;
@@ -394,23 +393,8 @@ __device__ inline int group_scan_t(const int* in, int* out, int n, int* scratch,
         out[i] = run;
         run += v;
     }
-    if (kVote)
-        any = __syncthreads_or(on && total > 0) != 0;
-    else
-        __syncthreads();
+    __syncthreads();
     return total;
-}
-
-__device__ inline int group_scan(const int* in, int* out, int n, int* scratch, int l, int G, bool on) {
-    bool any = false;
-    return group_scan_t<false>(in, out, n, scratch, l, G, on, any);
-}
-
-// The same, with the final barrier a vote: `any` tells whether some rod of the
-// workgroup has a positive total.
-__device__ inline int group_scan_vote(const int* in, int* out, int n, int* scratch, int l, int G, bool on,
-                                      bool& any) {
-    return group_scan_t<true>(in, out, n, scratch, l, G, on, any);
 }
 
 __device__ __forceinline__ real abs_max3(v3 a) { return r_max(r_abs(a.x), r_max(r_abs(a.y), r_abs(a.z))); }
@@ -774,27 +758,39 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
     constexpr int kMr = kM > 0 ? kM : 1;
     int kk[TP][kMr];
     int nk[TP];
+    bool counted = false;
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
         nk[t] = 0;
-        if (on && j < nS)
-            s.segCount[j] = nS > gap ? self_count_scan<kM>(s, x4, segEnd, listed && numLong > 0, nS, table, gap,
-                                                            radius, cell, lenCap, scaleMax, chain, j, segEnd[2 * j],
-                                                            segEnd[2 * j + 1], cj[t], kk[t], nk[t])
-                                     : 0;
+        if (on && j < nS) {
+            const int c = nS > gap ? self_count_scan<kM>(s, x4, segEnd, listed && numLong > 0, nS, table, gap,
+                                                          radius, cell, lenCap, scaleMax, chain, j, segEnd[2 * j],
+                                                          segEnd[2 * j + 1], cj[t], kk[t], nk[t])
+                                   : 0;
+            s.segCount[j] = c;
+            counted = counted || c > 0;
+        }
     }
-    __syncthreads();
-    // The fill pass runs only where some rod of the workgroup has contacts.
-    bool any = false;
-    const int total = group_scan_vote(s.segCount, s.segStart, on ? nS : 0, s.scan, l, G, on, any);
+    // The barrier after the counts is a vote: when no rod of the workgroup has
+    // a contact (the common case) the pool offsets are not summed and the fill
+    // does not run.
+    const bool any = __syncthreads_or(counted) != 0;
+    if (!any) {
+        if (on && l == 0) {
+            s.misc[0] = 0;
+            s.rmisc[0] = real_bits(cell);
+            self_list_reset(s, nS);
+        }
+        return false;
+    }
+    const int total = group_scan(s.segCount, s.segStart, on ? nS : 0, s.scan, l, G, on);
     if (on && l == 0) {
         s.misc[0] = total < cap ? total : cap;
         if (total > cap) s.misc[1] += total - cap;
         s.rmisc[0] = real_bits(cell);
         self_list_reset(s, nS);
     }
-    if (!any) return false;
     // The bucket tables are needed only for a segment with more contacts than
     // it keeps.
     bool over = kM == 0;
