@@ -613,13 +613,42 @@ __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const i
 // Inclusive max (kMin: min) of v over the lanes of this lane's group in its
 // wavefront, in wavefront order; the group's last lane in the wavefront ends
 // with the value of the whole stretch.
+//
+// The scan runs on DPP lane moves, in registers: four row shifts (1, 2, 4, 8
+// lanes) within each row of 16 lanes, then the last lane of row 0 and 2 into
+// rows 1 and 3, then lane 31 into rows 2 and 3. A lane whose source is outside
+// its row, or in a row the move does not write, receives its own value. A move
+// is taken only when its source lane belongs to the same group.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ rbits dpp_move(rbits v) {
+    if constexpr (sizeof(rbits) == 4) {
+        return __builtin_amdgcn_update_dpp(v, v, kCtrl, kRowMask, 0xF, false);
+    } else {
+        const int lo = int(static_cast<unsigned long long>(v) & 0xffffffffull);
+        const int hi = int(static_cast<unsigned long long>(v) >> 32);
+        const unsigned lo2 = unsigned(__builtin_amdgcn_update_dpp(lo, lo, kCtrl, kRowMask, 0xF, false));
+        const unsigned hi2 = unsigned(__builtin_amdgcn_update_dpp(hi, hi, kCtrl, kRowMask, 0xF, false));
+        return rbits((static_cast<unsigned long long>(hi2) << 32) | lo2);
+    }
+}
+
 template <bool kMin>
 __device__ __forceinline__ rbits group_wave_extreme(rbits v, int l) {
-    const int lane = int(threadIdx.x);
-    for (int off = 1; off < 64; off <<= 1) {
-        const rbits u = __shfl_up(v, off, 64);
-        if (l >= off && (lane & 63) >= off) v = kMin ? (u < v ? u : v) : (u > v ? u : v);
-    }
+    const int lane = int(threadIdx.x) & 63;
+    const int r = lane & 15;
+    const auto pick = [](rbits a, rbits b) { return kMin ? (b < a ? b : a) : (b > a ? b : a); };
+    rbits u = dpp_move<0x111, 0xF>(v);  // row_shr:1
+    if (l >= 1) v = pick(v, u);
+    u = dpp_move<0x112, 0xF>(v);  // row_shr:2
+    if (l >= 2) v = pick(v, u);
+    u = dpp_move<0x114, 0xF>(v);  // row_shr:4
+    if (l >= 4) v = pick(v, u);
+    u = dpp_move<0x118, 0xF>(v);  // row_shr:8
+    if (l >= 8) v = pick(v, u);
+    u = dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3: the row before's last lane
+    if (l > r) v = pick(v, u);
+    u = dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3: lane 31
+    if (l > lane - 32) v = pick(v, u);
     return v;
 }
 
