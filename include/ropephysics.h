@@ -194,7 +194,10 @@ int rp_batch_set_loads(void* batch, int first_rod, int num_rods, const float* fo
 int rp_batch_set_scales(void* batch, const float* youngs_scale, const float* friction_scale);
 
 /* Batch::step (src/gpu/gpu_batch.cpp:457-473), num_steps times, asynchronous
- * on the batch's stream. */
+ * on the batch's stream. TILED runs the call's steps in as few launches as
+ * possible (at most 8192 steps each; RP_STEPS_PER_LAUNCH in the environment at
+ * creation overrides the bound); PLAIN launches once per step. Results do not
+ * depend on how steps are split into calls or launches. */
 int rp_batch_step(void* batch, const rp_step_params* params, int num_steps);
 
 /* Batch::synchronize (src/gpu/gpu_batch.cpp:475), scoped to this batch. */
