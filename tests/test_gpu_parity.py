@@ -277,6 +277,35 @@ def test_self_collision_lane_groups_across_wavefronts(tile_config):
     assert_bitwise(diffs)
 
 
+def test_self_collision_longest_segment_in_every_row():
+    """The broadphase's cell is the rod's longest segment plus 2r, found by a
+    segmented max over each lane group's lanes within a wavefront (row moves
+    of 1-8 lanes, then across 16-lane rows). Coils with one segment three
+    times longer than the rest, at positions that land in different rows of a
+    wavefront, packed one particle per lane so lane groups start mid-wavefront;
+    small hash tables make the contact list depend on the exact cell. Bitwise
+    equal to the oracle."""
+    world = rp.CollisionWorld(primitives=[rp.Primitive.plane((0, 0, 0), (0, 0, 1), 0.6)], self_collision=True,
+                              self_friction=0.3, hash_table_size=64, self_capacity=700)
+    base = np.array(pinned_coil().x, dtype=np.float64)
+    rods = [benchmark_rod(24)]
+    for m in (2, 13, 21, 34, 45, 58, 71):
+        pts = base.copy()
+        t = pts[m + 1] - pts[m]
+        pts[m + 1:] += 2.0 * t  # segment m three times its neighbours' length
+        rod = rp.make_rod_from_centerline(pts, rp.RodMaterial(2e6, 0.35, 1000.0, 5e-3), (0, 0, 1))
+        rod.pin_particle(0)
+        rod.pin_particle(rod.num_segments)
+        rods.append(rod)
+    diffs, _, b = run_pair(rods, world, SELF_PARAMS, 8, "tiled", checkpoints=[2, 8], tile_config=12)
+    assert b.counters()["tile_config"] == 12
+    assert b.self_contact_overflow() == 0
+    assert_bitwise(diffs)
+    o = oracle_twin(rods[4], 32)
+    o.step(oracle_params(SELF_PARAMS), oracle_world(world), 2)
+    assert o.contact_counts()[1] >= 10  # the coils are in self contact
+
+
 @pytest.mark.parametrize("strategy", STRATEGIES)
 def test_ragged_batch_and_alone_equals_mixed(strategy):
     sizes = [7, 20, 33, 64, 97, 130, 160, 48]
