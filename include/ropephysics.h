@@ -155,7 +155,8 @@ typedef struct rp_counters {
     int lanes_busy;                  /* lanes assigned to rods, over all workgroups */
     int lanes_total;
     size_t lds_bytes_per_workgroup;  /* largest */
-    int launches_per_step;           /* PLAIN: 1; TILED: 1 / steps per launch */
+    int launches_per_step;           /* PLAIN: 1; TILED: 0 (a call's steps share launches; see
+                                        last_step_launches) */
     long long steps_taken;
     size_t device_bytes;             /* device memory held by the batch */
     float last_step_gpu_ms;          /* with timing on: HIP-event time of the last rp_batch_step */
@@ -195,9 +196,10 @@ int rp_batch_set_scales(void* batch, const float* youngs_scale, const float* fri
 
 /* Batch::step (src/gpu/gpu_batch.cpp:457-473), num_steps times, asynchronous
  * on the batch's stream. TILED runs the call's steps in as few launches as
- * possible (at most 8192 steps each; RP_STEPS_PER_LAUNCH in the environment at
- * creation overrides the bound); PLAIN launches once per step. Results do not
- * depend on how steps are split into calls or launches. */
+ * possible: at most 8192 steps each, and about 4 s of device time as estimated
+ * from the autotuner's trial (never fewer than 64 steps); RP_STEPS_PER_LAUNCH
+ * in the environment at creation replaces both bounds. PLAIN launches once per
+ * step. Results do not depend on how steps are split into calls or launches. */
 int rp_batch_step(void* batch, const rp_step_params* params, int num_steps);
 
 /* Batch::synchronize (src/gpu/gpu_batch.cpp:475), scoped to this batch. */

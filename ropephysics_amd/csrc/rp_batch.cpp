@@ -90,6 +90,10 @@ struct rp_batch_impl {
     // allows: the bursts of work of different rods (self-contact folds, stretched
     // segments) then overlap instead of each setting one short launch's length.
     int stepsPerLaunch = 8192;
+    bool stepsPerLaunchSet = false;  // set from RP_STEPS_PER_LAUNCH: no time bound
+    // Device time of one substep sweep of the whole batch in the autotuner's
+    // best trial: a launch is also held to about kLaunchBudgetMs by it.
+    double tileMsPerSweep = 0.0;
     // PLAIN: launches (one step each) captured per graph.
     int plainStepsPerGraph = 64;
     long long stepsTaken = 0;
@@ -204,6 +208,9 @@ int upload_tiles(PackedTiles& t) {
     return put(t.dSlots, t.slots.data(), t.slots.size() * sizeof(TileSlot));
 }
 
+// Estimated device time one TILED launch is held to.
+constexpr double kLaunchBudgetMs = 4000.0;
+
 template <typename Real>
 int launch_steps(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int steps, int strategy,
                  PackedTiles* t) {
@@ -316,6 +323,7 @@ int autotune(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int only
     if (rc) return rc;
     if (!bestTiles) return fail(RP_ERR_CAPACITY, "no tile configuration fits this batch");
     b.tiles = std::move(bestTiles);
+    b.tileMsPerSweep = best / (double(trialSteps) * p.substeps * std::max(1, p.iterations));
     return RP_OK;
 }
 
@@ -401,7 +409,10 @@ int create(const typename Prec<Real>::BatchDesc* desc, void** out_batch) {
                                          " bytes of LDS; the device has " + std::to_string(b->ldsLimit) +
                                          " per workgroup");
     b->strategy = strategy;
-    if (const char* e = std::getenv("RP_STEPS_PER_LAUNCH")) b->stepsPerLaunch = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("RP_STEPS_PER_LAUNCH")) {
+        b->stepsPerLaunch = std::max(1, std::atoi(e));
+        b->stepsPerLaunchSet = true;
+    }
 
     // ---- device arrays
     RP_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
@@ -587,7 +598,14 @@ int step(void* batch, const typename Prec<Real>::StepParams* params, int num_ste
     // Steps run in chunks: one TILED launch of at most stepsPerLaunch steps,
     // or plainStepsPerGraph PLAIN launches. Each chunk shape is captured once
     // into a HIP graph (cached per parameter set and length) and replayed.
-    const int chunk = b.strategy == RP_STRATEGY_PLAIN ? b.plainStepsPerGraph : b.stepsPerLaunch;
+    int chunk = b.strategy == RP_STRATEGY_PLAIN ? b.plainStepsPerGraph : b.stepsPerLaunch;
+    if (b.strategy == RP_STRATEGY_TILED && !b.stepsPerLaunchSet && b.tileMsPerSweep > 0.0) {
+        // Hold a launch to about kLaunchBudgetMs of estimated device time (at
+        // least 64 steps), so a very large batch still returns to the host
+        // every few seconds.
+        const double perStep = b.tileMsPerSweep * p.substeps * std::max(1, p.iterations);
+        chunk = int(std::max(64.0, std::min(double(chunk), std::floor(kLaunchBudgetMs / perStep))));
+    }
     const int full = num_steps / chunk, rem = num_steps % chunk;
     hipGraphExec_t gFull = nullptr, gRem = nullptr;
     if (b.graphs.size() > 62) {  // bound the cache; a call adds at most two graphs
