@@ -20,8 +20,11 @@ FP64 with its verdicts spot-checked against the reference CPU path; the
 cross-entropy harness-learning loop (10 iterations of 1024 rollouts); and the
 stream (requests of 1-256 rods of 32-256 segments, 0.5-8 s horizons, random
 contact shapes, self-collision in half) through the continuous-batching
-server: its capacity with every request arriving at once, then p50/p99 latency
-under Poisson arrivals at a fraction of that capacity.
+server in its two modes (64-step chunks in lockstep, and 50 ms time slices in
+which each workgroup runs the steps it can): each mode's capacity with every
+request arriving at once, then p50/p99 latency of both under the same Poisson
+arrivals, at a fraction of the chunked mode's capacity. The headline p50/p99
+is the time-sliced mode's.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -314,40 +317,49 @@ def cpu_baseline_routing(w: scenes.Workload, stride: int, steps: int, threads: i
 
 # ------------------------------------------------------------------ serving stream (latency)
 
-def serving_leg(num: int, load: float, budget_gib: float) -> dict:
-    """The serving workload: `num` requests, first all at once (the server's
-    capacity in requests/s), then as a Poisson stream at `load` times that
-    rate; latency from arrival to result on the host."""
+def serving_leg(num: int, load: float, budget_gib: float, slice_us: int) -> dict:
+    """The serving workload: `num` requests through the server in its two
+    modes. Chunked steps (64 steps per server step, every resident request in
+    lockstep): its capacity with every request arriving at once sets the
+    Poisson arrival rate, `load` times that capacity. Time-sliced steps
+    (`slice_us` per server step, each workgroup running the steps it starts in
+    the slice): its own capacity, then the same Poisson stream at the same rate.
+    Latency is from arrival to result on the host."""
     from ropephysics_amd import serving
 
     reqs = serving.make_requests(num)
     work = sum(q.work for q in reqs)
-    server = rp.Server(serving.PARAMS, hbm_budget_bytes=int(budget_gib * (1 << 30)))
-    serving.run_stream(server, serving.make_requests(2, seed=5), np.zeros(2))  # code objects, first launches
-    sat = serving.run_stream(server, reqs, np.zeros(num))
-    capacity = num / sat["wall"]
-    rate = load * capacity
-    run = serving.run_stream(server, reqs, serving.poisson_arrivals(num, rate))
-    lat = run["latency"]
     horizon = np.array([q.steps * serving.PARAMS.dt for q in reqs])
-    out = {
-        "requests": num, "segments_mean": float(np.mean([q.segments for q in reqs])),
-        "rods_mean": float(np.mean([q.info["rods"] for q in reqs])),
-        "horizon_s_mean": float(horizon.mean()), "self_collision_share": float(np.mean(
-            [q.info["self_collision"] for q in reqs])),
-        "hbm_budget_bytes": server.stats()["device_bytes"],
-        "saturated": {"wall_s": sat["wall"], "value": work / sat["wall"], "unit": "segment-substeps/s",
-                      "requests_per_s": capacity, "launches": sat["launches"]},
-        "poisson": {"rate_requests_per_s": rate, "load": load, "wall_s": run["wall"], "value": work / run["wall"],
-                    "unit": "segment-substeps/s", "p50_latency_s": float(np.percentile(lat, 50)),
-                    "p99_latency_s": float(np.percentile(lat, 99)), "max_latency_s": float(lat.max()),
-                    "p50_latency_over_horizon": float(np.percentile(lat / horizon, 50)),
-                    "max_resident_requests": run["max_resident"], "launches": run["launches"]},
-    }
-    log(f"serving: {num} requests; saturated {work / sat['wall'] / 1e9:.3f} G seg-substeps/s "
-        f"({capacity:.2f} req/s); Poisson at {rate:.2f} req/s: p50 {out['poisson']['p50_latency_s']:.3f} s, "
-        f"p99 {out['poisson']['p99_latency_s']:.3f} s")
-    server.destroy()
+    out = {"requests": num, "segments_mean": float(np.mean([q.segments for q in reqs])),
+           "rods_mean": float(np.mean([q.info["rods"] for q in reqs])),
+           "horizon_s_mean": float(horizon.mean()), "self_collision_share": float(np.mean(
+               [q.info["self_collision"] for q in reqs])), "load": load}
+    rate = None
+    for mode, kw in (("chunked", {}), ("time_sliced", {"time_slice_us": slice_us})):
+        server = rp.Server(serving.PARAMS, hbm_budget_bytes=int(budget_gib * (1 << 30)), **kw)
+        serving.run_stream(server, serving.make_requests(2, seed=5), np.zeros(2))  # code objects, first launches
+        sat = serving.run_stream(server, reqs, np.zeros(num))
+        capacity = num / sat["wall"]
+        if rate is None:
+            rate = load * capacity
+        run = serving.run_stream(server, reqs, serving.poisson_arrivals(num, rate))
+        lat = run["latency"]
+        out["hbm_budget_bytes"] = server.stats()["device_bytes"]
+        out[mode] = {
+            "saturated": {"wall_s": sat["wall"], "value": work / sat["wall"], "unit": "segment-substeps/s",
+                          "requests_per_s": capacity, "launches": sat["launches"]},
+            "poisson": {"rate_requests_per_s": rate, "wall_s": run["wall"], "value": work / run["wall"],
+                        "unit": "segment-substeps/s", "p50_latency_s": float(np.percentile(lat, 50)),
+                        "p99_latency_s": float(np.percentile(lat, 99)), "max_latency_s": float(lat.max()),
+                        "p50_latency_over_horizon": float(np.percentile(lat / horizon, 50)),
+                        "max_resident_requests": run["max_resident"], "launches": run["launches"]},
+        }
+        if kw:
+            out[mode]["time_slice_us"] = slice_us
+        log(f"serving {mode}: {num} requests; saturated {work / sat['wall'] / 1e9:.3f} G seg-substeps/s "
+            f"({capacity:.2f} req/s); Poisson at {rate:.2f} req/s: p50 {out[mode]['poisson']['p50_latency_s']:.3f} s, "
+            f"p99 {out[mode]['poisson']['p99_latency_s']:.3f} s, {work / run['wall'] / 1e9:.3f} G seg-substeps/s")
+        server.destroy()
     return out
 
 
@@ -569,7 +581,9 @@ def main():
     ap.add_argument("--cem-iterations", type=int, default=10, help="harness-learning iterations (0: skip)")
     ap.add_argument("--cem-population", type=int, default=1024)
     ap.add_argument("--serve-requests", type=int, default=40, help="requests of the serving leg (0: skip)")
-    ap.add_argument("--serve-load", type=float, default=0.8, help="Poisson rate as a fraction of capacity")
+    ap.add_argument("--serve-load", type=float, default=0.8,
+                    help="Poisson rate as a fraction of the chunked server's capacity")
+    ap.add_argument("--serve-slice-us", type=int, default=50000, help="time slice of the time-sliced server mode")
     ap.add_argument("--serve-budget-gib", type=float, default=4.0, help="server HBM budget")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-traffic kernel for "
@@ -617,7 +631,7 @@ def main():
         drape = drape_leg(args.drape_steps, W, 0 if args.no_cpu else args.drape_cpu_rods, threads)
     traj = trajectory_leg(w, args.record_every, W) if args.record_every > 0 and K == w.steps else None
     cem = cem_leg(args.cem_iterations, args.cem_population, not args.no_cpu) if args.cem_iterations > 0 else None
-    serving_out = serving_leg(args.serve_requests, args.serve_load, args.serve_budget_gib) \
+    serving_out = serving_leg(args.serve_requests, args.serve_load, args.serve_budget_gib, args.serve_slice_us) \
         if args.serve_requests > 0 else None
 
     cpu = None
@@ -668,8 +682,8 @@ def main():
         "harness_learning": cem,
         "trajectory": traj,
         "serving": serving_out,
-        "p50_latency_s": serving_out["poisson"]["p50_latency_s"] if serving_out else None,
-        "p99_latency_s": serving_out["poisson"]["p99_latency_s"] if serving_out else None,
+        "p50_latency_s": serving_out["time_sliced"]["poisson"]["p50_latency_s"] if serving_out else None,
+        "p99_latency_s": serving_out["time_sliced"]["poisson"]["p99_latency_s"] if serving_out else None,
         "device": {"arch": info["arch"], "compute_units": info["compute_units"]},
     }
     print(json.dumps(out))

@@ -412,6 +412,7 @@ typedef struct rp_server_desc {
                                     rods with it; else this shape for both */
     int steps_per_launch;        /* steps per rp_server_step (0: 64) */
     int self_tile_config;        /* when positive: the shape for rods with self-collision */
+    int time_slice_us;           /* when positive: time-sliced steps (below) */
 } rp_server_desc;
 
 /* A request's initial state and loads, packed rod after rod like the
@@ -460,7 +461,14 @@ int rp_server_submit(void* server, const rp_batch_desc* request, const rp_reques
                      int* out_id);
 
 /* Launch one chunk of steps for every running request (asynchronous); sets
- * *num_finished to the number of requests that reach their horizon in it. */
+ * *num_finished to the number of requests that reach their horizon in it.
+ * With time_slice_us > 0 the chunk is a time slice instead: every workgroup
+ * runs as many steps as it starts within the slice (at least one, at most
+ * 8192 and up to its rods' horizons), so light rods are not held to the pace
+ * of the slowest. Progress is then known only once the launch completes:
+ * *num_finished is 0, and rp_server_sync, rp_server_finished and the other
+ * queries read the rods' step clocks back first. Results are the same bit for
+ * bit whichever mode, slice or chunk length. */
 int rp_server_step(void* server, int* num_finished);
 
 /* Wait for every launched chunk. */

@@ -84,7 +84,8 @@ class RpCounters(C.Structure):
 
 class RpServerDesc(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_longlong), ("max_rods", C.c_int), ("params", RpStepParams),
-                ("tile_config", C.c_int), ("steps_per_launch", C.c_int), ("self_tile_config", C.c_int)]
+                ("tile_config", C.c_int), ("steps_per_launch", C.c_int), ("self_tile_config", C.c_int),
+                ("time_slice_us", C.c_int)]
 
 
 class RpRequestState(C.Structure):

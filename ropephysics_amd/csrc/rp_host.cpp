@@ -219,6 +219,7 @@ StepArgsT<Real> step_args(const typename Prec<Real>::StepParams& p, int steps) {
     a.iterations = p.iterations;
     a.contactInterval = p.contact_interval > 0 ? p.contact_interval : 1;
     a.steps = steps;
+    a.sliceTicks = 0;
     return a;
 }
 

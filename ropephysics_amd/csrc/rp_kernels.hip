@@ -500,9 +500,17 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     const int obsP = (active && d.obs) ? d.obs[rod].particle : -1;
     const bool observer = obsP >= 0 && obsP % G == l;
     double maxRise = observer ? d.obs[rod].maxRise : 0.0;
+    const long long sliceStart = sa.sliceTicks > 0 ? (long long)wall_clock64() : 0;
     __syncthreads();
 
     for (int step = 0; step < sa.steps; ++step) {
+        // Time-sliced launches: the workgroup decides together, on thread 0's
+        // clock, whether to start another step, and stops once none of its
+        // rods has steps left.
+        if (sa.sliceTicks > 0 && step > 0 &&
+            (__syncthreads_or(threadIdx.x == 0 && (long long)wall_clock64() - sliceStart >= sa.sliceTicks) ||
+             !__syncthreads_or(active && clk < limit)))
+            break;
         // Addresses that depend on the rod (its path and observer records) are
         // formed where they are used, not held in registers across the loop.
         asm volatile("" : "+v"(rod));

@@ -156,6 +156,10 @@ struct StepArgsT {
     int iterations;
     int contactInterval;
     int steps;  // steps in this launch
+    // When positive, a workgroup starts no new step once this many device
+    // wall-clock ticks have passed since it began (time-sliced launches; each
+    // workgroup still takes at least one step).
+    long long sliceTicks;
 };
 
 // The two arithmetic precisions. Device code is compiled once per precision

@@ -112,6 +112,11 @@ struct rp_server_impl {
     int configLean = 10;  // tp3-tc1-w4
     int configSelf = 13;  // tp1-tc1-w8
     int stepsPerLaunch = 64;
+    // Time-sliced steps: the slice in device wall-clock ticks (0: chunks of
+    // stepsPerLaunch), and whether a launch's progress is still to be read.
+    long long sliceTicks = 0;
+    bool settlePending = false;
+    std::vector<int> clockHost;
     LdsLimits lim;
     hipStream_t stream = nullptr;
     hipStream_t selfStream = nullptr;
@@ -174,6 +179,43 @@ void release_ranges(rp_server_impl& s, const Request& q) {
     for (long long r = q.start[kRods]; r < q.start[kRods] + q.count[kRods]; ++r) s.selfOf[size_t(r)] = 0;
 }
 
+// After a time-sliced launch: wait for it, read the rods' step clocks and set
+// each running request's progress to its slowest rod's; requests at their
+// horizon become finished.
+int settle(rp_server_impl& s) {
+    if (!s.settlePending) return RP_OK;
+    // One copy of the span of rod slots the running requests occupy.
+    long long lo = -1, hi = -1;
+    for (const auto& kv : s.requests) {
+        const Request& q = kv.second;
+        if (q.stepsDone >= q.steps || q.count[kRods] == 0) continue;
+        lo = lo < 0 ? q.start[kRods] : std::min(lo, q.start[kRods]);
+        hi = std::max(hi, q.start[kRods] + q.count[kRods]);
+    }
+    s.clockHost.resize(size_t(s.pools[kRods].capacity()));
+    if (lo >= 0)
+        RP_HIP(hipMemcpyAsync(s.clockHost.data() + lo, s.d.clock + lo, size_t(hi - lo) * sizeof(int),
+                              hipMemcpyDeviceToHost, s.stream));
+    RP_HIP(hipStreamSynchronize(s.stream));
+    s.settlePending = false;
+    int advance = 0;
+    for (auto& kv : s.requests) {
+        Request& q = kv.second;
+        if (q.stepsDone >= q.steps) continue;
+        int done = q.steps;
+        for (long long r = q.start[kRods]; r < q.start[kRods] + q.count[kRods]; ++r)
+            done = std::min(done, s.clockHost[size_t(r)]);
+        advance = std::max(advance, done - q.stepsDone);
+        q.stepsDone = done;
+        if (q.stepsDone >= q.steps) {
+            s.finished.push_back(q.id);
+            s.dirty = true;
+        }
+    }
+    s.stepsTaken += advance;
+    return RP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -194,6 +236,14 @@ int rp_server_create(const rp_server_desc* desc, void** out_server) {
     if (desc->tile_config > 0) s->configLean = s->configSelf = desc->tile_config;
     if (desc->self_tile_config > 0) s->configSelf = desc->self_tile_config;
     if (desc->steps_per_launch > 0) s->stepsPerLaunch = desc->steps_per_launch;
+    if (desc->time_slice_us < 0) return fail(RP_ERR_INVALID, "time_slice_us must not be negative");
+    if (desc->time_slice_us > 0) {
+        int dev = 0, khz = 0;
+        RP_HIP(hipGetDevice(&dev));
+        RP_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        if (khz <= 0) return fail(RP_ERR_DEVICE, "the device reports no wall-clock rate");
+        s->sliceTicks = std::max(1LL, (long long)desc->time_slice_us * khz / 1000);
+    }
     int rc = lds_limits(s->lim);
     if (rc) return rc;
 
@@ -379,6 +429,8 @@ int rp_server_step(void* server, int* num_finished) {
     if (!server) return fail(RP_ERR_INVALID, "null server");
     rp_server_impl& s = *S(server);
     if (num_finished) *num_finished = 0;
+    int rc0 = settle(s);
+    if (rc0) return rc0;
     std::vector<int> lean, self;
     bool coloured = false;  // a running rod asks for the coloured self order
     for (const auto& kv : s.requests) {
@@ -411,7 +463,17 @@ int rp_server_step(void* server, int* num_finished) {
         RP_HIP(hipStreamSynchronize(s.stream));
         s.dirty = false;
     }
-    const StepArgs a = step_args<float>(s.params, s.stepsPerLaunch);
+    // Chunked: stepsPerLaunch steps. Time-sliced: up to the furthest horizon
+    // (at most 8192 steps), the slice ending the launch.
+    int launchSteps = s.stepsPerLaunch;
+    if (s.sliceTicks > 0) {
+        launchSteps = 1;
+        for (const auto& kv : s.requests)
+            launchSteps = std::max(launchSteps, kv.second.steps - kv.second.stepsDone);
+        launchSteps = std::min(launchSteps, 8192);
+    }
+    StepArgs a = step_args<float>(s.params, launchSteps);
+    a.sliceTicks = s.sliceTicks;
     if (!self.empty()) {
         // Uploads and tiles are on the main stream: the second stream starts
         // after them, and the main stream continues after its launch.
@@ -430,6 +492,10 @@ int rp_server_step(void* server, int* num_finished) {
         ++s.launches;
     }
     if (!self.empty()) RP_HIP(hipStreamWaitEvent(s.stream, s.evSelfDone, 0));
+    if (s.sliceTicks > 0) {
+        s.settlePending = true;
+        return RP_OK;
+    }
     s.stepsTaken += s.stepsPerLaunch;
     int done = 0;
     for (auto& kv : s.requests) {
@@ -449,12 +515,14 @@ int rp_server_step(void* server, int* num_finished) {
 int rp_server_sync(void* server) {
     if (!server) return fail(RP_ERR_INVALID, "null server");
     RP_HIP(hipStreamSynchronize(S(server)->stream));
-    return RP_OK;
+    return settle(*S(server));
 }
 
 int rp_server_finished(void* server, int* ids, int capacity) {
     if (!server) return fail(RP_ERR_INVALID, "null server");
     rp_server_impl& s = *S(server);
+    const int rc = settle(s);
+    if (rc) return rc;
     const int n = int(s.finished.size());
     for (int i = 0; ids && i < std::min(n, capacity); ++i) ids[i] = s.finished[size_t(i)];
     return n;
@@ -463,6 +531,7 @@ int rp_server_finished(void* server, int* ids, int capacity) {
 int rp_server_request_info(void* server, int id, rp_request_info* out) {
     if (!server || !out) return fail(RP_ERR_INVALID, "null argument");
     rp_server_impl& s = *S(server);
+    if (const int rc = settle(s)) return rc;
     auto it = s.requests.find(id);
     if (it == s.requests.end()) return fail(RP_ERR_INVALID, "unknown request " + std::to_string(id));
     const Request& q = it->second;
@@ -478,6 +547,7 @@ int rp_server_request_info(void* server, int id, rp_request_info* out) {
 int rp_server_result(void* server, int id, float* x, float* q, float* v, float* omega) {
     if (!server) return fail(RP_ERR_INVALID, "null server");
     rp_server_impl& s = *S(server);
+    if (const int rc = settle(s)) return rc;
     auto it = s.requests.find(id);
     if (it == s.requests.end()) return fail(RP_ERR_INVALID, "unknown request " + std::to_string(id));
     const Request& r = it->second;
@@ -495,6 +565,7 @@ int rp_server_result(void* server, int id, float* x, float* q, float* v, float* 
 int rp_server_release(void* server, int id) {
     if (!server) return fail(RP_ERR_INVALID, "null server");
     rp_server_impl& s = *S(server);
+    if (const int rc = settle(s)) return rc;
     auto it = s.requests.find(id);
     if (it == s.requests.end()) return fail(RP_ERR_INVALID, "unknown request " + std::to_string(id));
     // The ranges may be reused by the next submission: let queued work on
@@ -510,6 +581,7 @@ int rp_server_release(void* server, int id) {
 int rp_server_get_stats(void* server, rp_server_stats* out) {
     if (!server || !out) return fail(RP_ERR_INVALID, "null argument");
     rp_server_impl& s = *S(server);
+    if (const int rc = settle(s)) return rc;
     std::memset(out, 0, sizeof(*out));
     for (const auto& kv : s.requests) {
         const Request& q = kv.second;

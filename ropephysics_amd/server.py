@@ -33,7 +33,7 @@ class ServerFull(RuntimeError):
 
 class Server:
     def __init__(self, params: SolverParams, hbm_budget_bytes: int, max_rods: int = 0, tile_config: int = 0,
-                 steps_per_launch: int = 0, self_tile_config: int = 0):
+                 steps_per_launch: int = 0, self_tile_config: int = 0, time_slice_us: int = 0):
         self._h = None
         d = _abi.RpServerDesc()
         d.hbm_budget_bytes = int(hbm_budget_bytes)
@@ -42,6 +42,7 @@ class Server:
         d.tile_config = int(tile_config)
         d.steps_per_launch = int(steps_per_launch)
         d.self_tile_config = int(self_tile_config)
+        d.time_slice_us = int(time_slice_us)
         h = C.c_void_p()
         _abi.check(_abi.lib().rp_server_create(C.byref(d), C.byref(h)))
         self._h = h

@@ -144,3 +144,31 @@ def test_admission_by_budget_and_reuse():
         assert np.array_equal(bits(first[k]), bits(second[k]))
     with pytest.raises(rp.ServerFull):
         server.submit([benchmark_rod(120)] * 40, world, 8)  # larger than the whole budget
+
+
+def test_time_sliced_steps_give_the_same_results():
+    """Time-sliced server steps (each workgroup runs the steps it can start in
+    a 300 us slice, up to its rods' horizons): every request's result equals
+    the one it gets alone in a Batch, bit for bit, and horizons are exact."""
+    requests = [_request(s) for s in range(8)]
+    server = rp.Server(PARAMS, hbm_budget_bytes=64 << 20, time_slice_us=300)
+    results = _serve(server, requests, arrivals=[0, 0, 1, 1, 2, 3, 5, 5])
+    for k, (rods, world, steps, ys, fs) in enumerate(requests):
+        _assert_same(_split(results[k], rods), _alone_in_batch(rods, world, steps, ys, fs))
+    rod = benchmark_rod(33)
+    world = _world(7, True)
+    solo = rp.Server(PARAMS, hbm_budget_bytes=8 << 20, time_slice_us=300)
+    rid = solo.submit([rod], world, 37)
+    rounds = 0
+    while not solo.finished():
+        solo.step()
+        solo.synchronize()
+        rounds += 1
+        assert rounds < 1000
+    assert solo.info(rid)["steps_done"] == 37
+    o = oracle_twin(rod, 32)
+    o.step(oracle_params(PARAMS), oracle_world(world), 37)
+    d = compare_bitwise(solo.result(rid), oracle_state(o))
+    assert all(v == 0 for v in d.values()), d
+    with pytest.raises(rp.RopePhysicsError):
+        rp.Server(PARAMS, hbm_budget_bytes=8 << 20, time_slice_us=-1)
