@@ -306,6 +306,37 @@ def test_self_collision_longest_segment_in_every_row():
     assert o.contact_counts()[1] >= 10  # the coils are in self contact
 
 
+def _reverse_particles(rod):
+    """The same rod with its particles numbered from the other end: stretch
+    constraint k joins particles n-1-k and n-2-k."""
+    r = rod.copy()
+    n = r.num_particles
+    for name in ("x", "xPrev", "v", "invMass", "mass", "extForce", "kinVel"):
+        setattr(r, name, getattr(r, name)[::-1].copy())
+    r.sP0 = (n - 1 - rod.sP0).astype(rod.sP0.dtype)
+    r.sP1 = (n - 1 - rod.sP1).astype(rod.sP1.dtype)
+    return r
+
+
+@pytest.mark.parametrize("tile_config", [0, 12])
+def test_self_collision_rod_with_relabelled_particles(tile_config):
+    """A coil whose segments do not join particles k and k + 1 (particles
+    numbered from the other end; the broadphase takes segment ends from the
+    constraint table): bitwise equal to the oracle, and, since relabelling
+    changes no arithmetic, to the same coil numbered the usual way, particle
+    for particle."""
+    world = rp.CollisionWorld(primitives=[rp.Primitive.plane((0, 0, 0), (0, 0, 1), 0.6)], self_collision=True,
+                              self_friction=0.3, hash_table_size=64, self_capacity=700)
+    coil = pinned_coil()
+    rods = [_reverse_particles(coil), benchmark_rod(24), coil]
+    diffs, _, b = run_pair(rods, world, SELF_PARAMS, 8, "tiled", checkpoints=[2, 8], tile_config=tile_config)
+    assert b.self_contact_overflow() == 0
+    assert_bitwise(diffs)
+    rel, plain = b.download(0), b.download(2)
+    assert np.array_equal(bits(rel["x"][::-1]), bits(plain["x"]))
+    assert np.array_equal(bits(rel["q"]), bits(plain["q"]))
+
+
 @pytest.mark.parametrize("strategy", STRATEGIES)
 def test_ragged_batch_and_alone_equals_mixed(strategy):
     sizes = [7, 20, 33, 64, 97, 130, 160, 48]
