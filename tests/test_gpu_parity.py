@@ -401,6 +401,23 @@ def test_launch_length_does_not_change_the_bits():
             assert np.array_equal(bits(o[k]), bits(outs[0][k])), k
 
 
+def test_calls_longer_than_one_launch():
+    """A call of 8200 steps runs as two launches (the bound is 8192 steps per
+    launch) and equals 82 calls of 100 steps, bit for bit."""
+    rods = [benchmark_rod(16), benchmark_rod(9)]
+    params = benchmark_params(substeps=2)
+    one = rp.Batch(rods, strategy="tiled", world=CONTACT_WORLD)
+    one.step(params, 8200)
+    assert one.counters()["last_step_launches"] == 2
+    many = rp.Batch(rods, strategy="tiled", world=CONTACT_WORLD)
+    for _ in range(82):
+        many.step(params, 100)
+    a, b = one.get_state(), many.get_state()
+    for k in a:
+        assert np.array_equal(bits(a[k]), bits(b[k])), k
+    assert np.all(np.isfinite(a["x"]))
+
+
 def test_full_size_uniform_batch_sampled_rods_bitwise():
     """The 4096 x 64 floor workload at full size for 30 steps: a sample of rods
     against the FP32 oracle, bit for bit; every rod finite and above the floor."""
