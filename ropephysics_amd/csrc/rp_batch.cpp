@@ -413,7 +413,8 @@ int create(const typename Prec<Real>::BatchDesc* desc, void** out_batch) {
     // for this precision has lanes enough for it.
     bool allTileable = true;
     for (int r = 0; r < R; ++r) {
-        if (b->tileable[r] && size_t(rod_tiled_lds(b->rods[r])) > b->ldsLimit) b->tileable[r] = 0;
+        // Checked against the budget the packer uses (one workgroup per CU).
+        if (b->tileable[r] && rod_tiled_lds(b->rods[r]) > lds_budget(*b, 1)) b->tileable[r] = 0;
         if (b->tileable[r]) {
             bool shape = false;
             for (int id = 1; id <= num_tile_configs() && !shape; ++id) {

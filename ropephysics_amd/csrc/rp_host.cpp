@@ -95,10 +95,13 @@ bool tile_config_available(int id) {
 }
 
 // Lanes a rod needs under (tp, tc): every particle and frame within tp trips,
-// every colour within tc trips.
+// every colour within tc trips, and -- with self-collision, whose broadphase
+// gives each lane the segments l, l + G, ... -- every stretch constraint
+// within tp trips.
 template <typename Real>
 int group_lanes(const RodInfoT<Real>& r, const std::vector<int>& colStart, int tp, int tc) {
     int g = std::max((r.nP + tp - 1) / tp, (r.nF + tp - 1) / tp);
+    if (r.self) g = std::max(g, (r.nS + tp - 1) / tp);
     const int* cs = colStart.data() + r.colOff;
     for (int c = 0; c < r.nSCol; ++c) g = std::max(g, (cs[c + 1] - cs[c] + tc - 1) / tc);
     const int* cb = cs + r.nSCol + 1;

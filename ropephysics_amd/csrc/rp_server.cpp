@@ -336,7 +336,9 @@ int rp_server_submit(void* server, const rp_batch_desc* req, const rp_request_st
         const RodInfo& ri = rec.rods[r];
         const TileConfig& cfg = tile_config(ri.self ? s.configSelf : s.configLean);
         if (!rec.colourFit[r]) return fail(RP_ERR_CAPACITY, "rod " + std::to_string(r) + " needs more than two colours");
-        if (size_t(rod_tiled_lds(ri)) > s.lim.perWorkgroup)
+        // The packer's one-workgroup-per-CU budget, not the raw per-workgroup
+        // limit: a rod between the two would be admitted but never packed.
+        if (rod_tiled_lds(ri) > lds_budget(s.lim, 1))
             return fail(RP_ERR_CAPACITY, "rod " + std::to_string(r) + " needs more LDS than a workgroup has");
         if (group_lanes(ri, rec.colStart, cfg.tp, cfg.tc) > 64 * cfg.waves)
             return fail(RP_ERR_CAPACITY, "rod " + std::to_string(r) + " is too long for the tile shape");
