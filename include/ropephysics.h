@@ -43,14 +43,22 @@ enum rp_status {
 };
 
 /* Execution strategy.
- *   PLAIN  one rod per 128-thread workgroup, one launch per step, state
- *          re-read from and written to HBM every step: the reference's fused
- *          launch shape (src/gpu/gpu_solver.cu:531-681), kept as the baseline.
- *   TILED  rods of any sizes packed side by side into wavefront lane groups,
- *          state resident in LDS and VGPRs across every substep of many steps
- *          per launch; launches replayed from a HIP graph.
- *   AUTO   TILED when every rod fits it, else PLAIN. */
-enum rp_strategy { RP_STRATEGY_PLAIN = 0, RP_STRATEGY_TILED = 1, RP_STRATEGY_AUTO = 2 };
+ *   PLAIN   one rod per 128-thread workgroup, one launch per step, state
+ *           re-read from and written to HBM every step: the reference's fused
+ *           launch shape (src/gpu/gpu_solver.cu:531-681), kept as the baseline.
+ *   TILED   rods of any sizes packed side by side into wavefront lane groups,
+ *           state resident in LDS and VGPRs across every substep of many steps
+ *           per launch; launches replayed from a HIP graph.
+ *   GLOBAL  for rods whose working arrays exceed a workgroup's LDS, the
+ *           counterpart of the reference's multi-kernel strategy
+ *           (src/gpu/gpu_solver.cu:417-529,691-730; GpuSim falls back to it,
+ *           src/gpu/gpu_suite.cpp:125-130): one rod per workgroup of up to 512
+ *           lanes with its working arrays in HBM (allocated at the first step),
+ *           many steps per launch. Any rod length; same arithmetic and results
+ *           as PLAIN, bit for bit.
+ *   AUTO    TILED when every rod fits it, else PLAIN when every rod fits a
+ *           workgroup's LDS, else GLOBAL. */
+enum rp_strategy { RP_STRATEGY_PLAIN = 0, RP_STRATEGY_TILED = 1, RP_STRATEGY_AUTO = 2, RP_STRATEGY_GLOBAL = 3 };
 
 enum rp_prim_type { RP_PLANE = 0, RP_SPHERE = 1, RP_CAPSULE = 2, RP_BOX = 3 };
 
@@ -161,6 +169,15 @@ typedef struct rp_counters {
     size_t device_bytes;             /* device memory held by the batch */
     float last_step_gpu_ms;          /* with timing on: HIP-event time of the last rp_batch_step */
     int last_step_launches;          /* kernel launches in the last rp_batch_step */
+    double motion_margin;            /* the reference's float motion margin (floatMotionMargin,
+                                        src/apps/cable_hanging.cpp:75-81): gravity's motion per substep
+                                        |g| h^2 over the spacing of the batch's precision at the largest
+                                        coordinate ever uploaded; the smallest over the steps taken, 0
+                                        before any step with gravity. Below about 1 a resting rod cannot
+                                        start to slide: rounding acts as static friction. */
+    long long low_margin_steps;      /* steps taken with motion_margin below 2, the reference's bar
+                                        (src/validation/application_cases.cpp:196-197) */
+    size_t scratch_bytes;            /* GLOBAL: the HBM working arrays (0 until the first step) */
 } rp_counters;
 
 /* cudaAvailable / deviceName / diagnostic / maxFusedSegments
@@ -199,7 +216,8 @@ int rp_batch_set_scales(void* batch, const float* youngs_scale, const float* fri
  * possible: at most 8192 steps each, and about 4 s of device time as estimated
  * from the autotuner's trial (never fewer than 64 steps); RP_STEPS_PER_LAUNCH
  * in the environment at creation replaces both bounds. PLAIN launches once per
- * step. Results do not depend on how steps are split into calls or launches. */
+ * step. GLOBAL runs about 2^22 element-substeps of its longest rod per launch.
+ * Results do not depend on how steps are split into calls or launches. */
 int rp_batch_step(void* batch, const rp_step_params* params, int num_steps);
 
 /* Batch::synchronize (src/gpu/gpu_batch.cpp:475), scoped to this batch. */

@@ -23,13 +23,16 @@ namespace rp {
 
 // kFused / kMultiKernel are the reference's names (src/gpu/gpu_solver.h:50):
 // the fused launch maps to the best strategy that fits (AUTO), the multi-kernel
-// fallback for rods too large for one workgroup to PLAIN.
+// strategy -- global-memory state for rods too large for one workgroup's
+// shared memory, which GpuSim falls back to (src/gpu/gpu_suite.cpp:125-130) --
+// to GLOBAL, which takes rods of any length.
 enum class Strategy {
     kPlain = RP_STRATEGY_PLAIN,
     kTiled = RP_STRATEGY_TILED,
     kAuto = RP_STRATEGY_AUTO,
+    kGlobal = RP_STRATEGY_GLOBAL,
     kFused = RP_STRATEGY_AUTO,
-    kMultiKernel = RP_STRATEGY_PLAIN
+    kMultiKernel = RP_STRATEGY_GLOBAL
 };
 
 // BatchParams (src/gpu/gpu_solver.h:52-60).

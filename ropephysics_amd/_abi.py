@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libropephysics.so")
 
 RP_OK, RP_ERR_INVALID, RP_ERR_DEVICE, RP_ERR_MEMORY, RP_ERR_CAPACITY, RP_ERR_STATE = 0, -1, -2, -3, -4, -5
-STRATEGY = {"plain": 0, "tiled": 1, "auto": 2}
+STRATEGY = {"plain": 0, "tiled": 1, "auto": 2, "global": 3}
 STRATEGY_NAME = {v: k for k, v in STRATEGY.items()}
 
 _fp = C.POINTER(C.c_float)
@@ -79,7 +79,8 @@ class RpCounters(C.Structure):
                 ("workgroups", C.c_int), ("lanes_busy", C.c_int), ("lanes_total", C.c_int),
                 ("lds_bytes_per_workgroup", C.c_size_t), ("launches_per_step", C.c_int),
                 ("steps_taken", C.c_longlong), ("device_bytes", C.c_size_t), ("last_step_gpu_ms", C.c_float),
-                ("last_step_launches", C.c_int)]
+                ("last_step_launches", C.c_int), ("motion_margin", C.c_double), ("low_margin_steps", C.c_longlong),
+                ("scratch_bytes", C.c_size_t)]
 
 
 class RpServerDesc(C.Structure):

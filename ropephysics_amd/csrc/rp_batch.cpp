@@ -96,6 +96,19 @@ struct rp_batch_impl {
     double tileMsPerSweep = 0.0;
     // PLAIN: launches (one step each) captured per graph.
     int plainStepsPerGraph = 64;
+    // GLOBAL: lanes per workgroup, per-rod offsets of the HBM working arrays
+    // and their total; the arrays are allocated at the first step.
+    int globalThreads = 128;
+    std::vector<long long> scratchOffHost;
+    size_t scratchBytes = 0;
+    void* scratch = nullptr;
+    long long* scratchOff = nullptr;
+    // FP32 motion margin: the largest coordinate magnitude ever uploaded, and
+    // over the steps taken the smallest |g| h^2 / spacing at it and the steps
+    // taken below the reference's bar of 2.
+    double coordScale = 0.0;
+    double minMargin = 0.0;
+    long long lowMarginSteps = 0;
     long long stepsTaken = 0;
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::vector<int> colStart;  // host copy of the colour-start table
@@ -217,6 +230,14 @@ int launch_steps(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int 
     if (strategy == RP_STRATEGY_PLAIN) {
         const StepArgsT<Real> a = step_args<Real>(p, 1);
         for (int s = 0; s < steps; ++s) RP_HIP(Prec<Real>::launch_plain(b.d, a, size_t(b.plainLds), b.stream));
+        return RP_OK;
+    }
+    if (strategy == RP_STRATEGY_GLOBAL) {
+        for (int left = steps; left > 0;) {
+            const int n = std::min(left, b.stepsPerLaunch);
+            RP_HIP(Prec<Real>::launch_global(b.d, step_args<Real>(p, n), b.globalThreads, b.stream));
+            left -= n;
+        }
         return RP_OK;
     }
     const TileConfig& cfg = tile_config(t->config);
@@ -377,7 +398,7 @@ template <typename Real>
 int create(const typename Prec<Real>::BatchDesc* desc, void** out_batch) {
     if (!desc || !out_batch) return fail(RP_ERR_INVALID, "null argument");
     *out_batch = nullptr;
-    if (desc->strategy < RP_STRATEGY_PLAIN || desc->strategy > RP_STRATEGY_AUTO)
+    if (desc->strategy < RP_STRATEGY_PLAIN || desc->strategy > RP_STRATEGY_GLOBAL)
         return fail(RP_ERR_INVALID, "unknown strategy");
     if (desc->tile_config < 0 || desc->tile_config > num_tile_configs())
         return fail(RP_ERR_INVALID, "unknown tile configuration");
@@ -427,15 +448,35 @@ int create(const typename Prec<Real>::BatchDesc* desc, void** out_batch) {
         allTileable = allTileable && b->tileable[r];
     }
     int strategy = desc->strategy;
-    if (strategy == RP_STRATEGY_AUTO) strategy = allTileable ? RP_STRATEGY_TILED : RP_STRATEGY_PLAIN;
+    // AUTO: the ragged stepper when every rod fits it, else the one-rod-per-
+    // workgroup stepper in LDS, else the same in HBM.
+    if (strategy == RP_STRATEGY_AUTO)
+        strategy = allTileable ? RP_STRATEGY_TILED
+                               : (size_t(rec.maxPlainLds) <= b->ldsLimit ? RP_STRATEGY_PLAIN : RP_STRATEGY_GLOBAL);
     if (strategy == RP_STRATEGY_TILED && !allTileable)
         return fail(RP_ERR_CAPACITY,
                     "a rod does not fit the TILED strategy (more than 2 colours, LDS budget or lanes)");
     if (strategy == RP_STRATEGY_PLAIN && size_t(rec.maxPlainLds) > b->ldsLimit)
         return fail(RP_ERR_CAPACITY, "a rod needs " + std::to_string(rec.maxPlainLds) +
                                          " bytes of LDS; the device has " + std::to_string(b->ldsLimit) +
-                                         " per workgroup");
+                                         " per workgroup (the GLOBAL strategy takes it)");
     b->strategy = strategy;
+    if (strategy == RP_STRATEGY_GLOBAL) {
+        // Each rod's working arrays in HBM, 256-byte aligned; the workgroup as
+        // wide as the longest rod's elements (64 to 512 lanes).
+        b->scratchOffHost.resize(R);
+        size_t off = 0;
+        int widest = 1;
+        for (int r = 0; r < R; ++r) {
+            const RodInfoT<Real>& ri = b->rods[r];
+            b->scratchOffHost[r] = (long long)off;
+            off += (size_t(Prec<Real>::plain_lds_bytes(ri.nP, ri.nF, ri.nS, ri.nB, ri.nPrim, ri.self, ri.table,
+                                                      ri.selfCap)) + 255) & ~size_t(255);
+            widest = std::max(widest, std::max(ri.nP, ri.nF));
+        }
+        b->scratchBytes = off;
+        b->globalThreads = std::min(Prec<Real>::global_max_threads(), std::max(64, (widest + 63) / 64 * 64));
+    }
     if (const char* e = std::getenv("RP_STEPS_PER_LAUNCH")) {
         b->stepsPerLaunch = std::max(1, std::atoi(e));
         b->stepsPerLaunchSet = true;
@@ -510,6 +551,10 @@ int create(const typename Prec<Real>::BatchDesc* desc, void** out_batch) {
     d.colStart = dCol;
     d.rods = dRods;
     d.prims = dPrims;
+    if (strategy == RP_STRATEGY_GLOBAL) {
+        if ((rc = b->upload(b->scratchOff, b->scratchOffHost))) return rc;
+        d.scratchOff = b->scratchOff;
+    }
     *out_batch = static_cast<rp_batch_impl*>(b.release());
     return RP_OK;
 }
@@ -533,6 +578,10 @@ int set_state(void* batch, int first_rod, int num_rods, const Real* x, const Rea
     if (rc) return rc;
     BatchT<Real>& b = *bp;
     if ((rc = check_range(b, first_rod, num_rods))) return rc;
+    if (x) {
+        const size_t n = size_t(b.pStart[first_rod + num_rods] - b.pStart[first_rod]) * 3;
+        for (size_t i = 0; i < n; ++i) b.coordScale = std::max(b.coordScale, std::fabs(double(x[i])));
+    }
     if ((rc = copy_range<Real>(b, first_rod, num_rods, b.d.x, x, nullptr, b.pStart, 3)) ||
         (rc = copy_range<Real>(b, first_rod, num_rods, b.d.v, v, nullptr, b.pStart, 3)) ||
         (rc = copy_range<Real>(b, first_rod, num_rods, b.d.q, q, nullptr, b.fStart, 4)) ||
@@ -609,6 +658,25 @@ int set_scales(void* batch, const Real* youngs_scale, const Real* friction_scale
     return RP_OK;
 }
 
+// The reference's float motion margin (HangingCable::floatMotionMargin,
+// src/apps/cable_hanging.cpp:75-81, required to be at least 2 by
+// application_cases.cpp:196-197): gravity's displacement per substep, |g| h^2,
+// over the spacing of the working precision at the largest coordinate the
+// batch holds. Below about 1, a resting rod cannot start to move under
+// gravity: rounding acts as static friction. Recorded per step call.
+template <typename Real>
+void note_motion_margin(rp_batch_impl& b, const typename Prec<Real>::StepParams& p, int steps) {
+    const double g = std::sqrt(double(p.gravity[0]) * p.gravity[0] + double(p.gravity[1]) * p.gravity[1] +
+                               double(p.gravity[2]) * p.gravity[2]);
+    if (!(g > 0.0)) return;
+    const Real c = Real(b.coordScale);
+    const double spacing = double(std::nextafter(c, Real(INFINITY)) - c);
+    const double h = double(p.dt) / p.substeps;
+    const double margin = g * h * h / spacing;
+    if (b.minMargin == 0.0 || margin < b.minMargin) b.minMargin = margin;
+    if (margin < 2.0) b.lowMarginSteps += steps;
+}
+
 template <typename Real>
 int step(void* batch, const typename Prec<Real>::StepParams* params, int num_steps) {
     if (!params) return fail(RP_ERR_INVALID, "null argument");
@@ -622,10 +690,25 @@ int step(void* batch, const typename Prec<Real>::StepParams* params, int num_ste
         return fail(RP_ERR_INVALID, "need substeps >= 1, iterations >= 0 and dt > 0");
     if (num_steps == 0) return RP_OK;
     if (b.strategy == RP_STRATEGY_TILED && (rc = resolve_tiles(b, p))) return rc;
+    if (b.strategy == RP_STRATEGY_GLOBAL && !b.scratch) {
+        char* region = nullptr;
+        if ((rc = b.alloc(region, b.scratchBytes))) return rc;
+        b.scratch = region;
+        b.d.scratch = region;
+    }
+    note_motion_margin<Real>(b, p, num_steps);
     // Steps run in chunks: one TILED launch of at most stepsPerLaunch steps,
     // or plainStepsPerGraph PLAIN launches. Each chunk shape is captured once
     // into a HIP graph (cached per parameter set and length) and replayed.
     int chunk = b.strategy == RP_STRATEGY_PLAIN ? b.plainStepsPerGraph : b.stepsPerLaunch;
+    if (b.strategy == RP_STRATEGY_GLOBAL && !b.stepsPerLaunchSet) {
+        // About 2^22 element-substeps of the longest rod per launch (a few
+        // hundred milliseconds), at least one step.
+        int widest = 1;
+        for (const RodInfoT<Real>& ri : b.rods) widest = std::max(widest, std::max(ri.nP, ri.nF));
+        const double perStep = double(widest) * p.substeps * std::max(1, p.iterations);
+        chunk = int(std::max(1.0, std::min(double(chunk), std::floor(double(1 << 22) / perStep))));
+    }
     if (b.strategy == RP_STRATEGY_TILED && !b.stepsPerLaunchSet && b.tileMsPerSweep > 0.0) {
         // Hold a launch to about kLaunchBudgetMs of estimated device time (at
         // least 64 steps), so a very large batch still returns to the host
@@ -1036,11 +1119,19 @@ int rp_batch_counters(void* batch, rp_counters* out) {
     out->strategy = b.strategy;
     out->steps_taken = b.stepsTaken;
     out->device_bytes = b.deviceBytes;
+    out->motion_margin = b.minMargin;
+    out->low_margin_steps = b.lowMarginSteps;
+    out->scratch_bytes = b.scratch ? b.scratchBytes : 0;
     if (b.strategy == RP_STRATEGY_PLAIN) {
         out->workgroups = b.numRods;
         out->lanes_total = b.numRods * 128;
         out->lds_bytes_per_workgroup = size_t(b.plainLds);
         out->launches_per_step = 1;
+    } else if (b.strategy == RP_STRATEGY_GLOBAL) {
+        out->workgroups = b.numRods;
+        out->lanes_busy = b.numRods * b.globalThreads;
+        out->lanes_total = b.numRods * b.globalThreads;
+        out->launches_per_step = 0;
     } else if (b.tiles) {
         const TileConfig& c = tile_config(b.tiles->config);
         out->tile_config = b.tiles->config;

@@ -20,6 +20,8 @@ namespace rp {
     int plain_lds_bytes(int nP, int nF, int nS, int nB, int nPrim, int self, int table, int cap);              \
     int tiled_lds_bytes(int nP, int nF, int nS, int nPrim, int self, int table, int cap);                      \
     hipError_t launch_plain(const DeviceArraysT<R>& d, const StepArgsT<R>& sa, size_t lds, hipStream_t s);      \
+    hipError_t launch_global(const DeviceArraysT<R>& d, const StepArgsT<R>& sa, int threads, hipStream_t s);    \
+    int global_max_threads();                                                                                  \
     hipError_t allow_dynamic_lds(int bytes, int* granted);                                                     \
     hipError_t launch_tiled(int tp, int tc, int self, const DeviceArraysT<R>& d, const StepArgsT<R>& sa,      \
                             const Tile* tiles, const TileSlot* slots, int numTiles, int threads, size_t lds,   \
@@ -51,6 +53,8 @@ struct Prec<float> {
     static constexpr auto tiled_lds_bytes = &f32::tiled_lds_bytes;
     static constexpr auto allow_dynamic_lds = &f32::allow_dynamic_lds;
     static constexpr auto launch_plain = &f32::launch_plain;
+    static constexpr auto launch_global = &f32::launch_global;
+    static constexpr auto global_max_threads = &f32::global_max_threads;
     static constexpr auto launch_tiled = &f32::launch_tiled;
     static constexpr auto tiled_shape_available = &f32::tiled_shape_available;
     static constexpr auto launch_fixed_frames = &f32::launch_fixed_frames;
@@ -69,6 +73,8 @@ struct Prec<double> {
     static constexpr auto tiled_lds_bytes = &f64::tiled_lds_bytes;
     static constexpr auto allow_dynamic_lds = &f64::allow_dynamic_lds;
     static constexpr auto launch_plain = &f64::launch_plain;
+    static constexpr auto launch_global = &f64::launch_global;
+    static constexpr auto global_max_threads = &f64::global_max_threads;
     static constexpr auto launch_tiled = &f64::launch_tiled;
     static constexpr auto tiled_shape_available = &f64::tiled_shape_available;
     static constexpr auto launch_fixed_frames = &f64::launch_fixed_frames;

@@ -96,11 +96,14 @@ __host__ __device__ inline PlainLayout plain_layout(int nP, int nF, int nS, int 
     return L;
 }
 
-__global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa) {
-    extern __shared__ real4 smem4[];
-    char* smem = reinterpret_cast<char*>(smem4);
+// The whole step of rod blockIdx.x by one workgroup, its working arrays
+// carved from `smem` as PlainLayout says: LDS for the PLAIN kernel, the rod's
+// own HBM region for the GLOBAL one. Every lane strides the rod's elements by
+// the workgroup width, so any width works; the phases are separated by
+// workgroup barriers, which order global memory within the workgroup as they
+// order LDS.
+__device__ __forceinline__ void plain_rod_steps(const DeviceArrays& d, const StepArgs& sa, char* smem) {
     const int r = blockIdx.x;
-    if (r >= d.numRods) return;
     if (d.stepLimit && d.clock[r] >= d.stepLimit[r]) return;
     const RodInfo ri = d.rods[r];
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -296,6 +299,29 @@ __global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa)
         st3(d.omega, f0 + j, f4v(om4[j]));
     }
     if (ri.self && tid == 0) d.selfOverflow[r] += sl.misc[1];
+}
+
+__global__ __launch_bounds__(128) void k_plain_step(DeviceArrays d, StepArgs sa) {
+    extern __shared__ real4 smem4[];
+    if (int(blockIdx.x) >= d.numRods) return;
+    plain_rod_steps(d, sa, reinterpret_cast<char*>(smem4));
+}
+
+// ------------------------------------------------------------------ GLOBAL
+
+// Lanes of a GLOBAL workgroup at most (two waves per SIMD).
+constexpr int kGlobalMaxThreads = 512;
+
+// The PLAIN step with the rod's working arrays in HBM (rod r's region starts
+// at scratch + scratchOff[r]), for rods whose arrays do not fit a workgroup's
+// LDS: the reference's multi-kernel strategy keeps state in global memory for
+// the same reason (src/gpu/gpu_solver.cu:417-529,691-730). Here one
+// workgroup still steps one rod through every phase, with barriers where the
+// reference launches a kernel per phase and colour, and a launch runs many
+// steps. The region stays in the L2 of the rod's XCD between phases.
+__global__ __launch_bounds__(kGlobalMaxThreads) void k_global_step(DeviceArrays d, StepArgs sa) {
+    if (int(blockIdx.x) >= d.numRods) return;
+    plain_rod_steps(d, sa, d.scratch + d.scratchOff[blockIdx.x]);
 }
 
 // ------------------------------------------------------------------ TILED
@@ -698,6 +724,15 @@ int tiled_lds_bytes(int nP, int nF, int nS, int nPrim, int self, int table, int 
 
 hipError_t launch_plain(const DeviceArrays& d, const StepArgs& sa, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL(k_plain_step, dim3(d.numRods), dim3(128), lds, s, d, sa);
+    return hipGetLastError();
+}
+
+int global_max_threads() { return kGlobalMaxThreads; }
+
+hipError_t launch_global(const DeviceArrays& d, const StepArgs& sa, int threads, hipStream_t s) {
+    if (threads < 64 || threads % 64 != 0 || threads > kGlobalMaxThreads || !d.scratch || !d.scratchOff)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_global_step, dim3(d.numRods), dim3(threads), 0, s, d, sa);
     return hipGetLastError();
 }
 

@@ -143,6 +143,10 @@ struct DeviceArraysT {
     const RecRec* rec;     // per rod, or null
     float* frames;         // maxFrames frames of framePitch floats (3 per particle)
     long long framePitch;
+    // GLOBAL strategy: rod r's working arrays (the PLAIN kernel's LDS layout)
+    // at scratch + scratchOff[r] in HBM; null otherwise.
+    char* scratch;
+    const long long* scratchOff;
     int numRods;
 };
 

@@ -117,6 +117,30 @@ def test_config4_drape_cables_bitwise_and_verdicts(prec, strategy):
           f"{[round(float(x), 3) for x in rises]}", flush=True)
 
 
+def test_motion_margin_is_reported():
+    """The reference's float motion margin (floatMotionMargin, src/apps/
+    cable_hanging.cpp:75-81; its sweep requires at least 2, application_cases.
+    cpp:196-197): an FP32 batch of drape cables reports |g| h^2 over the FP32
+    spacing at its largest coordinate (0.41 here) and counts its steps below
+    2; the FP64 batch of the same cables is far above."""
+    w = _drape()
+    rods = [w.rods[r] for r in DRAPE_CABLES[:2]]
+    coord = max(float(np.max(np.abs(np.float32(r.x)))) for r in rods)
+    c32 = np.float32(coord)
+    spacing = float(np.nextafter(c32, np.float32(np.inf)) - c32)
+    h = w.params.dt / w.params.substeps
+    for prec in (32, 64):
+        b = rp.Batch(rods, strategy="tiled", world=w.world, precision=prec)
+        assert b.counters()["motion_margin"] == 0.0
+        b.step(w.params, 3)
+        c = b.counters()
+        if prec == 32:
+            assert c["motion_margin"] == pytest.approx(9.81 * h * h / spacing, rel=1e-12)
+            assert 0.3 < c["motion_margin"] < 0.5 and c["low_margin_steps"] == 3
+        else:
+            assert c["motion_margin"] > 1e7 and c["low_margin_steps"] == 0
+
+
 # ------------------------------------------------------------------ serving stream
 
 # Requests of serving.make_requests(60) and the horizons the test cuts them
@@ -223,7 +247,7 @@ def test_config5_requests_equal_alone_and_oracle(mode):
 
 # ------------------------------------------------------------------ the reference GPU path's envelope
 
-@pytest.mark.parametrize("strategy", ["plain", "tiled"])
+@pytest.mark.parametrize("strategy", ["plain", "tiled", "global"])
 @pytest.mark.parametrize("name", ["gravity", "loaded", "contact"])
 def test_reference_gpu_envelope(strategy, name):
     """runGpuParity's scenarios (src/gpu/gpu_cases.cpp:126-300) at n = 16,
