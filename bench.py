@@ -7,24 +7,33 @@ floor, two capsule pegs and a box clip, self-collision on, dt = 1e-3, 10
 substeps, the whole 6 s = 6000 steps; synthetic inputs from a fixed seed). A
 "step" is one step of every rod in the batch. The ragged-packed TILED stepper
 is timed over --steps steps after --warmup untimed ones (the autotuner runs
-during the warm-up); the PLAIN one-rod-per-workgroup launch (the reference's
-launch shape) is timed over the same steps of the same workload beside it; the
-reference CPU path runs a bounded sample of it.
+during the warm-up), starting again from step 0; `config.timed_steps` states
+the window. The PLAIN one-rod-per-workgroup launch (the reference's launch
+shape) is timed over the same steps when there are at most 400 of them.
 
-Secondary: the routing workload again as an FP64 batch (the reference CPU
-solver's precision on the GPU), its routed verdicts checked against the
-reference CPU path in colour order; the uniform batch of 4096 rods x 64
-segments over a floor plane with per-rod stiffness and friction, 1000 steps,
-TILED and PLAIN; the hold/slide sweep of 3600 draped cables (8 s) in FP32 and
-FP64 with its verdicts spot-checked against the reference CPU path; the
-cross-entropy harness-learning loop (10 iterations of 1024 rollouts); and the
+Whatever --steps is, the full-rollout leg runs the whole 6000 steps: TILED in
+FP32 and in FP64 (one launch each; routed verdicts; `roofline` comes from the
+FP32 launch), and PLAIN over four 100-step windows spread over the rollout,
+each started from TILED's state at that step (bitwise PLAIN's own) and checked
+bit for bit at its end. The routed verdicts are compared with the reference
+CPU path's (its own sources, FP64) on a 64-cable sample, in sequential and in
+colour order; the same sample, sequential, is the all-core CPU baseline, and a
+smaller one the 1-thread baseline.
+
+Secondary: the routing rollout with the coloured self-contact order and with
+trajectory recording; 512 rods of 2000 segments (beyond LDS, the GLOBAL
+strategy); the uniform batch of 4096 rods x 64 segments over a floor plane
+with per-rod stiffness and friction, 1000 steps, TILED and PLAIN; the
+hold/slide sweep of 3600 draped cables (8 s) in FP32 and FP64 with its
+verdicts spot-checked against the reference CPU path; the cross-entropy
+harness-learning loop (10 iterations of 1024 rollouts); and the serving
 stream (requests of 1-256 rods of 32-256 segments, 0.5-8 s horizons, random
 contact shapes, self-collision in half) through the continuous-batching
-server in its two modes (64-step chunks in lockstep, and 50 ms time slices in
-which each workgroup runs the steps it can): each mode's capacity with every
-request arriving at once, then p50/p99 latency of both under the same Poisson
-arrivals, at a fraction of the chunked mode's capacity. The headline p50/p99
-is the time-sliced mode's.
+server: the capacity of each mode (64-step chunks in lockstep; 50 ms time
+slices in which each workgroup runs the steps it can) on 40 requests arriving
+at once, then 200 requests arriving as a Poisson stream at 80% of the chunked
+capacity through the time-sliced server (the headline p50/p99) and the first
+40 of them through the chunked one.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -90,9 +99,11 @@ def time_steps(b: rp.Batch, params, steps: int):
 def roofline(flops_per_unit: float, kernel_rate: float, kernel_ms: float, units_per_launch: float, traffic_key):
     achieved = flops_per_unit * kernel_rate / 1e12
     out = {
-        "bound": "mfma",
-        "bound_detail": "FP32 vector-ALU bound; the path has no dense contraction and issues no MFMA, so the "
-                        "FP32 peak is the compute roofline (unpacked FP32 tops out at half of it)",
+        "bound": "valu",
+        "bound_detail": "FP32 vector-ALU bound: the path has no dense contraction and issues no MFMA; the peak is "
+                        "MI355X's FP32 vector figure. Bitwise parity with the CPU rules out FMA contraction and "
+                        "packed FP32, and division and square root are IEEE sequences, so unpacked non-FMA issue "
+                        "(16 lanes x 4 SIMDs x 256 CUs x 2.4 GHz = 39.3e12 lane instructions/s) caps frac near 0.25",
         "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
         "traffic": None, "kernel": "k_tiled_step", "flops_per_segment_substep": flops_per_unit,
         "segment_substeps_per_launch": units_per_launch, "kernel_ms_avg": kernel_ms,
@@ -150,11 +161,6 @@ def routing_leg(w: scenes.Workload, steps: int, plain_steps: int, warmup: int, t
             x = b.download_positions()
             if not np.all(np.isfinite(x)):
                 raise RuntimeError("non-finite state after the timed routing run")
-            task = w.info["task"]
-            routed = [task.evaluate(b.download(i)["x"].astype(np.float64))["routed"] for i in range(w.num_rods)] \
-                if k == w.steps else None
-            ent["routed"] = None if routed is None else int(sum(routed))
-            out["_verdicts"] = routed
         log(f"routing {strategy} {ent.get('tile_config', '')}: {secs:.3f} s for {k} steps, "
             f"{ent['value'] / 1e9:.3f} G seg-substeps/s, device {c['last_step_gpu_ms']:.1f} ms over "
             f"{launches} launches")
@@ -163,10 +169,23 @@ def routing_leg(w: scenes.Workload, steps: int, plain_steps: int, warmup: int, t
     return out
 
 
-def routing_fp64_leg(w: scenes.Workload, steps: int, warmup: int) -> dict:
-    """The routing workload in an FP64 batch (the reference CPU solver's own
-    precision on the GPU): throughput and routed verdicts."""
-    b = make_ragged_batch(w, "tiled", precision=64)
+PLAIN_WINDOW_STARTS = (0, 1500, 3000, 4500)
+PLAIN_WINDOW_STEPS = 100
+
+
+def full_rollout_leg(w: scenes.Workload, precision: int, warmup: int, tile_config: int = 0,
+                     plain_windows: bool = True) -> tuple:
+    """The whole 6000-step routing rollout, whatever --steps is. TILED runs it
+    as one call (one launch), timed by wall clock and by HIP events on the
+    batch's stream; routed verdicts come from its final positions. PLAIN (one
+    launch per step) runs four stated windows of 100 steps spread over the
+    rollout, each started from the state a second TILED batch reaches at that
+    step (the two strategies are bitwise equal, so this is PLAIN's own state
+    there; each window's end state is checked against TILED's bit for bit).
+    PLAIN's rollout rate is taken as its rate over the windows."""
+    steps = w.steps
+    task = w.info["task"]
+    b = make_ragged_batch(w, "tiled", tile_config, precision)
     reset_ragged(b, w)
     b.step(w.params, warmup)
     b.synchronize()
@@ -175,19 +194,58 @@ def routing_fp64_leg(w: scenes.Workload, steps: int, warmup: int) -> dict:
     secs, c = time_steps(b, w.params, steps)
     work = float(w.segments_total) * w.params.substeps * steps
     launches = max(1, c["last_step_launches"])
-    ent = {"value": work / secs, "unit": "segment-substeps/s", "dtype": "f64", "steps": steps, "seconds": secs,
-           "kernel_ms_avg": c["last_step_gpu_ms"] / launches, "launches": launches,
-           "tile_config": c.get("tile_config_name"), "workgroups": c["workgroups"],
-           "self_contact_overflow": c["self_contact_overflow"]}
-    verdicts = None
-    if steps == w.steps:
-        task = w.info["task"]
-        verdicts = [task.evaluate(b.download(i)["x"])["routed"] for i in range(w.num_rods)]
-        ent["routed"] = int(sum(verdicts))
-    log(f"routing FP64 {ent['tile_config']}: {secs:.3f} s for {steps} steps, {ent['value'] / 1e9:.3f} G "
-        f"seg-substeps/s")
+    kernel_ms = c["last_step_gpu_ms"] / launches
+    x = b.get_state()["x"]
+    ps = b.particle_start
+    if not np.all(np.isfinite(x)):
+        raise RuntimeError("non-finite state after the routing rollout")
+    verdicts = [task.evaluate(x[ps[i]:ps[i + 1]].astype(np.float64))["routed"] for i in range(w.num_rods)]
+    cfg = c["tile_config"]
+    tiled = {"value": work / secs, "unit": "segment-substeps/s", "steps": steps, "seconds": secs,
+             "device_ms": c["last_step_gpu_ms"], "launches": launches, "kernel_ms_avg": kernel_ms,
+             "segment_substeps_per_launch": work / launches, "tile_config": c.get("tile_config_name"),
+             "workgroups": c["workgroups"], "self_contact_overflow": c["self_contact_overflow"],
+             "motion_margin": c["motion_margin"], "low_margin_steps": c["low_margin_steps"],
+             "routed": int(sum(verdicts))}
+    log(f"routing rollout FP{precision} TILED {tiled['tile_config']}: {secs:.3f} s for {steps} steps in {launches} "
+        f"launch(es), {tiled['value'] / 1e9:.3f} G seg-substeps/s, routed {tiled['routed']} of {w.num_rods}")
     b.destroy()
-    return ent, verdicts
+    out = {"dtype": f"f{precision}", "tiled": tiled}
+    if plain_windows:
+        snap = make_ragged_batch(w, "tiled", cfg, precision)
+        pl = make_ragged_batch(w, "plain", 0, precision)
+        reset_ragged(pl, w)
+        pl.step(w.params, PLAIN_WINDOW_STEPS)  # graph capture and code objects
+        pl.synchronize()
+        pl.set_timing(True)
+        reset_ragged(snap, w)
+        done, plain_secs, plain_dev, equal = 0, 0.0, 0.0, True
+        for s0 in PLAIN_WINDOW_STARTS:
+            snap.step(w.params, s0 - done)
+            st = snap.get_state()
+            pl.set_state(0, w.num_rods, **st)
+            pl.set_clock(clock=snap.get_clock())
+            wsecs, wc = time_steps(pl, w.params, PLAIN_WINDOW_STEPS)
+            plain_secs += wsecs
+            plain_dev += wc["last_step_gpu_ms"]
+            snap.step(w.params, PLAIN_WINDOW_STEPS)
+            done = s0 + PLAIN_WINDOW_STEPS
+            a, e = pl.get_state(), snap.get_state()
+            equal = equal and all(np.array_equal(np.ascontiguousarray(a[k]).view(np.uint8),
+                                                 np.ascontiguousarray(e[k]).view(np.uint8)) for k in a)
+        wwork = float(w.segments_total) * w.params.substeps * PLAIN_WINDOW_STEPS * len(PLAIN_WINDOW_STARTS)
+        out["plain_windows"] = {
+            "value": wwork / plain_secs, "unit": "segment-substeps/s",
+            "windows": [f"{s0}-{s0 + PLAIN_WINDOW_STEPS - 1}" for s0 in PLAIN_WINDOW_STARTS],
+            "steps": PLAIN_WINDOW_STEPS * len(PLAIN_WINDOW_STARTS), "seconds": plain_secs,
+            "device_ms": plain_dev, "bitwise_equal_tiled_at_window_ends": bool(equal)}
+        out["speedup_vs_plain"] = tiled["value"] / out["plain_windows"]["value"]
+        log(f"routing FP{precision} PLAIN over windows {out['plain_windows']['windows']}: "
+            f"{out['plain_windows']['value'] / 1e9:.4f} G seg-substeps/s; TILED rollout / PLAIN "
+            f"{out['speedup_vs_plain']:.1f}x; window ends bitwise equal to TILED: {equal}")
+        snap.destroy()
+        pl.destroy()
+    return out, verdicts
 
 
 def trajectory_leg(w: scenes.Workload, every: int, warmup: int) -> dict:
@@ -315,50 +373,106 @@ def cpu_baseline_routing(w: scenes.Workload, stride: int, steps: int, threads: i
             "seconds": dt}
 
 
+def cpu_model() -> str:
+    """The host CPU's model name and logical core count."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name} ({os.cpu_count()} logical CPUs visible)"
+
+
+# ------------------------------------------------------------------ long rods (GLOBAL strategy)
+
+def global_leg(num_rods: int, warmup: int, segments: int = 2000, steps: int = 200) -> dict:
+    """Rods too long for a workgroup's LDS: `num_rods` clamped rods of 2000
+    segments (4 m) over a floor with friction, gravity, dt = 1e-3, 10
+    substeps, stepped by the GLOBAL strategy (working arrays in HBM, one rod
+    per workgroup, many steps per launch). The reference runs such rods with
+    its multi-kernel strategy only."""
+    mat = scenes.REFERENCE_MATERIAL
+    rod = rp.make_straight_rod(segments, 4.0, mat, (0.0, 0.0, 0.1), (1.0, 0.0, 0.0))
+    rp.clamp_root_exact(rod)
+    world = rp.CollisionWorld(primitives=[rp.Primitive.plane((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 0.6)])
+    params = rp.SolverParams(dt=1e-3, substeps=10, iterations=1, gravity=(0.0, 0.0, -9.81))
+    b = rp.Batch(rod, num_rods, strategy="auto", world=world)
+    b.step(params, warmup)
+    b.synchronize()
+    b.upload(rod)
+    b.set_clock()
+    b.set_timing(True)
+    secs, c = time_steps(b, params, steps)
+    work = float(num_rods) * segments * params.substeps * steps
+    launches = max(1, c["last_step_launches"])
+    out = {"config": {"rods": num_rods, "segments": segments, "length_m": 4.0, "dt": 1e-3, "substeps": 10,
+                      "steps": steps, "contacts": "floor plane, Coulomb friction"},
+           "strategy": c["strategy"], "value": work / secs, "unit": "segment-substeps/s", "seconds": secs,
+           "kernel_ms_avg": c["last_step_gpu_ms"] / launches, "launches": launches,
+           "scratch_bytes": c["scratch_bytes"], "lanes_per_workgroup": c["lanes_total"] // max(1, c["workgroups"])}
+    log(f"long rods ({c['strategy']}): {num_rods} x {segments} segments, {steps} steps in {launches} launches, "
+        f"{out['value'] / 1e9:.3f} G seg-substeps/s, {c['scratch_bytes'] / 1e6:.0f} MB of HBM working arrays")
+    b.destroy()
+    return out
+
+
 # ------------------------------------------------------------------ serving stream (latency)
 
-def serving_leg(num: int, load: float, budget_gib: float, slice_us: int) -> dict:
-    """The serving workload: `num` requests through the server in its two
-    modes. Chunked steps (64 steps per server step, every resident request in
-    lockstep): its capacity with every request arriving at once sets the
+def serving_leg(num: int, load: float, budget_gib: float, slice_us: int, capacity_num: int = 40) -> dict:
+    """The serving workload through the server in its two modes. Chunked
+    steps (64 steps per server step, every resident request in lockstep): its
+    capacity with the first `capacity_num` requests arriving at once sets the
     Poisson arrival rate, `load` times that capacity. Time-sliced steps
     (`slice_us` per server step, each workgroup running the steps it starts in
-    the slice): its own capacity, then the same Poisson stream at the same rate.
-    Latency is from arrival to result on the host."""
+    the slice): its own capacity on the same requests, then `num` requests
+    arriving as a Poisson stream at that rate (the headline p50/p99); the
+    chunked mode serves the first `capacity_num` of the same stream for
+    comparison. Latency is from arrival to result on the host."""
     from ropephysics_amd import serving
 
     reqs = serving.make_requests(num)
-    work = sum(q.work for q in reqs)
+    arrivals = serving.poisson_arrivals(num, 1.0)  # scaled to the rate below
     horizon = np.array([q.steps * serving.PARAMS.dt for q in reqs])
-    out = {"requests": num, "segments_mean": float(np.mean([q.segments for q in reqs])),
+    out = {"requests": num, "share_of_stream": num / 10000.0,
+           "segments_mean": float(np.mean([q.segments for q in reqs])),
            "rods_mean": float(np.mean([q.info["rods"] for q in reqs])),
            "horizon_s_mean": float(horizon.mean()), "self_collision_share": float(np.mean(
-               [q.info["self_collision"] for q in reqs])), "load": load}
+               [q.info["self_collision"] for q in reqs])), "load": load, "capacity_requests": capacity_num}
     rate = None
     for mode, kw in (("chunked", {}), ("time_sliced", {"time_slice_us": slice_us})):
         server = rp.Server(serving.PARAMS, hbm_budget_bytes=int(budget_gib * (1 << 30)), **kw)
         serving.run_stream(server, serving.make_requests(2, seed=5), np.zeros(2))  # code objects, first launches
-        sat = serving.run_stream(server, reqs, np.zeros(num))
-        capacity = num / sat["wall"]
+        head = reqs[:capacity_num]
+        sat = serving.run_stream(server, head, np.zeros(len(head)))
+        capacity = len(head) / sat["wall"]
         if rate is None:
             rate = load * capacity
-        run = serving.run_stream(server, reqs, serving.poisson_arrivals(num, rate))
+        n = num if mode == "time_sliced" else len(head)
+        run = serving.run_stream(server, reqs[:n], arrivals[:n] / rate)
         lat = run["latency"]
+        work_sat = sum(q.work for q in head)
+        work = sum(q.work for q in reqs[:n])
         out["hbm_budget_bytes"] = server.stats()["device_bytes"]
         out[mode] = {
-            "saturated": {"wall_s": sat["wall"], "value": work / sat["wall"], "unit": "segment-substeps/s",
-                          "requests_per_s": capacity, "launches": sat["launches"]},
-            "poisson": {"rate_requests_per_s": rate, "wall_s": run["wall"], "value": work / run["wall"],
+            "saturated": {"requests": len(head), "wall_s": sat["wall"], "value": work_sat / sat["wall"],
+                          "unit": "segment-substeps/s", "requests_per_s": capacity, "launches": sat["launches"]},
+            "poisson": {"requests": n, "rate_requests_per_s": rate, "wall_s": run["wall"],
+                        "value": work / run["wall"],
                         "unit": "segment-substeps/s", "p50_latency_s": float(np.percentile(lat, 50)),
                         "p99_latency_s": float(np.percentile(lat, 99)), "max_latency_s": float(lat.max()),
-                        "p50_latency_over_horizon": float(np.percentile(lat / horizon, 50)),
+                        "p50_latency_over_horizon": float(np.percentile(lat / horizon[:n], 50)),
                         "max_resident_requests": run["max_resident"], "launches": run["launches"]},
         }
         if kw:
             out[mode]["time_slice_us"] = slice_us
-        log(f"serving {mode}: {num} requests; saturated {work / sat['wall'] / 1e9:.3f} G seg-substeps/s "
-            f"({capacity:.2f} req/s); Poisson at {rate:.2f} req/s: p50 {out[mode]['poisson']['p50_latency_s']:.3f} s, "
-            f"p99 {out[mode]['poisson']['p99_latency_s']:.3f} s, {work / run['wall'] / 1e9:.3f} G seg-substeps/s")
+        log(f"serving {mode}: saturated on {len(head)} requests {work_sat / sat['wall'] / 1e9:.3f} G "
+            f"seg-substeps/s ({capacity:.2f} req/s); {n} requests Poisson at {rate:.2f} req/s: p50 "
+            f"{out[mode]['poisson']['p50_latency_s']:.3f} s, p99 {out[mode]['poisson']['p99_latency_s']:.3f} s, "
+            f"{work / run['wall'] / 1e9:.3f} G seg-substeps/s")
         server.destroy()
     return out
 
@@ -561,7 +675,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed routing steps (default: the whole 6000)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--plain-steps", type=int, default=None, help="PLAIN timed routing steps (default: --steps)")
+    ap.add_argument("--plain-steps", type=int, default=None,
+                    help="PLAIN timed routing steps of the headline window (default: --steps when at most 400; "
+                         "longer windows take PLAIN's rate from the full-rollout leg's windows)")
     ap.add_argument("--rods", type=int, default=1024, help="routing cables")
     ap.add_argument("--tile-config", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true")
@@ -570,7 +686,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6000, help="routing steps of the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--fp64-routing", type=int, default=1, help="also run the routing workload in FP64 (0: skip)")
+    ap.add_argument("--fp64-routing", type=int, default=1, help="also run the routing rollout in FP64 (0: skip)")
+    ap.add_argument("--cpu-1thread-stride", type=int, default=128,
+                    help="1-thread CPU baseline: every this-many-th routing cable over the whole rollout")
+    ap.add_argument("--global-rods", type=int, default=512, help="long-rod (GLOBAL) leg: rods of 2000 segments "
+                                                                 "(0: skip)")
     ap.add_argument("--coloured-routing", type=int, default=1,
                     help="also run the routing workload with the coloured self-contact order (0: skip)")
     ap.add_argument("--drape-steps", type=int, default=None, help="steps of the hold/slide sweep (default: the "
@@ -580,7 +700,10 @@ def main():
                     "(0: skip)")
     ap.add_argument("--cem-iterations", type=int, default=10, help="harness-learning iterations (0: skip)")
     ap.add_argument("--cem-population", type=int, default=1024)
-    ap.add_argument("--serve-requests", type=int, default=40, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="requests of the time-sliced Poisson stream (0: skip the serving leg)")
+    ap.add_argument("--serve-capacity-requests", type=int, default=40,
+                    help="requests of the capacity runs and of the chunked Poisson stream")
     ap.add_argument("--serve-load", type=float, default=0.8,
                     help="Poisson rate as a fraction of the chunked server's capacity")
     ap.add_argument("--serve-slice-us", type=int, default=50000, help="time slice of the time-sliced server mode")
@@ -610,38 +733,47 @@ def main():
     log("device", info)
     w = scenes.routing_rollouts(num_rods=args.rods)
     K = args.steps if args.steps is not None else w.steps
-    Kp = args.plain_steps if args.plain_steps is not None else K
+    Kp = args.plain_steps if args.plain_steps is not None else (K if K <= 400 else 0)
     W = args.warmup
     log(f"workload {w.name}: {w.num_rods} cables, {w.segments_total} segments, K={K} (PLAIN {Kp}), W={W}")
 
     leg = routing_leg(w, K, Kp, W, args.tile_config, args.no_plain)
-    leg.setdefault("_verdicts", None)
     t = leg["tiled"]
-    work = float(w.segments_total) * w.params.substeps * K
-    kernel_rate = (work / t["launches"]) / (t["kernel_ms_avg"] * 1e-3)
-    roof = roofline(FLOPS_PER_SEGMENT_SUBSTEP_ROUTING, kernel_rate, t["kernel_ms_avg"], work / t["launches"],
-                    "routing" if not args.tile_config else None)
-
-    r64, v64 = routing_fp64_leg(w, K, W) if args.fp64_routing else (None, None)
-    rcol, vcol = routing_coloured_leg(w, K, W, leg.get("_verdicts")) if args.coloured_routing else (None, None)
+    # The whole rollout in FP32 and FP64 (verdicts, PLAIN windows), whatever K is.
+    full32, gv = full_rollout_leg(w, 32, W, args.tile_config, not args.no_plain)
+    full64, v64 = full_rollout_leg(w, 64, W, 0, not args.no_plain) if args.fp64_routing else (None, None)
+    ft = full32["tiled"]
+    kernel_rate = ft["segment_substeps_per_launch"] / (ft["kernel_ms_avg"] * 1e-3)
+    roof = roofline(FLOPS_PER_SEGMENT_SUBSTEP_ROUTING, kernel_rate, ft["kernel_ms_avg"],
+                    ft["segment_substeps_per_launch"], "routing" if not args.tile_config else None)
+    roof["launch"] = f"the full {w.steps}-step routing rollout (full_rollout.fp32.tiled)"
+    rcol, vcol = routing_coloured_leg(w, w.steps, W, gv) if args.coloured_routing else (None, None)
     uniform = uniform_leg(args.uniform_steps, W, args.no_plain) if args.uniform_steps > 0 else None
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     drape = None
     if args.drape_steps != 0:
         drape = drape_leg(args.drape_steps, W, 0 if args.no_cpu else args.drape_cpu_rods, threads)
-    traj = trajectory_leg(w, args.record_every, W) if args.record_every > 0 and K == w.steps else None
+    traj = trajectory_leg(w, args.record_every, W) if args.record_every > 0 else None
+    glob = global_leg(args.global_rods, W) if args.global_rods > 0 else None
     cem = cem_leg(args.cem_iterations, args.cem_population, not args.no_cpu) if args.cem_iterations > 0 else None
-    serving_out = serving_leg(args.serve_requests, args.serve_load, args.serve_budget_gib, args.serve_slice_us) \
+    serving_out = serving_leg(args.serve_requests, args.serve_load, args.serve_budget_gib, args.serve_slice_us,
+                              min(args.serve_capacity_requests, args.serve_requests)) \
         if args.serve_requests > 0 else None
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_routing(w, args.cpu_stride, min(args.cpu_steps, w.steps), threads)
+        cpu["cpu_model"] = cpu_model()
         log(f"CPU baseline: {cpu['value'] / 1e6:.2f} M seg-substeps/s ({cpu['sample']})")
-        cv, gv = cpu.pop("_verdicts"), leg.get("_verdicts")
+        one = cpu_baseline_routing(w, args.cpu_1thread_stride, w.steps, 1)
+        one.pop("_verdicts", None)
+        one["cpu_model"] = cpu["cpu_model"]
+        cpu["one_thread"] = one
+        log(f"CPU baseline, 1 thread: {one['value'] / 1e6:.2f} M seg-substeps/s ({one['sample']})")
+        cv = cpu.pop("_verdicts")
         if cv is not None and gv is not None:
             agree = sum(1 for r, v in cv.items() if gv[r] == v)
-            t["verdicts_vs_reference_cpu"] = {"sampled_cables": len(cv), "agree": agree,
+            ft["verdicts_vs_reference_cpu"] = {"sampled_cables": len(cv), "agree": agree,
                                               "routed_gpu": sum(1 for r in cv if gv[r]),
                                               "routed_cpu": sum(1 for v in cv.values() if v)}
             log(f"routed verdicts, GPU FP32 vs reference CPU FP64 on {len(cv)} cables: {agree} agree")
@@ -650,24 +782,27 @@ def main():
                                                      "agree": sum(1 for r, v in cv.items() if vcol[r] == v)}
                 log(f"routed verdicts, coloured self order vs reference CPU FP64 (list order) on {len(cv)} "
                     f"cables: {rcol['verdicts_vs_reference_cpu']['agree']} agree")
-        if v64 is not None and K == w.steps:
+        if v64 is not None and args.cpu_steps >= w.steps:
             col = cpu_baseline_routing(w, args.cpu_stride, w.steps, threads, coloured=True)
             cc = col.pop("_verdicts")
-            r64["verdicts_vs_reference_cpu_colour_order"] = {
+            full64["verdicts_vs_reference_cpu_colour_order"] = {
                 "sampled_cables": len(cc), "agree_fp64": sum(1 for r, v in cc.items() if v64[r] == v),
                 "agree_fp32": sum(1 for r, v in cc.items() if gv is not None and gv[r] == v),
                 "routed_cpu": sum(1 for v in cc.values() if v), "cpu_seconds": col["seconds"]}
             log(f"routed verdicts vs reference CPU FP64 in colour order on {len(cc)} cables: FP64 GPU "
-                f"{r64['verdicts_vs_reference_cpu_colour_order']['agree_fp64']} agree, FP32 GPU "
-                f"{r64['verdicts_vs_reference_cpu_colour_order']['agree_fp32']} agree")
+                f"{full64['verdicts_vs_reference_cpu_colour_order']['agree_fp64']} agree, FP32 GPU "
+                f"{full64['verdicts_vs_reference_cpu_colour_order']['agree_fp32']} agree")
 
     plain = leg.get("plain")
-    leg.pop("_verdicts", None)
+    if plain is None and "plain_windows" in full32 and K == w.steps:
+        plain = full32["plain_windows"]
     out = {
         "metric": METRIC, "value": t["value"], "unit": "segment-substeps/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": t["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded)",
         "config": {"workload": w.name, **{k: v for k, v in w.info.items() if k not in ("paths", "task")},
+                   "timed_steps": f"steps 0-{K - 1} of the {w.steps}-step rollout (after {W} untimed warm-up "
+                                  f"steps and a reset to step 0)",
                    "strategy": "tiled (ragged-packed persistent launches, hipGraph-captured)",
                    "parallelism": "single GPU"},
         "tiled": t,
@@ -675,12 +810,13 @@ def main():
         "speedup_vs_plain": (t["value"] / plain["value"]) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
-        "routing_fp64": r64,
+        "full_rollout": {"fp32": full32, "fp64": full64},
         "routing_coloured_self_order": rcol,
         "uniform": uniform,
         "drape": drape,
         "harness_learning": cem,
         "trajectory": traj,
+        "global_long_rods": glob,
         "serving": serving_out,
         "p50_latency_s": serving_out["time_sliced"]["poisson"]["p50_latency_s"] if serving_out else None,
         "p99_latency_s": serving_out["time_sliced"]["poisson"]["p99_latency_s"] if serving_out else None,
