@@ -128,14 +128,16 @@ def test_motion_margin_is_reported():
     coord = max(float(np.max(np.abs(np.float32(r.x)))) for r in rods)
     c32 = np.float32(coord)
     spacing = float(np.nextafter(c32, np.float32(np.inf)) - c32)
-    h = w.params.dt / w.params.substeps
+    # The step parameters as an FP32 batch receives them.
+    h = float(np.float32(w.params.dt)) / w.params.substeps
+    g = abs(float(np.float32(w.params.gravity[2])))
     for prec in (32, 64):
         b = rp.Batch(rods, strategy="tiled", world=w.world, precision=prec)
         assert b.counters()["motion_margin"] == 0.0
         b.step(w.params, 3)
         c = b.counters()
         if prec == 32:
-            assert c["motion_margin"] == pytest.approx(9.81 * h * h / spacing, rel=1e-12)
+            assert c["motion_margin"] == pytest.approx(g * h * h / spacing, rel=1e-12)
             assert 0.3 < c["motion_margin"] < 0.5 and c["low_margin_steps"] == 3
         else:
             assert c["motion_margin"] > 1e7 and c["low_margin_steps"] == 0
