@@ -220,6 +220,29 @@ int rp_batch_set_scales(void* batch, const float* youngs_scale, const float* fri
  * Results do not depend on how steps are split into calls or launches. */
 int rp_batch_step(void* batch, const rp_step_params* params, int num_steps);
 
+/* rp_batch_step without device allocation. No counterpart in the reference:
+ * for callers that cannot call hipMalloc -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid it.
+ *   - Steps an FP32 batch of the GLOBAL strategy exactly as rp_batch_step does
+ *     (the same kernel, the same results bit for bit), but with the rods'
+ *     working arrays in a fixed region of RP_NOALLOC_SCRATCH_BYTES that the
+ *     library holds on each device from the first call made with that device
+ *     current until the process ends, instead of the batch's own (which this
+ *     call never allocates). Launches go straight onto the batch's stream; no
+ *     graph is captured or instantiated; nothing is allocated or freed.
+ *   - Refused: batches whose working arrays need more than the region
+ *     (RP_ERR_CAPACITY; rp_counters cannot tell before the first
+ *     rp_batch_step, so the message gives the size), other strategies, FP64
+ *     batches, and a current device other than the batch's (RP_ERR_STATE).
+ *   - Every call on a device shares that device's region. Calls on one
+ *     batch's stream are ordered; calls whose work overlaps on one device
+ *     (different batches' streams, several host threads) race on the region
+ *     and must be serialised by the caller. The kernel writes every word of
+ *     the region it reads within each launch, so the region needs no clearing
+ *     between calls. */
+#define RP_NOALLOC_SCRATCH_BYTES (32u << 20)
+int rp_batch_step_noalloc(void* batch, const rp_step_params* params, int num_steps);
+
 /* Batch::synchronize (src/gpu/gpu_batch.cpp:475), scoped to this batch. */
 int rp_batch_sync(void* batch);
 

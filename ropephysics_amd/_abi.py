@@ -116,6 +116,7 @@ SIGNATURES = {
     "rp_batch_set_loads": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _fp, _fp, _fp, _fp]),
     "rp_batch_set_scales": (C.c_int, [C.c_void_p, _fp, _fp]),
     "rp_batch_step": (C.c_int, [C.c_void_p, C.POINTER(RpStepParams), C.c_int]),
+    "rp_batch_step_noalloc": (C.c_int, [C.c_void_p, C.POINTER(RpStepParams), C.c_int]),
     "rp_batch_sync": (C.c_int, [C.c_void_p]),
     "rp_batch_get_state": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _fp, _fp, _fp, _fp]),
     "rp_batch_counters": (C.c_int, [C.c_void_p, C.POINTER(RpCounters)]),

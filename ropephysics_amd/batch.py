@@ -268,6 +268,12 @@ class Batch:
         sp = step_params(params, self.precision)
         _abi.check(self._p.fn("rp_batch_step")(self._h, C.byref(sp), int(steps)))
 
+    def step_noalloc(self, params: SolverParams, steps: int = 1):
+        """rp_batch_step_noalloc: a GLOBAL FP32 batch stepped with its working
+        arrays in the library's fixed per-device region; allocates nothing."""
+        sp = step_params(params, 32)
+        _abi.check(_abi.lib().rp_batch_step_noalloc(self._h, C.byref(sp), int(steps)))
+
     # ------------------------------------------------------------------ drivers and observers
 
     def set_paths(self, first: int, paths: list):

@@ -59,6 +59,7 @@ struct GraphKey {
 // What a batch holds whatever its precision.
 struct rp_batch_impl {
     int bits = 32;  // arithmetic precision
+    int device = 0;  // current when the batch was created
     int numRods = 0;
     std::vector<int> pStart, fStart, sStart, bStart;
     std::vector<unsigned char> tileable;
@@ -413,6 +414,7 @@ int create(const typename Prec<Real>::BatchDesc* desc, void** out_batch) {
     if (rc) return rc;
     const int R = rec.R, P = rec.P, F = rec.F;
     auto b = std::make_unique<BatchT<Real>>();
+    RP_HIP(hipGetDevice(&b->device));
     b->numRods = R;
     b->requestedConfig = desc->tile_config;
     b->pStart = rec.pStart;
@@ -658,6 +660,17 @@ int set_scales(void* batch, const Real* youngs_scale, const Real* friction_scale
     return RP_OK;
 }
 
+// GLOBAL: about 2^22 element-substeps of the longest rod per launch (a few
+// hundred milliseconds), at least one step; RP_STEPS_PER_LAUNCH replaces it.
+template <typename Real>
+int global_steps_per_launch(const BatchT<Real>& b, const typename Prec<Real>::StepParams& p) {
+    if (b.stepsPerLaunchSet) return b.stepsPerLaunch;
+    int widest = 1;
+    for (const RodInfoT<Real>& ri : b.rods) widest = std::max(widest, std::max(ri.nP, ri.nF));
+    const double perStep = double(widest) * p.substeps * std::max(1, p.iterations);
+    return int(std::max(1.0, std::min(double(b.stepsPerLaunch), std::floor(double(1 << 22) / perStep))));
+}
+
 // The reference's float motion margin (HangingCable::floatMotionMargin,
 // src/apps/cable_hanging.cpp:75-81, required to be at least 2 by
 // application_cases.cpp:196-197): gravity's displacement per substep, |g| h^2,
@@ -701,14 +714,7 @@ int step(void* batch, const typename Prec<Real>::StepParams* params, int num_ste
     // or plainStepsPerGraph PLAIN launches. Each chunk shape is captured once
     // into a HIP graph (cached per parameter set and length) and replayed.
     int chunk = b.strategy == RP_STRATEGY_PLAIN ? b.plainStepsPerGraph : b.stepsPerLaunch;
-    if (b.strategy == RP_STRATEGY_GLOBAL && !b.stepsPerLaunchSet) {
-        // About 2^22 element-substeps of the longest rod per launch (a few
-        // hundred milliseconds), at least one step.
-        int widest = 1;
-        for (const RodInfoT<Real>& ri : b.rods) widest = std::max(widest, std::max(ri.nP, ri.nF));
-        const double perStep = double(widest) * p.substeps * std::max(1, p.iterations);
-        chunk = int(std::max(1.0, std::min(double(chunk), std::floor(double(1 << 22) / perStep))));
-    }
+    if (b.strategy == RP_STRATEGY_GLOBAL) chunk = global_steps_per_launch(b, p);
     if (b.strategy == RP_STRATEGY_TILED && !b.stepsPerLaunchSet && b.tileMsPerSweep > 0.0) {
         // Hold a launch to about kLaunchBudgetMs of estimated device time (at
         // least 64 steps), so a very large batch still returns to the host
@@ -864,6 +870,55 @@ int rp_batch_step(void* batch, const rp_step_params* params, int num_steps) {
 }
 int rp_batch_step64(void* batch, const rp_step_params64* params, int num_steps) {
     return step<double>(batch, params, num_steps);
+}
+
+// rp_batch_step for a GLOBAL FP32 batch with the working arrays in the
+// library's fixed per-device region instead of the batch's own: every launch
+// goes straight onto the batch's stream (no graph is captured or
+// instantiated) and nothing is allocated.
+int rp_batch_step_noalloc(void* batch, const rp_step_params* params, int num_steps) {
+    if (!params) return fail(RP_ERR_INVALID, "null argument");
+    BatchT<float>* bp;
+    int rc = as_precision(batch, bp);
+    if (rc) return rc;
+    BatchT<float>& b = *bp;
+    const rp_step_params& p = *params;
+    if (b.strategy != RP_STRATEGY_GLOBAL)
+        return fail(RP_ERR_STATE, "rp_batch_step_noalloc steps GLOBAL batches only (create with RP_STRATEGY_GLOBAL)");
+    if (num_steps < 0) return fail(RP_ERR_INVALID, "num_steps must be non-negative");
+    if (p.substeps < 1 || p.iterations < 0 || !(p.dt > 0))
+        return fail(RP_ERR_INVALID, "need substeps >= 1, iterations >= 0 and dt > 0");
+    int dev = 0;
+    RP_HIP(hipGetDevice(&dev));
+    if (dev != b.device)
+        return fail(RP_ERR_STATE, "the batch's device (" + std::to_string(b.device) + ") must be current, not " +
+                                      std::to_string(dev));
+    char* region = nullptr;
+    size_t cap = 0;
+    RP_HIP(f32::noalloc_scratch(&region, &cap));
+    if (b.scratchBytes > cap)
+        return fail(RP_ERR_CAPACITY, "the batch's working arrays need " + std::to_string(b.scratchBytes) +
+                                         " bytes; rp_batch_step_noalloc has " + std::to_string(cap) +
+                                         " (RP_NOALLOC_SCRATCH_BYTES)");
+    if (num_steps == 0) return RP_OK;
+    note_motion_margin<float>(b, p, num_steps);
+    DeviceArraysT<float> d = b.d;
+    d.scratch = region;
+    const int chunk = global_steps_per_launch(b, p);
+    if (b.timing) RP_HIP(hipEventRecord(b.ev0, b.stream));
+    int launches = 0;
+    for (int left = num_steps; left > 0; ++launches) {
+        const int n = std::min(left, chunk);
+        RP_HIP(Prec<float>::launch_global(d, step_args<float>(p, n), b.globalThreads, b.stream));
+        left -= n;
+    }
+    if (b.timing) {
+        RP_HIP(hipEventRecord(b.ev1, b.stream));
+        b.evPending = true;
+    }
+    b.lastLaunches = launches;
+    b.stepsTaken += num_steps;
+    return RP_OK;
 }
 
 int rp_batch_get_state(void* batch, int first_rod, int num_rods, float* x, float* v, float* q, float* omega) {

@@ -30,6 +30,9 @@ namespace rp {
     hipError_t launch_fixed_frames(R* q, const R* src, const R* invInertia4, int f0, int count, hipStream_t s);
 namespace f32 {
 RP_KERNEL_API(float)
+// Device address and size of the current device's fixed GLOBAL working-array
+// region (rp_batch_step_noalloc); looked up once per device, allocates nothing.
+hipError_t noalloc_scratch(char** out, size_t* bytes);
 }
 namespace f64 {
 RP_KERNEL_API(double)

@@ -18,6 +18,8 @@
 //                 launch without padding to the longest.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "rp_layout.h"
 #include "rp_math.h"
 #include "rp_phases.h"
@@ -806,6 +808,39 @@ bool tiled_shape_available(int tp, int tc, int threads) {
     if (tc == 1) return tp >= 1 && tp <= 3;
     return !RP_FP64 && tc == 2 && (tp == 3 || tp == 5);
 }
+
+#if !RP_FP64
+// The working arrays of GLOBAL steps taken through rp_batch_step_noalloc, for
+// callers that cannot allocate device memory: a fixed region of the code
+// object, one copy per device, held until the process ends. Its name is a
+// host-side stand-in; host code uses the device address noalloc_scratch gives.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+// Device address of the current device's `scratch`, looked up with
+// hipGetSymbolAddress on the first call made with that device current and
+// kept per device. Allocates nothing.
+constexpr int kNoAllocMaxDevices = 64;
+
+hipError_t noalloc_scratch(char** out, size_t* bytes) {
+    static std::mutex lock;
+    static char* addr[kNoAllocMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> guard(lock);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = static_cast<char*>(p);
+    }
+    *out = addr[dev];
+    *bytes = kNoAllocScratchBytes;
+    return hipSuccess;
+}
+#endif
 
 hipError_t launch_fixed_frames(real* q, const real* src, const real* invInertia4, int f0, int count,
                                hipStream_t s) {
