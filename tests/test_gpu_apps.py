@@ -168,3 +168,47 @@ def test_harness_learning_cross_entropy():
     assert a.learned_rate >= 0.8
     assert a.learned_rate - a.hand_rate >= 0.3
     assert a.mean.shape == (8, 2)
+
+
+def test_harness_learning_verdicts_agree_with_cpu():
+    """The reference's learning case reruns learned and hand-written attempts
+    on the CPU and requires the same routed verdicts as on the GPU
+    (application_cases.cpp:453-480). Here: after 4 iterations of 128
+    rollouts, the softest and the stiffest fresh cable under each motion
+    (four attempts) are rerun by the FP64 oracle in colour order with the
+    same stiffness, friction and gripper schedule; the GPU's verdicts agree
+    on all four, and the four hold both outcomes."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from ropephysics_amd.learning import HarnessLearner
+
+    learner = HarnessLearner(population=128, seed=11)
+    res = learner.run(iterations=4)
+    task = learner.task
+    ys, fr = res.fresh["youngs"], res.fresh["friction"]
+    picks = [int(np.argmin(ys)), int(np.argmax(ys))]
+    attempts = [(r, name, wp) for r in picks for name, wp in (("learned", res.mean),
+                                                              ("hand", task.hand_written_waypoints()))]
+
+    def cpu(attempt):
+        r, name, wp = attempt
+        n = int(res.fresh["segments"][r])
+        rod = task.build(n)
+        o = oracle_twin(rod, 64)
+        a = o.get()
+        o.set({"sComp": a["sComp"] / float(ys[r]), "bComp": a["bComp"] / float(ys[r])})
+        ow = oracle_world(task.world())
+        for pr in ow.prims:
+            pr.friction = pr.friction * float(fr[r])
+        ow.self_friction = ow.self_friction * float(fr[r])
+        path = task.path(n, wp)
+        kv = np.array([[path.velocity(s) for s in range(learner.steps)]])
+        oracle.step_many_driven([o], oracle_params(task.params(), precision=64), [ow], learner.steps, 1,
+                                [path.particle], kv)
+        return task.evaluate(o.get()["x"])["routed"], res.fresh[name][r]
+
+    with ThreadPoolExecutor(4) as ex:
+        pairs = list(ex.map(cpu, attempts))
+    print("CPU vs GPU routed verdicts", pairs, flush=True)
+    assert all(c == g for c, g in pairs)
+    assert any(c for c, _ in pairs) and not all(c for c, _ in pairs)

@@ -172,3 +172,43 @@ def test_time_sliced_steps_give_the_same_results():
     assert all(v == 0 for v in d.values()), d
     with pytest.raises(rp.RopePhysicsError):
         rp.Server(PARAMS, hbm_budget_bytes=8 << 20, time_slice_us=-1)
+
+
+def test_admission_and_batch_share_the_packer_lds_bound():
+    """A rod whose LDS need lies just above what the packer gives one
+    workgroup is refused at submission (and by a TILED batch), never admitted
+    and left unpackable; at the largest pool the bound allows, the request is
+    admitted and steps. The pool size is found by bisection on TILED batch
+    creation."""
+    from ropephysics_amd import _abi
+
+    def world(cap):
+        return rp.CollisionWorld(primitives=[rp.Primitive.plane((0, 0, -0.02), (0, 0, 1), 0.5)],
+                                 self_collision=True, self_friction=0.3, hash_table_size=256, self_capacity=cap)
+
+    rod = benchmark_rod(100)
+
+    def tiled_ok(cap):
+        try:
+            rp.Batch([rod], strategy="tiled", world=world(cap), tile_config=rp.server.DEFAULT_TILE_CONFIG).destroy()
+            return True
+        except rp.RopePhysicsError as e:
+            assert e.code == _abi.RP_ERR_CAPACITY
+            return False
+
+    lo, hi = 16, 8000
+    assert tiled_ok(lo) and not tiled_ok(hi)
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        lo, hi = (mid, hi) if tiled_ok(mid) else (lo, mid)
+    for cap in range(hi, hi + 6):
+        assert not tiled_ok(cap)
+        server = rp.Server(PARAMS, hbm_budget_bytes=16 << 20)
+        with pytest.raises(rp.ServerFull):
+            server.submit([rod], world(cap), 8)
+    server = rp.Server(PARAMS, hbm_budget_bytes=16 << 20)
+    rid = server.submit([rod], world(lo), 8)
+    server.step()
+    server.synchronize()
+    assert server.finished() == [rid]
+    assert np.all(np.isfinite(server.result(rid)["x"]))
