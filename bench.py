@@ -31,9 +31,9 @@ stream (requests of 1-256 rods of 32-256 segments, 0.5-8 s horizons, random
 contact shapes, self-collision in half) through the continuous-batching
 server: the capacity of each mode (64-step chunks in lockstep; 50 ms time
 slices in which each workgroup runs the steps it can) on 40 requests arriving
-at once, then 200 requests arriving as a Poisson stream at 80% of the chunked
-capacity through the time-sliced server (the headline p50/p99) and the first
-40 of them through the chunked one.
+at once, then 200 requests arriving as a Poisson stream at 80% of the
+time-sliced mode's capacity through the time-sliced server (the headline
+p50/p99) and the first 40 of them, at the same rate, through the chunked one.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -423,57 +423,60 @@ def global_leg(num_rods: int, warmup: int, segments: int = 2000, steps: int = 20
 # ------------------------------------------------------------------ serving stream (latency)
 
 def serving_leg(num: int, load: float, budget_gib: float, slice_us: int, capacity_num: int = 40) -> dict:
-    """The serving workload through the server in its two modes. Chunked
-    steps (64 steps per server step, every resident request in lockstep): its
-    capacity with the first `capacity_num` requests arriving at once sets the
-    Poisson arrival rate, `load` times that capacity. Time-sliced steps
-    (`slice_us` per server step, each workgroup running the steps it starts in
-    the slice): its own capacity on the same requests, then `num` requests
-    arriving as a Poisson stream at that rate (the headline p50/p99); the
-    chunked mode serves the first `capacity_num` of the same stream for
-    comparison. Latency is from arrival to result on the host."""
+    """The serving workload through the server in its two modes: chunked
+    steps (64 steps per server step, every resident request in lockstep) and
+    time-sliced steps (`slice_us` per server step, each workgroup running the
+    steps it starts in the slice). First each mode's capacity with the first
+    `capacity_num` requests arriving at once; then `num` requests arriving as
+    a Poisson stream at `load` times the time-sliced mode's capacity through
+    the time-sliced server (the headline p50/p99), and the first
+    `capacity_num` of the same stream at the same rate through the chunked one.
+    Latency is from arrival to result on the host."""
     from ropephysics_amd import serving
 
     reqs = serving.make_requests(num)
     arrivals = serving.poisson_arrivals(num, 1.0)  # scaled to the rate below
     horizon = np.array([q.steps * serving.PARAMS.dt for q in reqs])
+    head = reqs[:capacity_num]
+    work_sat = sum(q.work for q in head)
     out = {"requests": num, "share_of_stream": num / 10000.0,
            "segments_mean": float(np.mean([q.segments for q in reqs])),
            "rods_mean": float(np.mean([q.info["rods"] for q in reqs])),
            "horizon_s_mean": float(horizon.mean()), "self_collision_share": float(np.mean(
-               [q.info["self_collision"] for q in reqs])), "load": load, "capacity_requests": capacity_num}
-    rate = None
-    for mode, kw in (("chunked", {}), ("time_sliced", {"time_slice_us": slice_us})):
+               [q.info["self_collision"] for q in reqs])), "load": load, "capacity_requests": capacity_num,
+           "rate_basis": "load x the time-sliced mode's saturated capacity"}
+    modes = (("chunked", {}), ("time_sliced", {"time_slice_us": slice_us}))
+    servers, capacity = {}, {}
+    for mode, kw in modes:
         server = rp.Server(serving.PARAMS, hbm_budget_bytes=int(budget_gib * (1 << 30)), **kw)
         serving.run_stream(server, serving.make_requests(2, seed=5), np.zeros(2))  # code objects, first launches
-        head = reqs[:capacity_num]
         sat = serving.run_stream(server, head, np.zeros(len(head)))
-        capacity = len(head) / sat["wall"]
-        if rate is None:
-            rate = load * capacity
-        n = num if mode == "time_sliced" else len(head)
-        run = serving.run_stream(server, reqs[:n], arrivals[:n] / rate)
-        lat = run["latency"]
-        work_sat = sum(q.work for q in head)
-        work = sum(q.work for q in reqs[:n])
+        capacity[mode] = len(head) / sat["wall"]
+        servers[mode] = server
         out["hbm_budget_bytes"] = server.stats()["device_bytes"]
-        out[mode] = {
-            "saturated": {"requests": len(head), "wall_s": sat["wall"], "value": work_sat / sat["wall"],
-                          "unit": "segment-substeps/s", "requests_per_s": capacity, "launches": sat["launches"]},
-            "poisson": {"requests": n, "rate_requests_per_s": rate, "wall_s": run["wall"],
-                        "value": work / run["wall"],
-                        "unit": "segment-substeps/s", "p50_latency_s": float(np.percentile(lat, 50)),
-                        "p99_latency_s": float(np.percentile(lat, 99)), "max_latency_s": float(lat.max()),
-                        "p50_latency_over_horizon": float(np.percentile(lat / horizon[:n], 50)),
-                        "max_resident_requests": run["max_resident"], "launches": run["launches"]},
-        }
+        out[mode] = {"saturated": {"requests": len(head), "wall_s": sat["wall"], "value": work_sat / sat["wall"],
+                                   "unit": "segment-substeps/s", "requests_per_s": capacity[mode],
+                                   "launches": sat["launches"]}}
         if kw:
             out[mode]["time_slice_us"] = slice_us
         log(f"serving {mode}: saturated on {len(head)} requests {work_sat / sat['wall'] / 1e9:.3f} G "
-            f"seg-substeps/s ({capacity:.2f} req/s); {n} requests Poisson at {rate:.2f} req/s: p50 "
-            f"{out[mode]['poisson']['p50_latency_s']:.3f} s, p99 {out[mode]['poisson']['p99_latency_s']:.3f} s, "
-            f"{work / run['wall'] / 1e9:.3f} G seg-substeps/s")
-        server.destroy()
+            f"seg-substeps/s ({capacity[mode]:.2f} req/s)")
+    rate = load * capacity["time_sliced"]
+    for mode, _ in modes:
+        n = num if mode == "time_sliced" else len(head)
+        run = serving.run_stream(servers[mode], reqs[:n], arrivals[:n] / rate)
+        lat = run["latency"]
+        work = sum(q.work for q in reqs[:n])
+        out[mode]["poisson"] = {
+            "requests": n, "rate_requests_per_s": rate, "load_of_own_capacity": rate / capacity[mode],
+            "wall_s": run["wall"], "value": work / run["wall"], "unit": "segment-substeps/s",
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "max_latency_s": float(lat.max()), "p50_latency_over_horizon": float(np.percentile(lat / horizon[:n], 50)),
+            "max_resident_requests": run["max_resident"], "launches": run["launches"]}
+        log(f"serving {mode}: {n} requests Poisson at {rate:.2f} req/s ({100 * rate / capacity[mode]:.0f}% of its "
+            f"capacity): p50 {out[mode]['poisson']['p50_latency_s']:.3f} s, p99 "
+            f"{out[mode]['poisson']['p99_latency_s']:.3f} s, {work / run['wall'] / 1e9:.3f} G seg-substeps/s")
+        servers[mode].destroy()
     return out
 
 
@@ -705,7 +708,7 @@ def main():
     ap.add_argument("--serve-capacity-requests", type=int, default=40,
                     help="requests of the capacity runs and of the chunked Poisson stream")
     ap.add_argument("--serve-load", type=float, default=0.8,
-                    help="Poisson rate as a fraction of the chunked server's capacity")
+                    help="Poisson rate as a fraction of the time-sliced server's capacity")
     ap.add_argument("--serve-slice-us", type=int, default=50000, help="time slice of the time-sliced server mode")
     ap.add_argument("--serve-budget-gib", type=float, default=4.0, help="server HBM budget")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
