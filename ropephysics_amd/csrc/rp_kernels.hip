@@ -570,7 +570,11 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                 } else if (ri.self && l == 0 && live) {
                     self_reset(sl);
                 }
-                __syncthreads();
+                // The fill pass reads other lanes' positions, which predict
+                // then moves. Without self contacts in the workgroup there was
+                // no fill: the vote inside the regeneration was the last
+                // cross-lane step, and the lists stay empty.
+                if (wgContacts) __syncthreads();
                 if (anyColoured && regen && wgContacts) {
                     if (ri.self && ri.selfOrder && live && l == 0) self_colour(sl);
                     __syncthreads();
@@ -641,11 +645,18 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                             }
                         }
                     }
-                    __syncthreads();
+                    // Each lane moved only its own particles. Other lanes read
+                    // them next in the self-contact projection, the next sweep
+                    // or the next regeneration; after the last sweep of a
+                    // workgroup without self-collision, only the same lanes read
+                    // them (finish, then predict, which ends with a barrier).
+                    if (anySelf || it + 1 < sa.iterations) __syncthreads();
                 }
-                if (anySelf) {
+                // Self contacts exist in the workgroup only when the last
+                // regeneration's vote said so; otherwise nothing is projected.
+                if (anySelf && wgContacts) {
                     if (ri.self && !ri.selfOrder && l == 0 && live) self_project(sl, x4, xp4);
-                    if (anyColoured && wgContacts)
+                    if (anyColoured)
                         self_project_coloured(sl, x4, xp4, ri.self && ri.selfOrder && live, l, G);
                     __syncthreads();
                 }
