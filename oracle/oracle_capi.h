@@ -1,12 +1,15 @@
 /* C interface of the CPU oracle -- TEST INFRASTRUCTURE.
  *
  * Two libraries export exactly this interface:
- *   oracle/liboracle.so      the restatement in rp_oracle.hpp (FP64 and FP32);
- *   oracle/_ref/libref.so    the reference's own CPU sources, compiled from the
- *                            RopePhysics checkout by oracle/Makefile, used only
- *                            to pin the restatement (never shipped, never timed
- *                            as the product).
- * Only tests/, the smoke check and bench.py's CPU-baseline leg load them.
+ *   oracle/liboracle.so        the restatement in rp_oracle.hpp (FP64 and FP32);
+ *   oracle/_ref/libref64.so,   the reference's own CPU sources, compiled from the
+ *   oracle/_ref/libref32.so    RopePhysics checkout by oracle/Makefile (FP64, and
+ *                              FP32 with CRS_REAL_FLOAT): they pin the restatement,
+ *                              and libref64.so is the CPU baseline bench.py times.
+ *                              The built libraries travel with the tree; the
+ *                              reference's sources never do.
+ * Only tests/, the smoke check and bench.py's CPU-baseline and verdict-check legs
+ * load them, always as the checker or the baseline, never as the product.
  *
  * Every array crosses as double; a precision-32 rod stores float internally and
  * rounds on the way in, exactly as the reference's Batch does (toF).
