@@ -43,6 +43,7 @@ struct DevBuf {
 
 struct PackedTiles : Packing {
     DevBuf dTiles, dSlots;
+    DevBuf dProgress;  // one step count per workgroup (launch pairing)
 };
 
 struct GraphKey {
@@ -68,6 +69,7 @@ struct rp_batch_impl {
     int plainLds = 0;
     size_t ldsLimit = 65536;  // dynamic LDS a workgroup may take
     size_t cuLds = 65536;     // LDS of a compute unit
+    int computeUnits = 0;
     hipStream_t stream = nullptr;
 
     // device
@@ -219,7 +221,9 @@ int upload_tiles(PackedTiles& t) {
     };
     int rc = put(t.dTiles, t.tiles.data(), t.tiles.size() * sizeof(Tile));
     if (rc) return rc;
-    return put(t.dSlots, t.slots.data(), t.slots.size() * sizeof(TileSlot));
+    if ((rc = put(t.dSlots, t.slots.data(), t.slots.size() * sizeof(TileSlot)))) return rc;
+    const std::vector<int> zeros(t.tiles.size(), 0);
+    return put(t.dProgress, zeros.data(), zeros.size() * sizeof(int));
 }
 
 // Estimated device time one TILED launch is held to.
@@ -245,7 +249,9 @@ int launch_steps(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int 
     int left = steps;
     while (left > 0) {
         const int n = std::min(left, b.stepsPerLaunch);
-        const StepArgsT<Real> a = step_args<Real>(p, n);
+        StepArgsT<Real> a = step_args<Real>(p, n);
+        a.progress = static_cast<int*>(t->dProgress.p);
+        a.pairUnits = pair_units(int(t->tiles.size()), b.computeUnits);
         const int selfMode = b.anySelf ? (b.anyColoured ? 2 : 1) : 0;
         RP_HIP(Prec<Real>::launch_tiled(cfg.tp, cfg.tc, selfMode, b.d, a, static_cast<const Tile*>(t->dTiles.p),
                                         static_cast<const TileSlot*>(t->dSlots.p), int(t->tiles.size()),
@@ -432,6 +438,7 @@ int create(const typename Prec<Real>::BatchDesc* desc, void** out_batch) {
     if ((rc = lds_limits(lim))) return rc;
     b->ldsLimit = lim.perWorkgroup;
     b->cuLds = lim.perCU;
+    b->computeUnits = lim.computeUnits;
     // A rod runs TILED when it fits the LDS budget and some tile shape built
     // for this precision has lanes enough for it.
     bool allTileable = true;

@@ -120,6 +120,7 @@ int lds_limits(LdsLimits& out) {
     // Workgroups may use the whole LDS of a compute unit; where the runtime
     // refuses the larger dynamic allocation, stay within 64 KiB.
     out.perCU = prop.sharedMemPerBlock;
+    out.computeUnits = prop.multiProcessorCount;
     int granted32 = 0, granted64 = 0;
     if (Prec<float>::allow_dynamic_lds(int(prop.sharedMemPerBlock), &granted32) == hipSuccess &&
         Prec<double>::allow_dynamic_lds(int(prop.sharedMemPerBlock), &granted64) == hipSuccess) {
@@ -191,8 +192,13 @@ int pack_rods(const std::vector<RodInfoT<Real>>& rods, const std::vector<int>& i
         Tile t;
         t.firstSlot = int(out.slots.size());
         t.numSlots = int(o.rods.size());
-        int lane = 0, off = 0;
+        // A lane group starts on a wavefront boundary when the lanes left in
+        // the workgroup allow it: a colour pass then spans fewer wavefronts.
+        int lane = 0, off = 0, rest = o.lanesUsed;
         for (int r : o.rods) {
+            const int aligned = (lane + 63) & ~63;
+            if (aligned + rest <= laneCap) lane = aligned;
+            rest -= lanes[r];
             out.slots.push_back(TileSlot{r, lane, lanes[r], off});
             lane += lanes[r];
             off += lds[r];
@@ -200,7 +206,7 @@ int pack_rods(const std::vector<RodInfoT<Real>>& rods, const std::vector<int>& i
         t.ldsBytes = off;
         t.pad = 0;
         out.maxLds = std::max(out.maxLds, off);
-        out.lanesBusy += lane;
+        out.lanesBusy += o.lanesUsed;
         out.tiles.push_back(t);
     }
     return RP_OK;
@@ -223,6 +229,8 @@ StepArgsT<Real> step_args(const typename Prec<Real>::StepParams& p, int steps) {
     a.contactInterval = p.contact_interval > 0 ? p.contact_interval : 1;
     a.steps = steps;
     a.sliceTicks = 0;
+    a.progress = nullptr;
+    a.pairUnits = 0;
     return a;
 }
 

@@ -142,10 +142,18 @@ int build_records(const typename Prec<Real>::BatchDesc* desc, RecordsT<Real>& ou
 struct LdsLimits {
     size_t perWorkgroup = 65536;
     size_t perCU = 65536;
+    int computeUnits = 0;
 };
 int lds_limits(LdsLimits& out);
 // Dynamic LDS budget of a workgroup when `perCU` of them share a CU.
 int lds_budget(const LdsLimits& l, int perCU);
+// Workgroups of a launch of `numTiles` pair up on `units` compute units when
+// there are more than one and at most two per unit: dispatch gives every unit
+// one workgroup, then a second, so b and b + units share a unit. Returns that
+// distance, or 0.
+inline int pair_units(int numTiles, int units) {
+    return units > 0 && numTiles > units && numTiles <= 2 * units ? units : 0;
+}
 
 template <typename Real>
 int group_lanes(const RodInfoT<Real>& r, const std::vector<int>& colStart, int tp, int tc);

@@ -375,6 +375,30 @@ __host__ __device__ inline TiledLayout tiled_layout(int nP, int nF, int nS, int 
 constexpr int tiled_max_lanes(int tp) { return (!RP_FP64 && tp == 1) ? 512 : 256; }
 constexpr int tiled_waves_per_simd(int tp) { return (!RP_FP64 && tp <= 2) ? 2 : 1; }
 
+// Issue priority of a workgroup that shares its compute unit with another of
+// the same launch (b and b +- pairUnits). The SIMDs issue the older
+// workgroup's waves first, so without this the younger one falls behind and
+// finishes last, alone on the unit. Each workgroup publishes its step count at
+// the top of every step; the one behind its partner raises its waves'
+// priority, and at equal counts the two take turns on the wall clock (about
+// every 160 us). Only the issue order changes, never what a lane computes. The
+// partner's count may still be its previous launch's for the first step: that
+// costs one step of wrong order, nothing more.
+__device__ __forceinline__ void prioritise_lagging(const StepArgs& sa, int step) {
+    const int b = blockIdx.x;
+    const int partner = b < sa.pairUnits ? b + sa.pairUnits : b - sa.pairUnits;
+    if (threadIdx.x == 0) __hip_atomic_store(sa.progress + b, step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool behind = false;
+    if (partner < int(gridDim.x)) {
+        const int other = __hip_atomic_load(sa.progress + partner, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        behind = other > step || (other == step && (((wall_clock64() >> 14) ^ unsigned(b / sa.pairUnits)) & 1u));
+    }
+    if (__builtin_amdgcn_readfirstlane(int(behind)))
+        __builtin_amdgcn_s_setprio(1);
+    else
+        __builtin_amdgcn_s_setprio(0);
+}
+
 template <int TP, int TC, int SELF>
 __global__ __launch_bounds__(tiled_max_lanes(TP)) __attribute__((amdgpu_waves_per_eu(tiled_waves_per_simd(TP)))) void
 k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slots) {
@@ -543,6 +567,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
         // formed where they are used, not held in registers across the loop.
         asm volatile("" : "+v"(rod));
         const bool live = active && clk < limit;
+        if (sa.pairUnits > 0) prioritise_lagging(sa, step);
         if (driver && live) kv4[drivenP] = vf4(path_velocity(d.paths[rod], clk), real(0.0));
         for (int sub = 0; sub < sa.substeps; ++sub) {
             const bool regen = sub % sa.contactInterval == 0;

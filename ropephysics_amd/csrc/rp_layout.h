@@ -164,6 +164,11 @@ struct StepArgsT {
     // wall-clock ticks have passed since it began (time-sliced launches; each
     // workgroup still takes at least one step).
     long long sliceTicks;
+    // TILED launches whose workgroups pair up on the compute units: workgroups
+    // b and b + pairUnits share one (0: no pairing). progress holds one step
+    // count per workgroup of the launch.
+    int* progress;
+    int pairUnits;
 };
 
 // The two arithmetic precisions. Device code is compiled once per precision

@@ -81,7 +81,8 @@ constexpr long long kBytesPer[kNumPools] = {
     16 + 12 + 12 + 32,   // frame: q, omega, torque, invInertia4, inertia4
     16 + 16 + 8,         // stretch: sIdx, sPar, segEnd
     16 + 16 + 16,        // bend: bIdx, bPar, bRest4
-    int(sizeof(RodInfo)) + 6 * 4 + int(sizeof(Tile)) + int(sizeof(TileSlot)),  // rod: record, per-rod words, packing
+    // rod: record, per-rod words; tile, slot and step count for each of the two launches
+    int(sizeof(RodInfo)) + 6 * 4 + 2 * (int(sizeof(Tile)) + int(sizeof(TileSlot)) + 4),
     int(sizeof(Prim)),   // primitive
     4,                   // colour-start entry
 };
@@ -128,6 +129,7 @@ struct rp_server_impl {
     int* stepLimit = nullptr;
     Tile* dTiles = nullptr;
     TileSlot* dSlots = nullptr;
+    int* dProgress = nullptr;  // per workgroup of each launch (launch pairing)
     std::vector<RodInfo> rods;  // host mirror, indexed by rod slot
     std::vector<int> colStart;  // host mirror of the colour-start table
     std::vector<unsigned char> selfOf;
@@ -280,7 +282,7 @@ int rp_server_create(const rp_server_desc* desc, void** out_server) {
         (rc = s->alloc(bRest, B)) || (rc = s->alloc(dRods, R)) || (rc = s->alloc(cs, R)) ||
         (rc = s->alloc(fs, R)) || (rc = s->alloc(ovf, R)) || (rc = s->alloc(clock, R)) ||
         (rc = s->alloc(s->stepLimit, R)) || (rc = s->alloc(s->dTiles, 2 * R)) || (rc = s->alloc(s->dSlots, 2 * R)) ||
-        (rc = s->alloc(dPrims, cap[kPrims])) || (rc = s->alloc(col, cap[kColours])))
+        (rc = s->alloc(s->dProgress, 2 * R)) || (rc = s->alloc(dPrims, cap[kPrims])) || (rc = s->alloc(col, cap[kColours])))
         return rc;
     d.x = x;
     d.v = v;
@@ -476,12 +478,17 @@ int rp_server_step(void* server, int* num_finished) {
     }
     StepArgs a = step_args<float>(s.params, launchSteps);
     a.sliceTicks = s.sliceTicks;
+    // Pairing is by workgroup index within one launch: with both launches
+    // sharing the device it would pair workgroups of different launches.
+    const bool oneLaunch = lean.empty() != self.empty();
     if (!self.empty()) {
         // Uploads and tiles are on the main stream: the second stream starts
         // after them, and the main stream continues after its launch.
         const TileConfig& cfg = tile_config(s.configSelf);
         RP_HIP(hipEventRecord(s.evReady, s.stream));
         RP_HIP(hipStreamWaitEvent(s.selfStream, s.evReady, 0));
+        a.progress = s.dProgress + R;
+        a.pairUnits = oneLaunch ? pair_units(int(s.packSelf.tiles.size()), s.lim.computeUnits) : 0;
         RP_HIP(launch_tiled(cfg.tp, cfg.tc, coloured ? 2 : 1, s.d, a, s.dTiles + R, s.dSlots + R,
                             int(s.packSelf.tiles.size()), 64 * cfg.waves, size_t(s.packSelf.maxLds), s.selfStream));
         RP_HIP(hipEventRecord(s.evSelfDone, s.selfStream));
@@ -489,6 +496,8 @@ int rp_server_step(void* server, int* num_finished) {
     }
     if (!lean.empty()) {
         const TileConfig& cfg = tile_config(s.configLean);
+        a.progress = s.dProgress;
+        a.pairUnits = oneLaunch ? pair_units(int(s.packLean.tiles.size()), s.lim.computeUnits) : 0;
         RP_HIP(launch_tiled(cfg.tp, cfg.tc, 0, s.d, a, s.dTiles, s.dSlots, int(s.packLean.tiles.size()),
                             64 * cfg.waves, size_t(s.packLean.maxLds), s.stream));
         ++s.launches;
