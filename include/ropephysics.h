@@ -450,7 +450,8 @@ typedef struct rp_server_desc {
     int max_rods;                /* rod records (0: one per 16 segments of the budget) */
     rp_step_params params;       /* the step every request takes */
     int tile_config;             /* 0: tp3-tc1-w4 for rods without self-collision, tp1-tc1-w8 for
-                                    rods with it; else this shape for both */
+                                    rods with it (tp2-tc1-w4 when time-sliced); else this shape
+                                    for both */
     int steps_per_launch;        /* steps per rp_server_step (0: 64) */
     int self_tile_config;        /* when positive: the shape for rods with self-collision */
     int time_slice_us;           /* when positive: time-sliced steps (below) */

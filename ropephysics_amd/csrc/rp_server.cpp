@@ -110,8 +110,14 @@ struct rp_server_impl {
     // Rods without self-collision run the lean kernel, the others the
     // self-collision variant, each packed in its own tile shape and launched
     // on its own stream so the two launches share the device.
+    // Default shapes, from a sweep of the serving stream's first 40 requests
+    // arriving at once (4 GiB budget): lean tp3-tc1-w4 (tp2-tc1-w4 and
+    // tp3-tc1-w2 within 1-3%); self-colliding tp1-tc1-w8 in chunked steps
+    // (2.82 G segment-substeps/s against 2.77 G for tp2-tc1-w4) and
+    // tp2-tc1-w4 in time-sliced ones (2.56 G against 2.39 G), tp3-tc1-w4 well
+    // behind in both (2.11 and 1.82 G).
     int configLean = 10;  // tp3-tc1-w4
-    int configSelf = 13;  // tp1-tc1-w8
+    int configSelf = 13;  // tp1-tc1-w8; tp2-tc1-w4 when time-sliced
     int stepsPerLaunch = 64;
     // Time-sliced steps: the slice in device wall-clock ticks (0: chunks of
     // stepsPerLaunch), and whether a launch's progress is still to be read.
@@ -245,6 +251,7 @@ int rp_server_create(const rp_server_desc* desc, void** out_server) {
         RP_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
         if (khz <= 0) return fail(RP_ERR_DEVICE, "the device reports no wall-clock rate");
         s->sliceTicks = std::max(1LL, (long long)desc->time_slice_us * khz / 1000);
+        if (desc->tile_config == 0 && desc->self_tile_config == 0) s->configSelf = 9;  // tp2-tc1-w4
     }
     int rc = lds_limits(s->lim);
     if (rc) return rc;

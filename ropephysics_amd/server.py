@@ -21,9 +21,11 @@ from .batch import _f32, describe, step_params
 from .rod import SolverParams
 
 # Tile shapes the server uses by default: three particles per lane in 256-lane
-# workgroups for rods without self-collision, one per lane in 512-lane
-# workgroups for the others.
+# workgroups for rods without self-collision; for the others one per lane in
+# 512-lane workgroups, or two per lane in 256-lane workgroups when the server
+# is time-sliced.
 DEFAULT_TILE_CONFIG = 13
+DEFAULT_SLICED_TILE_CONFIG = 9
 DEFAULT_LEAN_TILE_CONFIG = 10
 
 
