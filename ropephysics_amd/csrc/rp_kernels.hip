@@ -667,7 +667,12 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                             }
                         }
                     }
-                    __syncthreads();
+                    // After the last bend colour the next readers of the
+                    // orientations are finish, a self projection or the next
+                    // sweep. When the contact sweep, which reads and writes
+                    // only positions and contact slots, ends with a barrier,
+                    // that barrier orders them.
+                    if (c == 0 || !(anyPrim && (anySelf || it + 1 < sa.iterations))) __syncthreads();
                 }
                 if (anyPrim) {
                     if (nPrim > 0) {

@@ -421,6 +421,38 @@ def test_calls_longer_than_one_launch():
     assert np.all(np.isfinite(a["x"]))
 
 
+def test_paired_workgroups_give_the_same_bits():
+    """A TILED launch with between one and two workgroups per compute unit,
+    where workgroups b and b + units trade issue priority by step count at
+    each of the first 64 steps and every 64 after: routing cables of 130-160
+    segments (self-collision, world contacts, gripper paths), one per 256-lane
+    workgroup whatever the LDS budget, 1.5 per compute unit, 300 steps in one
+    launch, equal PLAIN bit for bit."""
+    from ropephysics_amd import _abi, scenes
+
+    units = _abi.device_info()["compute_units"]
+    w = scenes.routing_rollouts(num_rods=8 * units, steps=300)
+    pick = [i for i, r in enumerate(w.rods) if r.num_segments >= 130][: 3 * units // 2]
+    assert len(pick) == 3 * units // 2
+    rods = [w.rods[i] for i in pick]
+    outs = []
+    for strategy in ("tiled", "plain"):
+        b = rp.Batch(rods, strategy=strategy, world=w.world, tile_config=12 if strategy == "tiled" else 0)
+        b.set_material_scales(w.youngs_scale[pick], w.friction_scale[pick])
+        b.set_paths(0, [w.info["paths"][i] for i in pick])
+        b.step(w.params, w.steps)
+        b.synchronize()
+        if strategy == "tiled":
+            c = b.counters()
+            assert c["tile_config_name"] == "tp1-tc1-w4" and c["last_step_launches"] == 1
+            assert c["workgroups"] == len(rods) and units < c["workgroups"] <= 2 * units
+        outs.append(b.get_state())
+        b.destroy()
+    for k in outs[0]:
+        assert np.array_equal(bits(outs[0][k]), bits(outs[1][k])), k
+    assert np.all(np.isfinite(outs[0]["x"]))
+
+
 def test_full_size_uniform_batch_sampled_rods_bitwise():
     """The 4096 x 64 floor workload at full size for 30 steps: a sample of rods
     against the FP32 oracle, bit for bit; every rod finite and above the floor."""
