@@ -413,9 +413,11 @@ def global_leg(num_rods: int, warmup: int, segments: int = 2000, steps: int = 20
                       "steps": steps, "contacts": "floor plane, Coulomb friction"},
            "strategy": c["strategy"], "value": work / secs, "unit": "segment-substeps/s", "seconds": secs,
            "kernel_ms_avg": c["last_step_gpu_ms"] / launches, "launches": launches,
-           "scratch_bytes": c["scratch_bytes"], "lanes_per_workgroup": c["lanes_total"] // max(1, c["workgroups"])}
+           "scratch_bytes": c["scratch_bytes"], "lanes_per_workgroup": c["lanes_total"] // max(1, c["workgroups"]),
+           "lds_bytes_per_workgroup": c["lds_bytes_per_workgroup"]}
     log(f"long rods ({c['strategy']}): {num_rods} x {segments} segments, {steps} steps in {launches} launches, "
-        f"{out['value'] / 1e9:.3f} G seg-substeps/s, {c['scratch_bytes'] / 1e6:.0f} MB of HBM working arrays")
+        f"{out['value'] / 1e9:.3f} G seg-substeps/s, {c['scratch_bytes'] / 1e6:.0f} MB of HBM working arrays, "
+        f"{c['lds_bytes_per_workgroup']} B of LDS per workgroup (positions and orientations)")
     b.destroy()
     return out
 

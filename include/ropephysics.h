@@ -53,9 +53,10 @@ enum rp_status {
  *           counterpart of the reference's multi-kernel strategy
  *           (src/gpu/gpu_solver.cu:417-529,691-730; GpuSim falls back to it,
  *           src/gpu/gpu_suite.cpp:125-130): one rod per workgroup of up to 512
- *           lanes with its working arrays in HBM (allocated at the first step),
- *           many steps per launch. Any rod length; same arithmetic and results
- *           as PLAIN, bit for bit.
+ *           lanes with its working arrays in HBM (allocated at the first step;
+ *           positions and orientations held in LDS when every rod's fit two
+ *           workgroups per compute unit), many steps per launch. Any rod length;
+ *           same arithmetic and results as PLAIN, bit for bit.
  *   AUTO    TILED when every rod fits it, else PLAIN when every rod fits a
  *           workgroup's LDS, else GLOBAL. */
 enum rp_strategy { RP_STRATEGY_PLAIN = 0, RP_STRATEGY_TILED = 1, RP_STRATEGY_AUTO = 2, RP_STRATEGY_GLOBAL = 3 };

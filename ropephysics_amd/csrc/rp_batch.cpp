@@ -102,6 +102,7 @@ struct rp_batch_impl {
     // GLOBAL: lanes per workgroup, per-rod offsets of the HBM working arrays
     // and their total; the arrays are allocated at the first step.
     int globalThreads = 128;
+    int globalXqLds = 0;  // dynamic LDS holding positions and orientations (0: they stay in HBM)
     std::vector<long long> scratchOffHost;
     size_t scratchBytes = 0;
     void* scratch = nullptr;
@@ -240,7 +241,8 @@ int launch_steps(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int 
     if (strategy == RP_STRATEGY_GLOBAL) {
         for (int left = steps; left > 0;) {
             const int n = std::min(left, b.stepsPerLaunch);
-            RP_HIP(Prec<Real>::launch_global(b.d, step_args<Real>(p, n), b.globalThreads, b.stream));
+            RP_HIP(Prec<Real>::launch_global(b.d, step_args<Real>(p, n), b.globalThreads, size_t(b.globalXqLds),
+                                             b.stream));
             left -= n;
         }
         return RP_OK;
@@ -485,6 +487,12 @@ int create(const typename Prec<Real>::BatchDesc* desc, void** out_batch) {
         }
         b->scratchBytes = off;
         b->globalThreads = std::min(Prec<Real>::global_max_threads(), std::max(64, (widest + 63) / 64 * 64));
+        // Positions and orientations in LDS when every rod's fit the budget of
+        // two workgroups per compute unit (a 2000-segment FP32 rod: 64 KB).
+        size_t xq = 0;
+        for (int r = 0; r < R; ++r)
+            xq = std::max(xq, size_t(b->rods[r].nP + b->rods[r].nF) * 4 * sizeof(Real));
+        b->globalXqLds = xq <= size_t(lds_budget(*b, 2)) ? int(xq) : 0;
     }
     if (const char* e = std::getenv("RP_STEPS_PER_LAUNCH")) {
         b->stepsPerLaunch = std::max(1, std::atoi(e));
@@ -916,7 +924,8 @@ int rp_batch_step_noalloc(void* batch, const rp_step_params* params, int num_ste
     int launches = 0;
     for (int left = num_steps; left > 0; ++launches) {
         const int n = std::min(left, chunk);
-        RP_HIP(Prec<float>::launch_global(d, step_args<float>(p, n), b.globalThreads, b.stream));
+        RP_HIP(Prec<float>::launch_global(d, step_args<float>(p, n), b.globalThreads, size_t(b.globalXqLds),
+                                          b.stream));
         left -= n;
     }
     if (b.timing) {
@@ -1193,6 +1202,7 @@ int rp_batch_counters(void* batch, rp_counters* out) {
         out->workgroups = b.numRods;
         out->lanes_busy = b.numRods * b.globalThreads;
         out->lanes_total = b.numRods * b.globalThreads;
+        out->lds_bytes_per_workgroup = size_t(b.globalXqLds);
         out->launches_per_step = 0;
     } else if (b.tiles) {
         const TileConfig& c = tile_config(b.tiles->config);

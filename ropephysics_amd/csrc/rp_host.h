@@ -20,7 +20,8 @@ namespace rp {
     int plain_lds_bytes(int nP, int nF, int nS, int nB, int nPrim, int self, int table, int cap);              \
     int tiled_lds_bytes(int nP, int nF, int nS, int nPrim, int self, int table, int cap);                      \
     hipError_t launch_plain(const DeviceArraysT<R>& d, const StepArgsT<R>& sa, size_t lds, hipStream_t s);      \
-    hipError_t launch_global(const DeviceArraysT<R>& d, const StepArgsT<R>& sa, int threads, hipStream_t s);    \
+    hipError_t launch_global(const DeviceArraysT<R>& d, const StepArgsT<R>& sa, int threads, size_t xqLds,     \
+                             hipStream_t s);                                                                   \
     int global_max_threads();                                                                                  \
     hipError_t allow_dynamic_lds(int bytes, int* granted);                                                     \
     hipError_t launch_tiled(int tp, int tc, int self, const DeviceArraysT<R>& d, const StepArgsT<R>& sa,      \

@@ -100,23 +100,27 @@ __host__ __device__ inline PlainLayout plain_layout(int nP, int nF, int nS, int 
 
 // The whole step of rod blockIdx.x by one workgroup, its working arrays
 // carved from `smem` as PlainLayout says: LDS for the PLAIN kernel, the rod's
-// own HBM region for the GLOBAL one. Every lane strides the rod's elements by
-// the workgroup width, so any width works; the phases are separated by
-// workgroup barriers, which order global memory within the workgroup as they
-// order LDS.
-__device__ __forceinline__ void plain_rod_steps(const DeviceArrays& d, const StepArgs& sa, char* smem) {
+// own HBM region for the GLOBAL one. With kXqLds the positions and
+// orientations, which every phase reads and the colour sweeps write, live in
+// LDS at `xq` instead (positions, then orientations). Every lane strides the
+// rod's elements by the workgroup width, so any width works; the phases are
+// separated by workgroup barriers, which order global memory within the
+// workgroup as they order LDS.
+template <bool kXqLds = false>
+__device__ __forceinline__ void plain_rod_steps(const DeviceArrays& d, const StepArgs& sa, char* smem,
+                                                char* xq = nullptr) {
     const int r = blockIdx.x;
     if (d.stepLimit && d.clock[r] >= d.stepLimit[r]) return;
     const RodInfo ri = d.rods[r];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nP = ri.nP, nF = ri.nF, nS = ri.nS, nB = ri.nB, nPrim = ri.nPrim;
     const PlainLayout L = plain_layout(nP, nF, nS, nB, nPrim, ri.self, ri.table, ri.selfCap);
-    real4* x4 = reinterpret_cast<real4*>(smem + L.x);
+    real4* x4 = reinterpret_cast<real4*>(kXqLds ? xq : smem + L.x);
     real4* xp4 = reinterpret_cast<real4*>(smem + L.xp);
     real4* v4 = reinterpret_cast<real4*>(smem + L.v);
     real4* f4 = reinterpret_cast<real4*>(smem + L.f);
     real4* kv4 = reinterpret_cast<real4*>(smem + L.kv);
-    real4* q4s = reinterpret_cast<real4*>(smem + L.q);
+    real4* q4s = reinterpret_cast<real4*>(kXqLds ? xq + nP * kR4 : smem + L.q);
     real4* qp4 = reinterpret_cast<real4*>(smem + L.qp);
     real4* om4 = reinterpret_cast<real4*>(smem + L.om);
     real4* tq4 = reinterpret_cast<real4*>(smem + L.tq);
@@ -320,10 +324,14 @@ constexpr int kGlobalMaxThreads = 512;
 // the same reason (src/gpu/gpu_solver.cu:417-529,691-730). Here one
 // workgroup still steps one rod through every phase, with barriers where the
 // reference launches a kernel per phase and colour, and a launch runs many
-// steps. The region stays in the L2 of the rod's XCD between phases.
+// steps. With XQ the rod's positions and orientations are held in LDS for
+// the launch (the launch's dynamic LDS covers every rod's), and only the
+// other arrays are read from and written to HBM in every phase.
+template <bool XQ>
 __global__ __launch_bounds__(kGlobalMaxThreads) void k_global_step(DeviceArrays d, StepArgs sa) {
+    extern __shared__ real4 xq4[];
     if (int(blockIdx.x) >= d.numRods) return;
-    plain_rod_steps(d, sa, d.scratch + d.scratchOff[blockIdx.x]);
+    plain_rod_steps<XQ>(d, sa, d.scratch + d.scratchOff[blockIdx.x], reinterpret_cast<char*>(xq4));
 }
 
 // ------------------------------------------------------------------ TILED
@@ -783,10 +791,13 @@ hipError_t launch_plain(const DeviceArrays& d, const StepArgs& sa, size_t lds, h
 
 int global_max_threads() { return kGlobalMaxThreads; }
 
-hipError_t launch_global(const DeviceArrays& d, const StepArgs& sa, int threads, hipStream_t s) {
+hipError_t launch_global(const DeviceArrays& d, const StepArgs& sa, int threads, size_t xqLds, hipStream_t s) {
     if (threads < 64 || threads % 64 != 0 || threads > kGlobalMaxThreads || !d.scratch || !d.scratchOff)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_global_step, dim3(d.numRods), dim3(threads), 0, s, d, sa);
+    if (xqLds > 0)
+        hipLaunchKernelGGL(k_global_step<true>, dim3(d.numRods), dim3(threads), xqLds, s, d, sa);
+    else
+        hipLaunchKernelGGL(k_global_step<false>, dim3(d.numRods), dim3(threads), 0, s, d, sa);
     return hipGetLastError();
 }
 
@@ -830,6 +841,7 @@ static hipError_t allow_tiled_lds(int bytes, int* granted) {
 hipError_t allow_dynamic_lds(int bytes, int* granted) {
     *granted = bytes;
     hipError_t e = allow_lds(reinterpret_cast<const void*>(&k_plain_step), bytes, granted);
+    if (e == hipSuccess) e = allow_lds(reinterpret_cast<const void*>(&k_global_step<true>), bytes, granted);
     if (e == hipSuccess) e = allow_tiled_lds<1, 1>(bytes, granted);
     if (e == hipSuccess) e = allow_tiled_lds<2, 1>(bytes, granted);
     if (e == hipSuccess) e = allow_tiled_lds<3, 1>(bytes, granted);
