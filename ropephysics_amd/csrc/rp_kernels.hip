@@ -556,7 +556,11 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
         sl.misc[1] = 0;
         self_regen_reset(sl);
         self_list_reset(sl, nS);
+        self_candidates_reset(sl);
     }
+    // How far this lane's particles moved since the last regeneration (a
+    // bound: per substep the largest displacement, summed).
+    real moved = real(0.0);
     const v3 g = mk(sa.gx, sa.gy, sa.gz);
     const real h = sa.h;
     int clk = active ? d.clock[rod] : 0;
@@ -610,7 +614,8 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                 if (regen) {
                     wgContacts = self_regen_group<TP>(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap,
                                                       ri.radius, ri.selfFriction * fscale, ri.selfChain != 0,
-                                                      ri.self != 0 && live, l, G);
+                                                      ri.self != 0 && live, l, G, moved);
+                    moved = real(0.0);
                 } else if (ri.self && l == 0 && live) {
                     self_reset(sl);
                 }
@@ -712,14 +717,20 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
             }
 
             // ---- finish: velocities from positions; nothing shared is written
+            real moved2 = real(0.0);
 #pragma unroll
             for (int t = 0; t < TP; ++t) {
                 const int i = l + G * t;
-                if (live && i < nP)
-                    vel[t] = finish_particle(f4v(x4[i]), xpv[t], wP[t], f4v(kv4[i]), h, sa.linDecay);
+                if (live && i < nP) {
+                    const v3 x = f4v(x4[i]);
+                    vel[t] = finish_particle(x, xpv[t], wP[t], f4v(kv4[i]), h, sa.linDecay);
+                    if (SELF) moved2 = r_max(moved2, norm2(mk(x.x - xpv[t].x, x.y - xpv[t].y, x.z - xpv[t].z)));
+                }
                 const int j = l + G * t;
                 if (live && j < nF) om[t] = finish_frame(f4q(q4s[j]), qpv[t], iIF[t], h, sa.angDecay);
             }
+            // Rounded up past the rounding of the distance.
+            if (SELF) moved += r_sqrt(moved2) * real(1.000001);
         }
         if (!live) continue;
         ++clk;

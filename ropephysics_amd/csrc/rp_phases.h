@@ -219,8 +219,10 @@ struct SelfLds {
     int* misc;       // [0] live, [1] overflow, [2] long segments, [3] colours in use (coloured order),
                      // [4..7] the first four long segments (nS: none)
     rbits* rmisc;    // [0] cell size, [1] longest segment, [2] largest |coordinate| of a centre, [3] shortest
-                     // segment, [4] longest segment that is not long, as bits (non-negative: ordered as
-                     // integers)
+                     // segment, [4] longest segment that is not long, [5] largest particle displacement
+                     // since the last regeneration, [6] displacement bound since the candidate lists were
+                     // built, as bits (non-negative: ordered as integers); [7] 1 when the candidate lists
+                     // are valid; [8] the length above which a segment is long at the next regeneration
     int* scan;       // kScanWaves, scratch of the group-parallel scans
 };
 
@@ -229,7 +231,7 @@ constexpr int kScanWaves = 8;
 
 // misc (8 ints) then rmisc (5 reals), padded so the int arrays after them stay
 // 16-byte aligned.
-constexpr int kMiscBytes = 80;
+constexpr int kMiscBytes = 112;
 constexpr int kLongSlots = 4;
 
 __host__ __device__ inline int self_lds_bytes(int nS, int table, int cap) {
@@ -518,35 +520,89 @@ __device__ inline void self_fill_listed(const SelfLds& s, const real4* x4, const
         }
 }
 
+// One later segment k whose centre passed the cull: segment j's contacts with
+// it, each counted once for every one of j's 27 cells whose bucket is k's (the
+// times the bucketed query reaches k). Before the closest-point test, the pair
+// is dropped when segment j is farther from k's centre than k's half-length
+// plus 2r (with the cull's margins): every point of segment k lies within its
+// half-length of its centre. The first M contacts reached (nonzero count, in
+// the order the caller visits k) are kept in kk[]; nk counts them all.
+template <int M>
+__device__ __forceinline__ void self_count_one(const SelfLds& s, const real4* x4, const int* segEnd, int table,
+                                               real radius, real cell, real eps, v3 cj, int j0, int j1, int k,
+                                               real4 ck4, int (&kk)[M > 0 ? M : 1], int& nk, int& count) {
+    const v3 ck = f4v(ck4);
+    const v3 a0 = f4v(x4[j0]), a1 = f4v(x4[j1]);
+    const v3 dj = sub(a1, a0);
+    const real a = norm2(dj);
+    // Distance from segment j to k's centre (the nearest point of j up to
+    // rounding, far inside the margin).
+    const real tq = a > real(1e-30) ? r_max(real(0.0), min_ab(real(1.0), dot(sub(ck, a0), dj) / a)) : real(0.0);
+    const v3 off = sub(add(a0, scale(dj, tq)), ck);
+    const real limP = real(1.001) * (real(0.5) * ck4.w + real(2.0) * radius) + eps;
+    if (norm2(off) > limP * limP) return;
+    int c = 0;
+    self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * k], segEnd[2 * k + 1], radius, real(0.0), false, 0, 0, c);
+    if (c == 0) return;
+    const unsigned tbl = static_cast<unsigned>(table);
+    const bool pow2 = (tbl & (tbl - 1u)) == 0u;
+    const int cx = cell_coord(cj.x, cell), cy = cell_coord(cj.y, cell), cz = cell_coord(cj.z, cell);
+    const int hk = int(cell_hash(cell_coord(ck.x, cell), cell_coord(ck.y, cell), cell_coord(ck.z, cell), table));
+    int mult = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy) {
+            const unsigned hzy =
+                (static_cast<unsigned>(cz + dz) * 83492791u) ^ (static_cast<unsigned>(cy + dy) * 19349663u);
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) mult += row_bucket(cx, dx, hzy, tbl, pow2) == hk ? 1 : 0;
+        }
+    if (M > 0 && mult > 0) {
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (nk == m) kk[m] = k;
+        ++nk;
+    }
+    count += mult;
+}
+
+// Candidate lists: the later segments k >= j + gap whose centres lie within
+// the cull radius plus a skin of segment j's, found by a scan and kept while
+// no particle can have moved far enough for another segment to enter the cull
+// radius. One 64-bit word per segment: bits 0-2 the number of candidates (0
+// to kCandidates; kCandidateOverflow when there were more), then k - j in 10
+// bits each, ascending.
+constexpr int kCandidates = 6;
+constexpr unsigned long long kCandidateOverflow = 7ull;
+
 // Number of contacts segment j has in the list: its later segments k >= j +
-// gap scanned in index order, each contact counted once for every one of j's
-// 27 cells whose bucket is k's (the times the bucketed query reaches k). The
-// fill pass's cull, with k's own half-length and the rod's largest coordinate
-// for the scale, keeps every contact. When consecutive segments share a
-// particle (`chain`), consecutive centres are at most the longer of the two
-// segments apart, so a segment k at distance D also rules out the following
-// segments k' while D stays above the cull radius at their bound:
-// floor((D - lim) / step) of them, lim being the cull radius at the bound
-// lenCap and step lenCap with a 1e-3 margin. No segment of such a stretch may
-// be longer than lenCap: the long segments are listed in misc[4..7] (when
-// `longs`: the rod has some and all are listed), the first one at or after k
-// ends the stretch, and from a long segment the scan steps by one. Far parts of the rod cost one test per stretch of segments.
-// Before the closest-point test, a pair is also dropped when segment j is
-// farther from k's centre than k's half-length plus 2r (same margins): every
-// point of segment k lies within its half-length of its centre.
+// gap scanned in index order, each passing the centre cull counted by
+// self_count_one. The cull drops a pair whose centres are farther apart than
+// the two half-lengths plus 2r (with a relative margin of 1e-3 and an absolute
+// one of 1e-6 times the coordinate scale, far above the rounding of centres and
+// lengths); it keeps every contact. When consecutive segments share a particle
+// (`chain`), consecutive centres are at most the longer of the two segments
+// apart, so a segment k at distance D also rules out the following segments k'
+// while D stays above the cull radius at their bound: floor((D - lim) / step)
+// of them, lim being the cull radius at the bound lenCap and step lenCap with
+// a 1e-3 margin. No segment of such a stretch may be longer than lenCap: the
+// long segments are listed in misc[4..7] (when `longs`: the rod has some and
+// all are listed), the first one at or after k ends the stretch, and from a
+// long segment the scan steps by one. Far parts of the rod cost one test per
+// stretch of segments.
 //
-// The first M contacts reached (those with a nonzero count, in ascending k)
-// are kept in kk[]; nk is the number of them, which may exceed M.
+// With `cand`, the scan also collects segment j's candidates, those within the
+// cull radius plus `skin` (the skips then use that radius), into *cand.
 template <int M>
 __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const int* segEnd, bool longs, int nS,
                                       int table, int gap, real radius, real cell, real lenCap, real scaleMax,
                                       bool chain, int j, int j0, int j1, real4 cj4, int (&kk)[M > 0 ? M : 1],
-                                      int& nk) {
+                                      int& nk, real skin, unsigned long long* cand) {
     const v3 cj = f4v(cj4);
     const real reachJ = real(0.5) * cj4.w + real(2.0) * radius;
     const real eps = real(1e-6) * (real(1.0) + scaleMax);
-    const float limCap = float(real(1.001) * (reachJ + real(0.5) * lenCap) + eps);
+    const float limCap = float(real(1.001) * (reachJ + real(0.5) * lenCap) + eps + skin);
     const float invCap = chain && lenCap > real(0.0) ? 0.999f / (1.001f * float(lenCap)) : 0.0f;
+    unsigned long long word = 0;
     int count = 0;
     int k = j + gap;
     while (k < nS) {
@@ -554,7 +610,8 @@ __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const i
         const v3 ck = f4v(ck4);
         const real D2 = norm2(sub(ck, cj));
         const real limK = real(1.001) * (reachJ + real(0.5) * ck4.w) + eps;
-        if (D2 > limK * limK) {
+        const real limS = limK + skin;
+        if (D2 > limS * limS) {
             int lf = nS;
             if (longs) {
                 const int4 L = *reinterpret_cast<const int4*>(s.misc + 4);
@@ -568,41 +625,42 @@ __device__ inline int self_count_scan(const SelfLds& s, const real4* x4, const i
             k = lf > k ? min(next, lf) : k + 1;
             continue;
         }
-        const v3 a0 = f4v(x4[j0]), a1 = f4v(x4[j1]);
-        const v3 dj = sub(a1, a0);
-        const real a = norm2(dj);
-        // Distance from segment j to k's centre (the nearest point of j up to
-        // rounding, far inside the margin).
-        const real tq = a > real(1e-30) ? r_max(real(0.0), min_ab(real(1.0), dot(sub(ck, a0), dj) / a)) : real(0.0);
-        const v3 off = sub(add(a0, scale(dj, tq)), ck);
-        const real limP = real(1.001) * (real(0.5) * ck4.w + real(2.0) * radius) + eps;
-        if (!(norm2(off) > limP * limP)) {
-            int c = 0;
-            self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * k], segEnd[2 * k + 1], radius, real(0.0), false, 0, 0, c);
-            if (c > 0) {
-                const unsigned tbl = static_cast<unsigned>(table);
-                const bool pow2 = (tbl & (tbl - 1u)) == 0u;
-                const int cx = cell_coord(cj.x, cell), cy = cell_coord(cj.y, cell), cz = cell_coord(cj.z, cell);
-                const int hk =
-                    int(cell_hash(cell_coord(ck.x, cell), cell_coord(ck.y, cell), cell_coord(ck.z, cell), table));
-                int mult = 0;
-                for (int dz = -1; dz <= 1; ++dz)
-                    for (int dy = -1; dy <= 1; ++dy) {
-                        const unsigned hzy = (static_cast<unsigned>(cz + dz) * 83492791u) ^
-                                             (static_cast<unsigned>(cy + dy) * 19349663u);
-#pragma unroll
-                        for (int dx = -1; dx <= 1; ++dx) mult += row_bucket(cx, dx, hzy, tbl, pow2) == hk ? 1 : 0;
-                    }
-                if (M > 0 && mult > 0) {
-#pragma unroll
-                    for (int m = 0; m < M; ++m)
-                        if (nk == m) kk[m] = k;
-                    ++nk;
-                }
-                count += mult;
-            }
+        if (cand) {
+            const unsigned long long n = word & 7ull;
+            word = n < kCandidates && k - j < 1024
+                       ? ((word + 1ull) | (static_cast<unsigned long long>(k - j) << (3 + 10 * n)))
+                       : ((word & ~7ull) | kCandidateOverflow);
         }
+        if (!(D2 > limK * limK))
+            self_count_one<M>(s, x4, segEnd, table, radius, cell, eps, cj, j0, j1, k, ck4, kk, nk, count);
         ++k;
+    }
+    if (cand) *cand = word;
+    return count;
+}
+
+// The count of self_count_scan from segment j's candidate list (not in
+// overflow): the same centre cull and self_count_one on each candidate, in
+// ascending order. Every segment outside the list is beyond the cull radius
+// (the caller keeps the lists only while that holds), so the count and kk[]
+// are the scan's.
+template <int M>
+__device__ inline int self_count_listed(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
+                                        real radius, real cell, real scaleMax, int j, int j0, int j1, real4 cj4,
+                                        int (&kk)[M > 0 ? M : 1], int& nk, unsigned long long word) {
+    const v3 cj = f4v(cj4);
+    const real reachJ = real(0.5) * cj4.w + real(2.0) * radius;
+    const real eps = real(1e-6) * (real(1.0) + scaleMax);
+    const int n = int(word & 7ull);
+    int count = 0;
+    for (int m = 0; m < n; ++m) {
+        // In range whatever the word holds.
+        const int k = min(j + int((word >> (3 + 10 * m)) & 1023ull), nS - 1);
+        const real4 ck4 = s.cen[k];
+        const real D2 = norm2(sub(f4v(ck4), cj));
+        const real limK = real(1.001) * (reachJ + real(0.5) * ck4.w) + eps;
+        if (!(D2 > limK * limK))
+            self_count_one<M>(s, x4, segEnd, table, radius, cell, eps, cj, j0, j1, k, ck4, kk, nk, count);
     }
     return count;
 }
@@ -662,6 +720,13 @@ __device__ __forceinline__ void self_regen_reset(const SelfLds& s) {
     s.rmisc[1] = 0;
     s.rmisc[2] = 0;
     s.rmisc[3] = real_bits(real(1e30));
+    s.rmisc[5] = 0;
+}
+// At launch start: no candidate lists yet, and no segment is long.
+__device__ __forceinline__ void self_candidates_reset(const SelfLds& s) {
+    s.rmisc[6] = 0;
+    s.rmisc[7] = 0;
+    s.rmisc[8] = real_bits(real(1e30));
 }
 __device__ __forceinline__ void self_list_reset(const SelfLds& s, int nS) {
     s.misc[2] = 0;
@@ -717,17 +782,45 @@ __device__ inline void self_build_tables(const SelfLds& s, int nS, int table, re
     __syncthreads();
 }
 
+// The skin of the candidate lists, in rod radii.
+constexpr float kSkinRadii = 1.0f;
+
 // Returns whether some rod of the workgroup has contacts (the same on every
-// lane).
+// lane). `moved` bounds how far this lane's particles moved since the
+// previous regeneration.
+//
+// Candidate lists. The count pass scans a segment's later segments only when
+// the rod's lists are rebuilt; otherwise it tests the segment's candidates,
+// the segments whose centres were within the cull radius plus a skin of r
+// (kSkinRadii radii) at the last rebuild. Since then every particle moved at
+// most `drift` (the sum over regenerations of the rod's largest per-particle
+// displacement bound), so every centre and every half-length moved or changed
+// by at most drift, and a pair's centre distance fell and its cull radius grew
+// by at most 2 drift and 2.002 drift (the cull radius carries the 1.001
+// margin). A segment outside the list was farther than its cull radius plus
+// the skin, so while 4.01 drift stays below half the skin (less an absolute
+// 1e-6 of the coordinate scale for rounding) it is still beyond the cull
+// radius and the count pass would have dropped it: the lists give the scan's
+// counts and contacts exactly. The lists are rebuilt at a launch's first
+// regeneration, when the bound is reached, and after the bucket tables (whose
+// arrays hold them) were built; a segment with more than kCandidates
+// candidates is scanned at every regeneration.
 template <int TP>
 __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
                                         int gap, int cap, real radius, real friction, bool chain, bool on, int l,
-                                        int G) {
+                                        int G, real moved) {
     real4 cj[TP];
-    // The longest and shortest segment and the largest coordinate: per lane,
-    // per wavefront stretch of the group, then one atomic per stretch (on
-    // values reset when the previous regeneration read them).
-    rbits lenHi = 0, lenLo = real_bits(real(1e30)), scaleHi = 0;
+    // The longest and shortest segment, the largest coordinate and the largest
+    // particle displacement: per lane, per wavefront stretch of the group, then
+    // one atomic per stretch (on values reset when the previous regeneration
+    // read them).
+    rbits lenHi = 0, lenLo = real_bits(real(1e30)), scaleHi = 0, movedHi = on ? real_bits(moved) : rbits(0);
+    // Segments longer than twice the shortest of the previous regeneration
+    // are "long"; the first few are listed, and skips stop at them. Any
+    // threshold gives the same counts: it only decides where the scan's skips
+    // stop.
+    const real lenLong = on ? bits_real(s.rmisc[8]) : real(0.0);
+    rbits shortHi = 0;
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
@@ -742,39 +835,37 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
             lenHi = lb > lenHi ? lb : lenHi;
             lenLo = lb < lenLo ? lb : lenLo;
             scaleHi = sb > scaleHi ? sb : scaleHi;
+            if (len > lenLong) {
+                const int slot = atomicAdd(&s.misc[2], 1);
+                if (slot < kLongSlots) s.misc[4 + slot] = j;
+            } else {
+                shortHi = lb > shortHi ? lb : shortHi;
+            }
         }
     }
     lenHi = group_wave_extreme<false>(lenHi, l);
     scaleHi = group_wave_extreme<false>(scaleHi, l);
     lenLo = group_wave_extreme<true>(lenLo, l);
+    movedHi = group_wave_extreme<false>(movedHi, l);
+    shortHi = group_wave_extreme<false>(shortHi, l);
     if (on && ((threadIdx.x & 63) == 63 || l == G - 1)) {
         atomicMax(&s.rmisc[1], lenHi);
         atomicMax(&s.rmisc[2], scaleHi);
         atomicMin(&s.rmisc[3], lenLo);
+        atomicMax(&s.rmisc[5], movedHi);
+        atomicMax(&s.rmisc[4], shortHi);
     }
     __syncthreads();
+    const real lenShort = on ? bits_real(s.rmisc[3]) : real(0.0);
     const real lenMax = on ? bits_real(s.rmisc[1]) : real(0.0);
     const real scaleMax = on ? bits_real(s.rmisc[2]) : real(0.0);
-    // Segments longer than twice the shortest are "long"; the first few are
-    // listed, and skips stop at them.
-    const real lenLong = on ? real(2.0) * bits_real(s.rmisc[3]) : real(0.0);
+    // Candidate lists (above): the displacement bound since they were built,
+    // and whether they are rebuilt now (the same on every lane of the rod).
+    const real drift = on ? bits_real(s.rmisc[6]) + bits_real(s.rmisc[5]) : real(0.0);
+    const real skin = real(kSkinRadii) * radius;
+    const bool rebuild = on && (s.rmisc[7] == 0 || !(real(4.01) * drift + real(1e-6) * (real(1.0) + scaleMax) <=
+                                                         real(0.5) * skin));
     const real cell = lenMax + real(2.0) * radius;
-    rbits shortHi = 0;
-#pragma unroll
-    for (int t = 0; t < TP; ++t) {
-        const int j = l + G * t;
-        if (on && j < nS && cj[t].w > lenLong) {
-            const int slot = atomicAdd(&s.misc[2], 1);
-            if (slot < kLongSlots) s.misc[4 + slot] = j;
-        } else if (on && j < nS) {
-            const rbits lb = real_bits(cj[t].w);
-            shortHi = lb > shortHi ? lb : shortHi;
-        }
-    }
-    shortHi = group_wave_extreme<false>(shortHi, l);
-    if (on && ((threadIdx.x & 63) == 63 || l == G - 1)) atomicMax(&s.rmisc[4], shortHi);
-    __syncthreads();
-    if (on && l == 0) self_regen_reset(s);
     // With more long segments than slots, the longest segment bounds every
     // stretch and none ends one. Otherwise a stretch holds no long segment, and
     // the longest of the others bounds it.
@@ -793,10 +884,28 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
         const int j = l + G * t;
         nk[t] = 0;
         if (on && j < nS) {
-            const int c = nS > gap ? self_count_scan<kM>(s, x4, segEnd, listed && numLong > 0, nS, table, gap,
-                                                          radius, cell, lenCap, scaleMax, chain, j, segEnd[2 * j],
-                                                          segEnd[2 * j + 1], cj[t], kk[t], nk[t])
-                                   : 0;
+            int c = 0;
+            if (nS > gap) {
+                // Each lane reads and writes only its own segments' words.
+                const bool longs = listed && numLong > 0;
+                if (rebuild) {
+                    unsigned long long w = 0;
+                    c = self_count_scan<kM>(s, x4, segEnd, longs, nS, table, gap, radius, cell, lenCap, scaleMax,
+                                            chain, j, segEnd[2 * j], segEnd[2 * j + 1], cj[t], kk[t], nk[t], skin,
+                                            &w);
+                    s.cellOf[j] = int(unsigned(w));
+                    s.sorted[j] = int(unsigned(w >> 32));
+                } else {
+                    const unsigned long long w =
+                        unsigned(s.cellOf[j]) | (static_cast<unsigned long long>(unsigned(s.sorted[j])) << 32);
+                    c = (w & 7ull) == kCandidateOverflow
+                            ? self_count_scan<kM>(s, x4, segEnd, longs, nS, table, gap, radius, cell, lenCap,
+                                                  scaleMax, chain, j, segEnd[2 * j], segEnd[2 * j + 1], cj[t], kk[t],
+                                                  nk[t], real(0.0), nullptr)
+                            : self_count_listed<kM>(s, x4, segEnd, nS, table, radius, cell, scaleMax, j, segEnd[2 * j],
+                                                    segEnd[2 * j + 1], cj[t], kk[t], nk[t], w);
+                }
+            }
             s.segCount[j] = c;
             counted = counted || c > 0;
         }
@@ -805,6 +914,15 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
     // a contact (the common case) the pool offsets are not summed and the fill
     // does not run.
     const bool any = __syncthreads_or(counted) != 0;
+    // Every lane of the rod has read the reductions, the long segments, the
+    // bound and the flag: record the new ones and reset the rest for the next
+    // regeneration.
+    if (on && l == 0) {
+        s.rmisc[6] = real_bits(rebuild ? real(0.0) : drift);
+        s.rmisc[7] = 1;
+        s.rmisc[8] = real_bits(real(2.0) * lenShort);
+        self_regen_reset(s);
+    }
     if (!any) {
         if (on && l == 0) {
             s.misc[0] = 0;
@@ -831,7 +949,11 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
         }
         over = __syncthreads_or(over) != 0;
     }
-    if (over) self_build_tables<TP>(s, nS, table, cell, on && total > 0, cj, l, G);
+    if (over) {
+        self_build_tables<TP>(s, nS, table, cell, on && total > 0, cj, l, G);
+        // The tables took the arrays that held the candidate lists.
+        if (on && total > 0 && l == 0) s.rmisc[7] = 0;
+    }
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int j = l + G * t;
@@ -999,10 +1121,13 @@ __device__ __forceinline__ int self_colour_of(const SelfLds& s, int c) {
 // word. misc[3] receives the number of colours in use below the tail, plus 64
 // when the tail is not empty. The per-particle colour masks take the
 // broadphase's per-segment arrays (4 nS ints from cellOf, at least nP; free once
-// the list is built); only the contacts' particles are cleared and read.
+// the list is built); only the contacts' particles are cleared and read. Those
+// arrays also hold the candidate lists, which are then rebuilt at the next
+// regeneration.
 __device__ inline void self_colour(const SelfLds& s) {
     unsigned* used = reinterpret_cast<unsigned*>(s.cellOf);
     const int live = s.misc[0];
+    if (live > 0) s.rmisc[7] = 0;
     for (int c = 0; c < live; ++c) {
         const real* rec = s.pool + c * kSelfFloats;
         for (int m = 0; m < 4; ++m) used[slot_int(rec[m])] = 0u;

@@ -82,9 +82,13 @@ def test_long_rods_do_not_fit_lds_and_auto_picks_global():
 @pytest.mark.parametrize("precision", [32, 64])
 def test_long_rods_bitwise_vs_oracle(precision):
     """A 2000-segment rod free of contacts and one resting its sag on a
-    floor, 60 steps (checked after 1, 10 and 60)."""
+    floor, 60 steps (checked after 1, 10 and 60). In FP32 its positions and
+    orientations (64 KB) are held in LDS for the launch; in FP64 (128 KB) they
+    stay in HBM with the other arrays."""
     for world in (None, FLOOR):
-        _check_against_oracle([long_rod()], world, benchmark_params(), [1, 10, 60], "global", precision)
+        b = _check_against_oracle([long_rod()], world, benchmark_params(), [1, 10, 60], "global", precision)
+        lds = b.counters()["lds_bytes_per_workgroup"]
+        assert lds == (2001 + 2001) * 16 if precision == 32 else lds == 0
 
 
 def test_long_rods_with_scales_contacts_and_ragged_sizes():
