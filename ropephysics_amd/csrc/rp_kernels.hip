@@ -456,14 +456,19 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     const int nP = ri.nP, nF = ri.nF, nS = ri.nS, nPrim = ri.nPrim;
     const TiledLayout L = tiled_layout(nP, nF, nS, nPrim, ri.self, ri.table, ri.selfCap);
     char* base = smem + ldsOff;
-    real4* x4 = reinterpret_cast<real4*>(base + L.x);
+    // Positions with inverse mass. A self-colliding rod keeps two buffers and
+    // alternates them: each substep predicts from one (which then holds the
+    // substep-start positions for the rest of the substep) into the other.
+    // Its broadphase can then read start positions before and after predict
+    // with no barrier between. Other rods update one buffer in place.
+    real4* const x4a = reinterpret_cast<real4*>(base + L.x);
+    real4* const x4b = reinterpret_cast<real4*>(base + L.xp);
     real4* q4s = reinterpret_cast<real4*>(base + L.q);
     real4* f4 = reinterpret_cast<real4*>(base + L.f);
     real4* kv4 = reinterpret_cast<real4*>(base + L.kv);
     real4* tq4 = reinterpret_cast<real4*>(base + L.tq);
     real4* cont4 = reinterpret_cast<real4*>(base + L.cont);
     real* prims = reinterpret_cast<real*>(base + L.prims);
-    real4* xp4 = reinterpret_cast<real4*>(base + L.xp);
     SelfLds sl;
     if (SELF && ri.self) sl = self_carve(base + L.self, nS, ri.table, ri.selfCap);
     // Any lane group in the workgroup may run self-collision; the barriers
@@ -484,7 +489,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
         if (active && i < nP) {
             vel[t] = ld3(d.v, p0 + i);
             wP[t] = d.invMass[p0 + i];
-            x4[i] = vf4(ld3(d.x, p0 + i), wP[t]);
+            x4a[i] = vf4(ld3(d.x, p0 + i), wP[t]);
             f4[i] = vf4(ld3(d.force, p0 + i), real(0.0));
             kv4[i] = vf4(ld3(d.kinVel, p0 + i), real(0.0));
         }
@@ -502,16 +507,20 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     for (int k = l; active && k < nPrim; k += G) stage_prim(prims + k * kPrimFloats, d.prims[ri.primOff + k]);
 
     // Stretch constraints: colour c, trip t -> position colS[c] + l + G t.
+    // Each constraint's fixed quotients and scaled compliance are formed here
+    // once (StretchConst, BendConst); a constraint whose participants are all
+    // fixed is off.
     int sA[2][TC], sB[2][TC], sF[2][TC];
-    real sRest[2][TC];
-    v3 sComp[2][TC], sIIv[2][TC], sLam[2][TC];
-    bool sOn[2][TC];
+    StretchConst sK[2][TC];
+    v3 sAlpha[2][TC], sIIv[2][TC], sLam[2][TC];
+    bool sOn[2][TC], sRot[2][TC];
     int bA_[2][TC], bB_[2][TC];
-    real bRest[2][TC];
-    v3 bComp[2][TC], bRD[2][TC], bIa[2][TC], bIb[2][TC], bLam[2][TC];
-    bool bOn[2][TC];
+    BendConst bK[2][TC];
+    v3 bAlpha[2][TC], bRD[2][TC], bIa[2][TC], bIb[2][TC], bLam[2][TC];
+    bool bOn[2][TC], bRotA[2][TC], bRotB[2][TC];
     const int* colS = d.colStart + ri.colOff;
     const int* colB = colS + ri.nSCol + 1;
+    const real cscale = active ? d.complianceScale[rod] : real(1.0);
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -523,13 +532,15 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                 if (k < colS[c + 1]) {
                     const int4 id = reinterpret_cast<const int4*>(d.sIdx)[ri.sStart + k];
                     const real4 par = reinterpret_cast<const real4*>(d.sPar)[ri.sStart + k];
-                    sOn[c][t] = true;
+                    const real w0 = d.invMass[p0 + id.x], w1 = d.invMass[p0 + id.y];
                     sA[c][t] = id.x;
                     sB[c][t] = id.y;
                     sF[c][t] = id.z;
-                    sRest[c][t] = par.x;
-                    sComp[c][t] = mk(par.y, par.z, par.w);
                     sIIv[c][t] = f4v(reinterpret_cast<const real4*>(d.invInertia4)[f0 + id.z]);
+                    sOn[c][t] = !stretch_fixed(w0, w1, sIIv[c][t]);
+                    sRot[c][t] = norm2(sIIv[c][t]) > real(0.0);
+                    sK[c][t] = stretch_const(w0, w1, par.x);
+                    sAlpha[c][t] = scaled_compliance(mk(par.y, par.z, par.w), sa.invH2, cscale);
                 }
             }
             if (active && c < ri.nBCol) {
@@ -537,18 +548,19 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                 if (k < colB[c + 1]) {
                     const int4 id = reinterpret_cast<const int4*>(d.bIdx)[ri.bStart + k];
                     const real4 par = reinterpret_cast<const real4*>(d.bPar)[ri.bStart + k];
-                    bOn[c][t] = true;
                     bA_[c][t] = id.x;
                     bB_[c][t] = id.y;
-                    bRest[c][t] = par.x;
-                    bComp[c][t] = mk(par.y, par.z, par.w);
                     bRD[c][t] = f4v(reinterpret_cast<const real4*>(d.bRest4)[ri.bStart + k]);
                     bIa[c][t] = f4v(reinterpret_cast<const real4*>(d.invInertia4)[f0 + id.x]);
                     bIb[c][t] = f4v(reinterpret_cast<const real4*>(d.invInertia4)[f0 + id.y]);
+                    bOn[c][t] = !bend_fixed(bIa[c][t], bIb[c][t]);
+                    bRotA[c][t] = norm2(bIa[c][t]) > real(0.0);
+                    bRotB[c][t] = norm2(bIb[c][t]) > real(0.0);
+                    bK[c][t] = bend_const(par.x);
+                    bAlpha[c][t] = scaled_compliance(mk(par.y, par.z, par.w), sa.invH2, cscale);
                 }
             }
         }
-    const real cscale = active ? d.complianceScale[rod] : real(1.0);
     const real fscale = active ? d.frictionScale[rod] : real(1.0);
     const int* segEnd = d.segEnd + 2 * ri.sStart;
     if (SELF && ri.self && l == 0) {
@@ -558,9 +570,15 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
         self_list_reset(sl, nS);
         self_candidates_reset(sl);
     }
-    // How far this lane's particles moved since the last regeneration (a
-    // bound: per substep the largest displacement, summed).
+    // How far this lane's particles moved since the rod's candidate lists were
+    // last rebuilt (a bound: per substep the largest displacement, summed),
+    // and whether that bound has invalidated the lists (self_lists_stale).
     real moved = real(0.0);
+    bool stale = false;
+    // Which position buffer holds the current positions (double-buffered
+    // rods); it flips at every substep the rod takes.
+    bool flip = false;
+    const bool dbl = SELF && ri.self;
     const v3 g = mk(sa.gx, sa.gy, sa.gz);
     const real h = sa.h;
     int clk = active ? d.clock[rod] : 0;
@@ -594,6 +612,10 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
         if (driver && live) kv4[drivenP] = vf4(path_velocity(d.paths[rod], clk), real(0.0));
         for (int sub = 0; sub < sa.substeps; ++sub) {
             const bool regen = sub % sa.contactInterval == 0;
+            // xs: positions at the start of the substep; x4: the positions the
+            // substep moves (the same buffer for single-buffered rods).
+            real4* const xs = dbl && flip ? x4b : x4a;
+            real4* const x4 = dbl ? (flip ? x4a : x4b) : x4a;
 #pragma unroll
             for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -606,38 +628,30 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                 for (int t = 0; t < TP; ++t) {
                     const int i = l + G * t;
                     if (live && i < nP)
-                        world_contacts_begin(cont4 + i, nP, prims, nPrim, f4v(x4[i]), wP[t],
+                        world_contacts_begin(cont4 + i, nP, prims, nPrim, f4v(xs[i]), wP[t],
                                              ri.radius, ri.margin, fscale, regen);
                 }
             }
+            // Self contacts: each lane first checks, with no barrier, whether its
+            // segments need the full regeneration (self_quick_check); the vote
+            // at the barrier after predict decides whether the workgroup runs
+            // it, on the substep-start positions.
+            bool needFull = false;
             if (anySelf) {
-                if (regen) {
-                    wgContacts = self_regen_group<TP>(sl, x4, segEnd, nS, ri.table, ri.selfGap, ri.selfCap,
-                                                      ri.radius, ri.selfFriction * fscale, ri.selfChain != 0,
-                                                      ri.self != 0 && live, l, G, moved);
-                    moved = real(0.0);
-                } else if (ri.self && l == 0 && live) {
+                if (regen)
+                    needFull = self_quick_check<TP>(sl, xs, segEnd, nS, ri.selfGap, ri.radius, ri.self != 0 && live,
+                                                    stale, l, G);
+                else if (ri.self && l == 0 && live)
                     self_reset(sl);
-                }
-                // The fill pass reads other lanes' positions, which predict
-                // then moves. Without self contacts in the workgroup there was
-                // no fill: the vote inside the regeneration was the last
-                // cross-lane step, and the lists stay empty.
-                if (wgContacts) __syncthreads();
-                if (anyColoured && regen && wgContacts) {
-                    if (ri.self && ri.selfOrder && live && l == 0) self_colour(sl);
-                    __syncthreads();
-                }
             }
             // ---- predict
 #pragma unroll
             for (int t = 0; t < TP; ++t) {
                 const int i = l + G * t;
                 if (live && i < nP) {
-                    v3 x = f4v(x4[i]);
+                    v3 x = f4v(xs[i]);
                     predict_particle(x, vel[t], xpv[t], wP[t], f4v(f4[i]), f4v(kv4[i]), h, g);
                     x4[i] = vf4(x, wP[t]);
-                    if (SELF && ri.self) xp4[i] = vf4(xpv[t], wP[t]);
                 }
                 const int j = l + G * t;
                 if (live && j < nF) {
@@ -646,7 +660,28 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                     q4s[j] = qf4(q);
                 }
             }
-            __syncthreads();
+            if (anySelf && regen) {
+                if (__syncthreads_or(needFull)) {
+                    bool rebuilt = false;
+                    wgContacts = self_regen_group<TP>(sl, xs, segEnd, nS, ri.table, ri.selfGap, ri.selfCap,
+                                                      ri.radius, ri.selfFriction * fscale, ri.selfChain != 0,
+                                                      ri.self != 0 && live, l, G, rebuilt);
+                    if (rebuilt) {
+                        moved = real(0.0);
+                        stale = false;
+                    }
+                } else {
+                    wgContacts = false;
+                }
+                // The coloured order colours the contacts the fill wrote.
+                if (anyColoured && wgContacts) {
+                    __syncthreads();
+                    if (ri.self && ri.selfOrder && live && l == 0) self_colour(sl);
+                    __syncthreads();
+                }
+            } else {
+                __syncthreads();
+            }
 
             for (int it = 0; it < sa.iterations; ++it) {
 #pragma unroll
@@ -657,8 +692,8 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                             const real4 ax = x4[sA[c][t]], bx = x4[sB[c][t]];
                             v3 xa = f4v(ax), xb = f4v(bx);
                             q4 q = f4q(q4s[sF[c][t]]);
-                            if (project_stretch(xa, xb, q, sLam[c][t], ax.w, bx.w, sIIv[c][t], sRest[c][t],
-                                                sComp[c][t], sa.invH2, cscale)) {
+                            if (project_stretch_k(xa, xb, q, sLam[c][t], sK[c][t], sIIv[c][t], sRot[c][t],
+                                                  sAlpha[c][t])) {
                                 x4[sA[c][t]] = vf4(xa, ax.w);
                                 x4[sB[c][t]] = vf4(xb, bx.w);
                                 q4s[sF[c][t]] = qf4(q);
@@ -673,8 +708,8 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                     for (int t = 0; t < TC; ++t) {
                         if (live && bOn[c][t]) {
                             q4 qa = f4q(q4s[bA_[c][t]]), qb = f4q(q4s[bB_[c][t]]);
-                            if (project_bend(qa, qb, bLam[c][t], bIa[c][t], bIb[c][t], bRest[c][t], bComp[c][t],
-                                             bRD[c][t], sa.invH2, cscale)) {
+                            if (project_bend_k(qa, qb, bLam[c][t], bIa[c][t], bIb[c][t], bK[c][t], bAlpha[c][t],
+                                               bRD[c][t], bRotA[c][t], bRotB[c][t])) {
                                 q4s[bA_[c][t]] = qf4(qa);
                                 q4s[bB_[c][t]] = qf4(qb);
                             }
@@ -709,9 +744,9 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                 // Self contacts exist in the workgroup only when the last
                 // regeneration's vote said so; otherwise nothing is projected.
                 if (anySelf && wgContacts) {
-                    if (ri.self && !ri.selfOrder && l == 0 && live) self_project(sl, x4, xp4);
+                    if (ri.self && !ri.selfOrder && l == 0 && live) self_project(sl, x4, xs);
                     if (anyColoured)
-                        self_project_coloured(sl, x4, xp4, ri.self && ri.selfOrder && live, l, G);
+                        self_project_coloured(sl, x4, xs, ri.self && ri.selfOrder && live, l, G);
                     __syncthreads();
                 }
             }
@@ -731,7 +766,14 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
             }
             // Rounded up past the rounding of the distance.
             if (SELF) moved += r_sqrt(moved2) * real(1.000001);
+            if (SELF && ri.self && live && !stale && self_lists_stale(sl, moved, ri.radius)) {
+                stale = true;
+                sl.rmisc[7] = 0;
+            }
+            if (live) flip = !flip;
         }
+        // The rod's current positions.
+        real4* const x4 = dbl && flip ? x4b : x4a;
         if (!live) continue;
         ++clk;
         if (observer && clk % d.obs[rod].every == 0) {
@@ -751,11 +793,12 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     // The write-back addresses are formed here, after the loop.
     int p0e = p0, f0e = f0;
     asm volatile("" : "+v"(p0e), "+v"(f0e), "+v"(rod));
+    const real4* const x4f = dbl && flip ? x4b : x4a;
 #pragma unroll
     for (int t = 0; t < TP; ++t) {
         const int i = l + G * t;
         if (active && i < nP) {
-            st3(d.x, p0e + i, f4v(x4[i]));
+            st3(d.x, p0e + i, f4v(x4f[i]));
             st3(d.v, p0e + i, vel[t]);
         }
         const int j = l + G * t;

@@ -308,14 +308,34 @@ RP_HD int cell_coord(real v, real cell) { return static_cast<int>(r_floor(v / ce
 // false when the constraint is skipped (all participants fixed, or a failed
 // Cholesky), in which case nothing is written.
 
-// Stretch/shear: C = R(q)^T (x1 - x0)/l - e3.
-RP_HD bool project_stretch(v3& x0, v3& x1, q4& q, v3& lam, real w0, real w1, v3 iI, real l, v3 compliance,
-                           real invH2, real cscale) {
-    if (w0 == real(0.0) && w1 == real(0.0) && norm2(iI) == real(0.0)) return false;
-    const v3 u = divs(rotate_inv(q, sub(x1, x0)), l);
+// The quotients and scaled compliances a projection forms from values that
+// stay fixed while a batch steps (inverse masses, rest lengths, compliances,
+// the substep): a kernel that runs many substeps forms them once, by the same
+// operations, so its projections give the same results.
+struct StretchConst {
+    real invL, wL2, w0L, w1L;  // 1/l, (w0 + w1)/(l l), w0/l, w1/l
+};
+RP_HD StretchConst stretch_const(real w0, real w1, real l) {
+    return StretchConst{real(1.0) / l, (w0 + w1) / (l * l), w0 / l, w1 / l};
+}
+struct BendConst {
+    real twoL, invL;  // 2/lbar, 1/lbar
+};
+RP_HD BendConst bend_const(real lbar) { return BendConst{real(2.0) / lbar, real(1.0) / lbar}; }
+// alpha = compliance / h^2, with the batch's per-rod scale.
+RP_HD v3 scaled_compliance(v3 compliance, real invH2, real cscale) { return scale(compliance, invH2 * cscale); }
+// Whether a constraint is skipped whatever the state (all participants fixed).
+RP_HD bool stretch_fixed(real w0, real w1, v3 iI) {
+    return w0 == real(0.0) && w1 == real(0.0) && norm2(iI) == real(0.0);
+}
+RP_HD bool bend_fixed(v3 iIa, v3 iIb) { return norm2(iIa) == real(0.0) && norm2(iIb) == real(0.0); }
+
+// Stretch/shear: C = R(q)^T (x1 - x0)/l - e3, for a constraint that is not
+// fixed; `rotates` is norm2(iI) > 0.
+RP_HD bool project_stretch_k(v3& x0, v3& x1, q4& q, v3& lam, const StretchConst& k, v3 iI, bool rotates, v3 alpha) {
+    const v3 u = scale(rotate_inv(q, sub(x1, x0)), k.invL);
     const v3 C = sub(u, mk(real(0.0), real(0.0), real(1.0)));
-    const v3 alpha = scale(compliance, invH2 * cscale);
-    m3 A = madd(mident((w0 + w1) / (l * l)), sandwich(mskew(u), iI));
+    m3 A = madd(mident(k.wL2), sandwich(mskew(u), iI));
     A.m[0][0] += alpha.x;
     A.m[1][1] += alpha.y;
     A.m[2][2] += alpha.z;
@@ -323,24 +343,30 @@ RP_HD bool project_stretch(v3& x0, v3& x1, q4& q, v3& lam, real w0, real w1, v3 
     if (!chol3(A, neg(add(C, cwise(alpha, lam))), dl)) return false;
     lam = add(lam, dl);
     const v3 wv = rotate(q, dl);
-    x0 = sub(x0, scale(wv, w0 / l));
-    x1 = add(x1, scale(wv, w1 / l));
-    if (norm2(iI) > real(0.0)) q = body_delta(q, cwise(iI, cross(dl, u)));
+    x0 = sub(x0, scale(wv, k.w0L));
+    x1 = add(x1, scale(wv, k.w1L));
+    if (rotates) q = body_delta(q, cwise(iI, cross(dl, u)));
     return true;
 }
 
+RP_HD bool project_stretch(v3& x0, v3& x1, q4& q, v3& lam, real w0, real w1, v3 iI, real l, v3 compliance,
+                           real invH2, real cscale) {
+    if (stretch_fixed(w0, w1, iI)) return false;
+    return project_stretch_k(x0, x1, q, lam, stretch_const(w0, w1, l), iI, norm2(iI) > real(0.0),
+                             scaled_compliance(compliance, invH2, cscale));
+}
+
 // Bend/twist: p = conj(qa) qb on the identity's hemisphere,
-// C = 2 Im(p)/lbar - Omega_rest.
-RP_HD bool project_bend(q4& qa, q4& qb, v3& lam, v3 iIa, v3 iIb, real lbar, v3 compliance, v3 restD, real invH2,
-                        real cscale) {
-    if (norm2(iIa) == real(0.0) && norm2(iIb) == real(0.0)) return false;
+// C = 2 Im(p)/lbar - Omega_rest, for a constraint that is not fixed; rotA and
+// rotB are norm2(iIa) > 0 and norm2(iIb) > 0.
+RP_HD bool project_bend_k(q4& qa, q4& qb, v3& lam, v3 iIa, v3 iIb, const BendConst& k, v3 alpha, v3 restD,
+                          bool rotA, bool rotB) {
     const q4 p = hemi_identity(qmul(conj(qa), qb));
     const v3 pv = im(p);
-    const v3 C = sub(scale(pv, real(2.0) / lbar), restD);
+    const v3 C = sub(scale(pv, k.twoL), restD);
     const m3 sk = mskew(pv);
-    const m3 Ja = mscale(madd(mident(-p.w), sk), real(1.0) / lbar);
-    const m3 Jb = mscale(madd(mident(p.w), sk), real(1.0) / lbar);
-    const v3 alpha = scale(compliance, invH2 * cscale);
+    const m3 Ja = mscale(madd(mident(-p.w), sk), k.invL);
+    const m3 Jb = mscale(madd(mident(p.w), sk), k.invL);
     m3 A = madd(sandwich(Ja, iIa), sandwich(Jb, iIb));
     A.m[0][0] += alpha.x;
     A.m[1][1] += alpha.y;
@@ -348,15 +374,22 @@ RP_HD bool project_bend(q4& qa, q4& qb, v3& lam, v3 iIa, v3 iIb, real lbar, v3 c
     v3 dl;
     if (!chol3(A, neg(add(C, cwise(alpha, lam))), dl)) return false;
     lam = add(lam, dl);
-    if (norm2(iIa) > real(0.0)) {
-        const v3 g = scale(sub(scale(dl, -p.w), cross(pv, dl)), real(1.0) / lbar);
+    if (rotA) {
+        const v3 g = scale(sub(scale(dl, -p.w), cross(pv, dl)), k.invL);
         qa = body_delta(qa, cwise(iIa, g));
     }
-    if (norm2(iIb) > real(0.0)) {
-        const v3 g = scale(sub(scale(dl, p.w), cross(pv, dl)), real(1.0) / lbar);
+    if (rotB) {
+        const v3 g = scale(sub(scale(dl, p.w), cross(pv, dl)), k.invL);
         qb = body_delta(qb, cwise(iIb, g));
     }
     return true;
+}
+
+RP_HD bool project_bend(q4& qa, q4& qb, v3& lam, v3 iIa, v3 iIb, real lbar, v3 compliance, v3 restD, real invH2,
+                        real cscale) {
+    if (bend_fixed(iIa, iIb)) return false;
+    return project_bend_k(qa, qb, lam, iIa, iIb, bend_const(lbar), scaled_compliance(compliance, invH2, cscale),
+                          restD, norm2(iIa) > real(0.0), norm2(iIb) > real(0.0));
 }
 
 // One rod-vs-world contact slot (a single particle, weight 1).

@@ -219,10 +219,10 @@ struct SelfLds {
     int* misc;       // [0] live, [1] overflow, [2] long segments, [3] colours in use (coloured order),
                      // [4..7] the first four long segments (nS: none)
     rbits* rmisc;    // [0] cell size, [1] longest segment, [2] largest |coordinate| of a centre, [3] shortest
-                     // segment, [4] longest segment that is not long, [5] largest particle displacement
-                     // since the last regeneration, [6] displacement bound since the candidate lists were
-                     // built, as bits (non-negative: ordered as integers); [7] 1 when the candidate lists
-                     // are valid; [8] the length above which a segment is long at the next regeneration
+                     // segment, [4] longest segment that is not long, as bits (non-negative: ordered as
+                     // integers); [5] unused; [6] the largest |coordinate| of a centre at the last full
+                     // regeneration; [7] 1 while the candidate lists are valid (any lane of the rod
+                     // clears it); [8] the length above which a segment is long at the next regeneration
     int* scan;       // kScanWaves, scratch of the group-parallel scans
 };
 
@@ -720,7 +720,6 @@ __device__ __forceinline__ void self_regen_reset(const SelfLds& s) {
     s.rmisc[1] = 0;
     s.rmisc[2] = 0;
     s.rmisc[3] = real_bits(real(1e30));
-    s.rmisc[5] = 0;
 }
 // At launch start: no candidate lists yet, and no segment is long.
 __device__ __forceinline__ void self_candidates_reset(const SelfLds& s) {
@@ -786,35 +785,41 @@ __device__ inline void self_build_tables(const SelfLds& s, int nS, int table, re
 constexpr float kSkinRadii = 1.0f;
 
 // Returns whether some rod of the workgroup has contacts (the same on every
-// lane). `moved` bounds how far this lane's particles moved since the
-// previous regeneration.
+// lane); `rebuilt` tells the rod's lanes whether its candidate lists were
+// rebuilt. x4 holds the substep-start positions.
 //
 // Candidate lists. The count pass scans a segment's later segments only when
 // the rod's lists are rebuilt; otherwise it tests the segment's candidates,
 // the segments whose centres were within the cull radius plus a skin of r
-// (kSkinRadii radii) at the last rebuild. Since then every particle moved at
-// most `drift` (the sum over regenerations of the rod's largest per-particle
-// displacement bound), so every centre and every half-length moved or changed
-// by at most drift, and a pair's centre distance fell and its cull radius grew
-// by at most 2 drift and 2.002 drift (the cull radius carries the 1.001
-// margin). A segment outside the list was farther than its cull radius plus
-// the skin, so while 4.01 drift stays below half the skin (less an absolute
-// 1e-6 of the coordinate scale for rounding) it is still beyond the cull
-// radius and the count pass would have dropped it: the lists give the scan's
-// counts and contacts exactly. The lists are rebuilt at a launch's first
-// regeneration, when the bound is reached, and after the bucket tables (whose
-// arrays hold them) were built; a segment with more than kCandidates
-// candidates is scanned at every regeneration.
+// (kSkinRadii radii) at the last rebuild. Each lane sums, over the substeps
+// since then, a bound on how far its particles moved (self_lists_stale), so
+// every particle moved at most D, the largest of these sums over the rod's
+// lanes; every centre and every half-length then moved or changed by at most
+// D, and a pair's centre distance fell and its cull radius grew by at most 2 D
+// and 2.002 D (the cull radius carries the 1.001 margin). A segment outside
+// the list was farther than its cull radius plus the skin, so while 4.01 D
+// stays below half the skin (less an absolute 1e-6 of the coordinate scale
+// for rounding) it is still beyond the cull radius and the count pass would
+// have dropped it: the lists give the scan's counts and contacts exactly. A
+// lane whose sum passes that bound clears rmisc[7]; the lists are rebuilt at
+// a launch's first regeneration, at the first regeneration after a lane
+// cleared the flag, and after the bucket tables (whose arrays hold them) were
+// built; a segment with more than kCandidates candidates is scanned at every
+// regeneration.
 template <int TP>
 __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const int* segEnd, int nS, int table,
                                         int gap, int cap, real radius, real friction, bool chain, bool on, int l,
-                                        int G, real moved) {
+                                        int G, bool& rebuilt) {
     real4 cj[TP];
-    // The longest and shortest segment, the largest coordinate and the largest
-    // particle displacement: per lane, per wavefront stretch of the group, then
-    // one atomic per stretch (on values reset when the previous regeneration
-    // read them).
-    rbits lenHi = 0, lenLo = real_bits(real(1e30)), scaleHi = 0, movedHi = on ? real_bits(moved) : rbits(0);
+    // Whether the rod's candidate lists are rebuilt now (the same on every
+    // lane of the rod). Every lane reads the flag before lane 0 rewrites it
+    // after the vote below; the lanes that cleared it did so before the
+    // barrier that precedes this call.
+    const bool rebuild = on && s.rmisc[7] == 0;
+    // The longest and shortest segment and the largest coordinate: per lane,
+    // per wavefront stretch of the group, then one atomic per stretch (on
+    // values reset when the previous regeneration read them).
+    rbits lenHi = 0, lenLo = real_bits(real(1e30)), scaleHi = 0;
     // Segments longer than twice the shortest of the previous regeneration
     // are "long"; the first few are listed, and skips stop at them. Any
     // threshold gives the same counts: it only decides where the scan's skips
@@ -846,25 +851,18 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
     lenHi = group_wave_extreme<false>(lenHi, l);
     scaleHi = group_wave_extreme<false>(scaleHi, l);
     lenLo = group_wave_extreme<true>(lenLo, l);
-    movedHi = group_wave_extreme<false>(movedHi, l);
     shortHi = group_wave_extreme<false>(shortHi, l);
     if (on && ((threadIdx.x & 63) == 63 || l == G - 1)) {
         atomicMax(&s.rmisc[1], lenHi);
         atomicMax(&s.rmisc[2], scaleHi);
         atomicMin(&s.rmisc[3], lenLo);
-        atomicMax(&s.rmisc[5], movedHi);
         atomicMax(&s.rmisc[4], shortHi);
     }
     __syncthreads();
     const real lenShort = on ? bits_real(s.rmisc[3]) : real(0.0);
     const real lenMax = on ? bits_real(s.rmisc[1]) : real(0.0);
     const real scaleMax = on ? bits_real(s.rmisc[2]) : real(0.0);
-    // Candidate lists (above): the displacement bound since they were built,
-    // and whether they are rebuilt now (the same on every lane of the rod).
-    const real drift = on ? bits_real(s.rmisc[6]) + bits_real(s.rmisc[5]) : real(0.0);
     const real skin = real(kSkinRadii) * radius;
-    const bool rebuild = on && (s.rmisc[7] == 0 || !(real(4.01) * drift + real(1e-6) * (real(1.0) + scaleMax) <=
-                                                         real(0.5) * skin));
     const real cell = lenMax + real(2.0) * radius;
     // With more long segments than slots, the longest segment bounds every
     // stretch and none ends one. Otherwise a stretch holds no long segment, and
@@ -914,11 +912,12 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
     // a contact (the common case) the pool offsets are not summed and the fill
     // does not run.
     const bool any = __syncthreads_or(counted) != 0;
-    // Every lane of the rod has read the reductions, the long segments, the
-    // bound and the flag: record the new ones and reset the rest for the next
+    // Every lane of the rod has read the reductions, the long segments and
+    // the flag: record the new ones and reset the rest for the next
     // regeneration.
+    rebuilt = rebuild;
     if (on && l == 0) {
-        s.rmisc[6] = real_bits(rebuild ? real(0.0) : drift);
+        s.rmisc[6] = real_bits(scaleMax);
         s.rmisc[7] = 1;
         s.rmisc[8] = real_bits(real(2.0) * lenShort);
         self_regen_reset(s);
@@ -969,6 +968,76 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
                        segEnd[2 * j + 1], cj[t], s.segStart[j]);
     }
     return true;
+}
+
+// Whether this lane's displacement bound, `moved` (summed since the rod's
+// lists were rebuilt), may let a segment outside the lists reach its cull
+// radius (self_regen_group): 4.01 moved plus 1e-6 of the coordinate scale
+// reaches half the skin. The coordinate scale is the largest centre
+// coordinate at the last full regeneration (rmisc[6]) plus `moved`, a bound on
+// how far any of this lane's particles moved since. When every lane of the
+// rod says no, the rod's largest sum D passes the bound there too.
+__device__ __forceinline__ bool self_lists_stale(const SelfLds& s, real moved, real radius) {
+    const real scale = bits_real(s.rmisc[6]) + moved;
+    return !(real(4.01) * moved + real(1e-6) * (real(1.0) + scale) <= real(0.5) * real(kSkinRadii) * radius);
+}
+
+// Before a regeneration, with no barrier: whether the workgroup must run the
+// full one (self_regen_group) for this lane's segments. It must when the
+// rod's candidate lists are not valid (not built, or `stale`: this lane's
+// bound says they may miss a pair), when a segment's list overflowed (it is
+// scanned at every regeneration), or when a segment is in contact with one of
+// its candidates. Otherwise these segments have no contact: with valid lists
+// every pair in contact is a candidate pair, and the two culls below drop
+// only pairs that are not in contact (their margins, 1e-3 relative and 1e-6
+// times the larger centre coordinate of the pair, are far above the rounding
+// of the pair's centres, lengths and distances). The centres and lengths are
+// formed here from the substep-start positions xs, which no lane writes
+// during the substep, by the operations self_regen_group uses.
+template <int TP>
+__device__ inline bool self_quick_check(const SelfLds& s, const real4* xs, const int* segEnd, int nS, int gap,
+                                        real radius, bool on, bool stale, int l, int G) {
+    if (!on) return false;
+    if (stale || s.rmisc[7] == 0) return true;
+    if (nS <= gap) return false;
+#pragma unroll
+    for (int t = 0; t < TP; ++t) {
+        const int j = l + G * t;
+        if (j >= nS) continue;
+        const unsigned long long w =
+            unsigned(s.cellOf[j]) | (static_cast<unsigned long long>(unsigned(s.sorted[j])) << 32);
+        if ((w & 7ull) == kCandidateOverflow) return true;
+        const int n = int(w & 7ull);
+        if (n == 0) continue;
+        const int j0 = segEnd[2 * j], j1 = segEnd[2 * j + 1];
+        const v3 a0 = f4v(xs[j0]), a1 = f4v(xs[j1]);
+        const real lenJ = norm(sub(a1, a0));
+        const v3 cj = scale(add(a0, a1), real(0.5));
+        const real reachJ = real(0.5) * lenJ + real(2.0) * radius;
+        const v3 dj = sub(a1, a0);
+        const real aj = norm2(dj);
+        for (int m = 0; m < n; ++m) {
+            // In range whatever the word holds.
+            const int k = min(j + int((w >> (3 + 10 * m)) & 1023ull), nS - 1);
+            const int k0 = segEnd[2 * k], k1 = segEnd[2 * k + 1];
+            const v3 b0 = f4v(xs[k0]), b1 = f4v(xs[k1]);
+            const real lenK = norm(sub(b1, b0));
+            const v3 ck = scale(add(b0, b1), real(0.5));
+            const real eps = real(1e-6) * (real(1.0) + r_max(abs_max3(cj), abs_max3(ck)));
+            const real limK = real(1.001) * (reachJ + real(0.5) * lenK) + eps;
+            if (norm2(sub(ck, cj)) > limK * limK) continue;
+            // Distance from segment j to k's centre (self_count_one's cull).
+            const real tq =
+                aj > real(1e-30) ? r_max(real(0.0), min_ab(real(1.0), dot(sub(ck, a0), dj) / aj)) : real(0.0);
+            const v3 off = sub(add(a0, scale(dj, tq)), ck);
+            const real limP = real(1.001) * (real(0.5) * lenK + real(2.0) * radius) + eps;
+            if (norm2(off) > limP * limP) continue;
+            int c = 0;
+            self_pair(s, xs, j0, j1, a0, a1, k0, k1, radius, real(0.0), false, 0, 0, c);
+            if (c > 0) return true;
+        }
+    }
+    return false;
 }
 
 // Counts -> pool offsets; contacts past the capacity are tallied as overflow.
