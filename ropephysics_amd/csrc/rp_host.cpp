@@ -406,14 +406,20 @@ int build_records(const typename Prec<Real>::BatchDesc* desc, RecordsT<Real>& ou
                 for (int k = 0; k < ri.nS; ++k) lMin = std::min(lMin, desc->stretch_rest[ri.sStart + k]);
                 const int geometric = int(std::floor(Real(2) * ri.radius / lMin)) + 2;
                 ri.selfGap = std::max(wd.self_skip, geometric);
-                // Whether consecutive segments (index order) share a particle.
-                ri.selfChain = 1;
-                for (int k = 0; k + 1 < ri.nS; ++k) {
+                // Bit 0: consecutive segments (index order) share a particle.
+                // Bit 1: segment k joins particles k and k + 1 for every k, so
+                // the kernels form segment ends without the segEnd table.
+                int chain = 1, identity = 2;
+                for (int k = 0; k < ri.nS; ++k) {
                     const int g = ri.sStart + k;
                     const int a = desc->stretch_p0[g], b = desc->stretch_p1[g];
-                    const int c = desc->stretch_p0[g + 1], e = desc->stretch_p1[g + 1];
-                    if (a != c && a != e && b != c && b != e) ri.selfChain = 0;
+                    if (a != k || b != k + 1) identity = 0;
+                    if (k + 1 < ri.nS) {
+                        const int c = desc->stretch_p0[g + 1], e = desc->stretch_p1[g + 1];
+                        if (a != c && a != e && b != c && b != e) chain = 0;
+                    }
                 }
+                ri.selfChain = chain | identity;
                 if (wd.self_capacity > 0)
                     ri.selfCap = wd.self_capacity;
                 else if (wd.self_capacity_per_segment > 0)

@@ -562,7 +562,8 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
             }
         }
     const real fscale = active ? d.frictionScale[rod] : real(1.0);
-    const int* segEnd = d.segEnd + 2 * ri.sStart;
+    // Null for a rod whose segment k joins particles k and k + 1 (seg_a, seg_b).
+    const int* segEnd = (ri.selfChain & 2) ? nullptr : d.segEnd + 2 * ri.sStart;
     if (SELF && ri.self && l == 0) {
         sl.misc[0] = 0;
         sl.misc[1] = 0;
@@ -664,7 +665,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                 if (__syncthreads_or(needFull)) {
                     bool rebuilt = false;
                     wgContacts = self_regen_group<TP>(sl, xs, segEnd, nS, ri.table, ri.selfGap, ri.selfCap,
-                                                      ri.radius, ri.selfFriction * fscale, ri.selfChain != 0,
+                                                      ri.radius, ri.selfFriction * fscale, (ri.selfChain & 1) != 0,
                                                       ri.self != 0 && live, l, G, rebuilt);
                     if (rebuilt) {
                         moved = real(0.0);

@@ -38,7 +38,8 @@ struct RodInfoT {
     R radius;
     R margin;  // contact margin (margin radii * radius)
     R selfFriction;
-    int selfChain;  // consecutive segments (stretch constraints in index order) share a particle
+    int selfChain;  // bit 0: consecutive segments (stretch constraints in index order) share a particle;
+                    // bit 1: segment k joins particles k and k + 1 for every k
     int selfOrder;  // 0: self contacts projected in list order; 1: coloured order
     int pad;
 };

@@ -15,6 +15,11 @@ __device__ __forceinline__ v3 f4v(real4 a) { return mk(a.x, a.y, a.z); }
 __device__ __forceinline__ q4 f4q(real4 a) { return q4{a.x, a.y, a.z, a.w}; }
 __device__ __forceinline__ real4 vf4(v3 a, real w) { return make_real4(a.x, a.y, a.z, w); }
 __device__ __forceinline__ real4 qf4(q4 a) { return make_real4(a.w, a.x, a.y, a.z); }
+// The particles at the ends of a rod's segment k (its stretch constraint k):
+// from the rod's segEnd table, or, for a rod whose table is null, k and k + 1
+// (constraint k joins particles k and k + 1, the usual chain).
+__device__ __forceinline__ int seg_a(const int* segEnd, int k) { return segEnd ? segEnd[2 * k] : k; }
+__device__ __forceinline__ int seg_b(const int* segEnd, int k) { return segEnd ? segEnd[2 * k + 1] : k + 1; }
 __device__ __forceinline__ v3 ld3(const real* p, int i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 __device__ __forceinline__ void st3(real* p, int i, v3 a) {
     p[3 * i] = a.x;
@@ -259,7 +264,7 @@ __device__ inline void self_build_hash(const SelfLds& s, const real4* x4, const 
                                        real radius) {
     real maxSeg = real(0.0);
     for (int j = 0; j < nS; ++j) {
-        const v3 a = f4v(x4[segEnd[2 * j]]), b = f4v(x4[segEnd[2 * j + 1]]);
+        const v3 a = f4v(x4[seg_a(segEnd, j)]), b = f4v(x4[seg_b(segEnd, j)]);
         s.cen[j] = vf4(scale(add(a, b), real(0.5)), real(0.0));
         const real len = norm(sub(b, a));
         maxSeg = maxSeg < len ? len : maxSeg;
@@ -326,7 +331,7 @@ __device__ inline void self_query(const SelfLds& s, const real4* x4, const int* 
         const real cell = bits_real(s.rmisc[0]);
         const v3 cj = f4v(s.cen[j]);
         const int cx = cell_coord(cj.x, cell), cy = cell_coord(cj.y, cell), cz = cell_coord(cj.z, cell);
-        const int j0 = segEnd[2 * j], j1 = segEnd[2 * j + 1];
+        const int j0 = seg_a(segEnd, j), j1 = seg_b(segEnd, j);
         const v3 a0 = f4v(x4[j0]), a1 = f4v(x4[j1]);
         const int slotBase = fill ? s.segStart[j] : 0;
         for (int dz = -1; dz <= 1; ++dz)
@@ -337,7 +342,7 @@ __device__ inline void self_query(const SelfLds& s, const real4* x4, const int* 
                     for (int e = s.cellStart[h]; e < e1; ++e) {
                         const int k = s.sorted[e];
                         if (k - j < gap) continue;
-                        self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * k], segEnd[2 * k + 1], radius, friction, fill,
+                        self_pair(s, x4, j0, j1, a0, a1, seg_a(segEnd, k), seg_b(segEnd, k), radius, friction, fill,
                                   slotBase, cap, count);
                     }
                 }
@@ -472,7 +477,7 @@ __device__ inline void self_fill_rows(const SelfLds& s, const real4* x4, const i
                     const int u = __builtin_ctz(pass);
                     pass &= pass - 1u;
                     const int k = u == 0 ? kq[0] : u == 1 ? kq[1] : u == 2 ? kq[2] : kq[3];
-                    self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * k], segEnd[2 * k + 1], radius, friction, true,
+                    self_pair(s, x4, j0, j1, a0, a1, seg_a(segEnd, k), seg_b(segEnd, k), radius, friction, true,
                               slotBase, cap, count);
                 }
             }
@@ -514,7 +519,7 @@ __device__ inline void self_fill_listed(const SelfLds& s, const real4* x4, const
 #pragma unroll
                 for (int e = 0; e < M; ++e)
                     if (hk[e] == h)
-                        self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * kk[e]], segEnd[2 * kk[e] + 1], radius, friction,
+                        self_pair(s, x4, j0, j1, a0, a1, seg_a(segEnd, kk[e]), seg_b(segEnd, kk[e]), radius, friction,
                                   true, slotBase, cap, count);
             }
         }
@@ -542,7 +547,7 @@ __device__ __forceinline__ void self_count_one(const SelfLds& s, const real4* x4
     const real limP = real(1.001) * (real(0.5) * ck4.w + real(2.0) * radius) + eps;
     if (norm2(off) > limP * limP) return;
     int c = 0;
-    self_pair(s, x4, j0, j1, a0, a1, segEnd[2 * k], segEnd[2 * k + 1], radius, real(0.0), false, 0, 0, c);
+    self_pair(s, x4, j0, j1, a0, a1, seg_a(segEnd, k), seg_b(segEnd, k), radius, real(0.0), false, 0, 0, c);
     if (c == 0) return;
     const unsigned tbl = static_cast<unsigned>(table);
     const bool pow2 = (tbl & (tbl - 1u)) == 0u;
@@ -831,7 +836,7 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
         const int j = l + G * t;
         cj[t] = make_real4(real(0.), real(0.), real(0.), real(0.));
         if (on && j < nS) {
-            const v3 a = f4v(x4[segEnd[2 * j]]), b = f4v(x4[segEnd[2 * j + 1]]);
+            const v3 a = f4v(x4[seg_a(segEnd, j)]), b = f4v(x4[seg_b(segEnd, j)]);
             const real len = norm(sub(b, a));
             const v3 c = scale(add(a, b), real(0.5));
             cj[t] = vf4(c, len);
@@ -889,7 +894,7 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
                 if (rebuild) {
                     unsigned long long w = 0;
                     c = self_count_scan<kM>(s, x4, segEnd, longs, nS, table, gap, radius, cell, lenCap, scaleMax,
-                                            chain, j, segEnd[2 * j], segEnd[2 * j + 1], cj[t], kk[t], nk[t], skin,
+                                            chain, j, seg_a(segEnd, j), seg_b(segEnd, j), cj[t], kk[t], nk[t], skin,
                                             &w);
                     s.cellOf[j] = int(unsigned(w));
                     s.sorted[j] = int(unsigned(w >> 32));
@@ -898,10 +903,10 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
                         unsigned(s.cellOf[j]) | (static_cast<unsigned long long>(unsigned(s.sorted[j])) << 32);
                     c = (w & 7ull) == kCandidateOverflow
                             ? self_count_scan<kM>(s, x4, segEnd, longs, nS, table, gap, radius, cell, lenCap,
-                                                  scaleMax, chain, j, segEnd[2 * j], segEnd[2 * j + 1], cj[t], kk[t],
+                                                  scaleMax, chain, j, seg_a(segEnd, j), seg_b(segEnd, j), cj[t], kk[t],
                                                   nk[t], real(0.0), nullptr)
-                            : self_count_listed<kM>(s, x4, segEnd, nS, table, radius, cell, scaleMax, j, segEnd[2 * j],
-                                                    segEnd[2 * j + 1], cj[t], kk[t], nk[t], w);
+                            : self_count_listed<kM>(s, x4, segEnd, nS, table, radius, cell, scaleMax, j, seg_a(segEnd, j),
+                                                    seg_b(segEnd, j), cj[t], kk[t], nk[t], w);
                 }
             }
             s.segCount[j] = c;
@@ -959,13 +964,13 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
         if (!(on && j < nS && s.segCount[j] > 0)) continue;
         if constexpr (kM > 0) {
             if (nk[t] <= kM) {
-                self_fill_listed<kMr>(s, x4, segEnd, table, cap, radius, friction, cell, segEnd[2 * j],
-                                      segEnd[2 * j + 1], cj[t], s.segStart[j], kk[t], nk[t]);
+                self_fill_listed<kMr>(s, x4, segEnd, table, cap, radius, friction, cell, seg_a(segEnd, j),
+                                      seg_b(segEnd, j), cj[t], s.segStart[j], kk[t], nk[t]);
                 continue;
             }
         }
-        self_fill_rows(s, x4, segEnd, nS, table, gap, cap, radius, friction, cell, j, segEnd[2 * j],
-                       segEnd[2 * j + 1], cj[t], s.segStart[j]);
+        self_fill_rows(s, x4, segEnd, nS, table, gap, cap, radius, friction, cell, j, seg_a(segEnd, j),
+                       seg_b(segEnd, j), cj[t], s.segStart[j]);
     }
     return true;
 }
@@ -1009,7 +1014,7 @@ __device__ inline bool self_quick_check(const SelfLds& s, const real4* xs, const
         if ((w & 7ull) == kCandidateOverflow) return true;
         const int n = int(w & 7ull);
         if (n == 0) continue;
-        const int j0 = segEnd[2 * j], j1 = segEnd[2 * j + 1];
+        const int j0 = seg_a(segEnd, j), j1 = seg_b(segEnd, j);
         const v3 a0 = f4v(xs[j0]), a1 = f4v(xs[j1]);
         const real lenJ = norm(sub(a1, a0));
         const v3 cj = scale(add(a0, a1), real(0.5));
@@ -1019,7 +1024,7 @@ __device__ inline bool self_quick_check(const SelfLds& s, const real4* xs, const
         for (int m = 0; m < n; ++m) {
             // In range whatever the word holds.
             const int k = min(j + int((w >> (3 + 10 * m)) & 1023ull), nS - 1);
-            const int k0 = segEnd[2 * k], k1 = segEnd[2 * k + 1];
+            const int k0 = seg_a(segEnd, k), k1 = seg_b(segEnd, k);
             const v3 b0 = f4v(xs[k0]), b1 = f4v(xs[k1]);
             const real lenK = norm(sub(b1, b0));
             const v3 ck = scale(add(b0, b1), real(0.5));
