@@ -330,15 +330,37 @@ RP_HD bool stretch_fixed(real w0, real w1, v3 iI) {
 }
 RP_HD bool bend_fixed(v3 iIa, v3 iIb) { return norm2(iIa) == real(0.0) && norm2(iIb) == real(0.0); }
 
+// The stretch system matrix mident(wL2) + sandwich(mskew(u), iI), plus alpha
+// on the diagonal, with the entries chol3 reads (the lower triangle). Each
+// sandwich entry sums three products, one or two of which have a zero factor
+// of mskew(u): for finite operands they are signed zeros. The sum then equals
+// the remaining products' sum whenever that is nonzero. An off-diagonal entry
+// is then added to mident's +0, which turns a zero of either sign into +0 as
+// it does 0 + t, so the entry is 0 + t exactly. A diagonal entry can differ
+// only in the sign of a zero, and only when wL2 + entry + alpha is zero, where
+// chol3 fails whatever that sign (its pivots are tested with <= 0 and any
+// product subtracted from them has no negative zero to cancel). The results
+// are therefore those of the full products for every finite input.
+RP_HD m3 stretch_system(v3 u, v3 d, real wL2, v3 alpha) {
+    m3 A;
+    A.m[0][0] = (wL2 + ((u.z * d.y) * u.z + (u.y * d.z) * u.y)) + alpha.x;
+    A.m[1][1] = (wL2 + ((u.z * d.x) * u.z + (u.x * d.z) * u.x)) + alpha.y;
+    A.m[2][2] = (wL2 + ((u.y * d.x) * u.y + (u.x * d.y) * u.x)) + alpha.z;
+    A.m[1][0] = real(0.0) + ((-u.x * d.z) * u.y);
+    A.m[2][0] = real(0.0) + ((u.x * d.y) * -u.z);
+    A.m[2][1] = real(0.0) + ((-u.y * d.x) * u.z);
+    A.m[0][1] = A.m[1][0];
+    A.m[0][2] = A.m[2][0];
+    A.m[1][2] = A.m[2][1];
+    return A;
+}
+
 // Stretch/shear: C = R(q)^T (x1 - x0)/l - e3, for a constraint that is not
 // fixed; `rotates` is norm2(iI) > 0.
 RP_HD bool project_stretch_k(v3& x0, v3& x1, q4& q, v3& lam, const StretchConst& k, v3 iI, bool rotates, v3 alpha) {
     const v3 u = scale(rotate_inv(q, sub(x1, x0)), k.invL);
     const v3 C = sub(u, mk(real(0.0), real(0.0), real(1.0)));
-    m3 A = madd(mident(k.wL2), sandwich(mskew(u), iI));
-    A.m[0][0] += alpha.x;
-    A.m[1][1] += alpha.y;
-    A.m[2][2] += alpha.z;
+    const m3 A = stretch_system(u, iI, k.wL2, alpha);
     v3 dl;
     if (!chol3(A, neg(add(C, cwise(alpha, lam))), dl)) return false;
     lam = add(lam, dl);
