@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -103,6 +104,87 @@ struct DevArray {
     }
 };
 
+// Pinned host blocks that uploads are copied through, so a call can queue its
+// copies and return: a block is reused once the event recorded after its
+// copies has completed, and freed with the server.
+class Staging {
+   public:
+    ~Staging() {
+        for (Block& b : blocks_) {
+            if (b.done) (void)hipEventSynchronize(b.done);
+            if (b.done) (void)hipEventDestroy(b.done);
+            if (b.p) (void)hipHostFree(b.p);
+        }
+    }
+    // A block of at least `bytes` whose earlier copies are done; its index in
+    // *idx. Null when the pinned allocation fails.
+    char* get(size_t bytes, int* idx) {
+        for (size_t i = 0; i < blocks_.size(); ++i) {
+            Block& b = blocks_[i];
+            if (b.bytes < bytes) continue;
+            if (b.busy && hipEventQuery(b.done) != hipSuccess) continue;
+            b.busy = false;
+            *idx = int(i);
+            return static_cast<char*>(b.p);
+        }
+        Block b;
+        b.bytes = std::max<size_t>(bytes, size_t(1) << 20);
+        if (hipHostMalloc(&b.p, b.bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+        if (hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipHostFree(b.p);
+            return nullptr;
+        }
+        blocks_.push_back(b);
+        *idx = int(blocks_.size()) - 1;
+        return static_cast<char*>(b.p);
+    }
+    // The copies out of block `idx` are queued on `stream`.
+    hipError_t queued(int idx, hipStream_t stream) {
+        blocks_[size_t(idx)].busy = true;
+        return hipEventRecord(blocks_[size_t(idx)].done, stream);
+    }
+
+   private:
+    struct Block {
+        void* p = nullptr;
+        size_t bytes = 0;
+        hipEvent_t done = nullptr;
+        bool busy = false;
+    };
+    std::vector<Block> blocks_;
+};
+
+// Host-to-device copies gathered into one pinned block and queued together.
+class Upload {
+   public:
+    template <typename T>
+    void add(T* dst, long long at, const T* src, long long count) {
+        if (count <= 0) return;
+        parts_.push_back({dst + at, src, size_t(count) * sizeof(T), total_});
+        total_ += (size_t(count) * sizeof(T) + 15) & ~size_t(15);
+    }
+    int queue(Staging& staging, hipStream_t stream) {
+        if (parts_.empty()) return RP_OK;
+        int idx = 0;
+        char* block = staging.get(total_, &idx);
+        if (!block) return fail(RP_ERR_MEMORY, "pinned staging allocation failed");
+        for (const Part& p : parts_) std::memcpy(block + p.off, p.src, p.bytes);
+        for (const Part& p : parts_)
+            RP_HIP(hipMemcpyAsync(p.dst, block + p.off, p.bytes, hipMemcpyHostToDevice, stream));
+        RP_HIP(staging.queued(idx, stream));
+        return RP_OK;
+    }
+
+   private:
+    struct Part {
+        void* dst;
+        const void* src;
+        size_t bytes, off;
+    };
+    std::vector<Part> parts_;
+    size_t total_ = 0;
+};
+
 }  // namespace
 
 struct rp_server_impl {
@@ -139,6 +221,7 @@ struct rp_server_impl {
     std::vector<RodInfo> rods;  // host mirror, indexed by rod slot
     std::vector<int> colStart;  // host mirror of the colour-start table
     std::vector<unsigned char> selfOf;
+    Staging staging;
     std::map<int, Request> requests;  // resident, by id
     std::vector<int> finished;        // ids whose horizon is reached, not yet released
     int nextId = 1;
@@ -150,8 +233,7 @@ struct rp_server_impl {
     ~rp_server_impl() {
         if (evReady) (void)hipEventDestroy(evReady);
         if (evSelfDone) (void)hipEventDestroy(evSelfDone);
-        if (selfStream) (void)hipStreamDestroy(selfStream);
-        if (stream) (void)hipStreamDestroy(stream);
+        // The streams are the device's shared pair (server_streams).
     }
 
     template <typename T>
@@ -168,19 +250,44 @@ struct rp_server_impl {
         owned.push_back(std::move(b));
         return RP_OK;
     }
-
-    // Copy `count` host elements to device array `dst` at element `at`.
-    template <typename T>
-    int put(T* dst, long long at, const T* src, long long count) {
-        if (count <= 0) return RP_OK;
-        RP_HIP(hipMemcpyAsync(dst + at, src, size_t(count) * sizeof(T), hipMemcpyHostToDevice, stream));
-        return RP_OK;
-    }
 };
 
 namespace {
 
 rp_server_impl* S(void* p) { return static_cast<rp_server_impl*>(p); }
+
+// The two launches of a server step must run side by side. Streams created
+// with a compute-unit mask get hardware queues of their own, where plain
+// streams share the process's few queues once other streams exist, and two
+// launches on one queue run one after the other (measured: a time-sliced
+// stream's p50 and p99 latency nearly doubled while another server's streams
+// existed). Each such stream holds a hardware queue, so every server on a
+// device shares one pair, made on the first server's creation with that
+// device current and kept until the process ends: servers on one device
+// then queue behind each other, and each server's synchronisations also wait
+// for the others' queued work. The mask names every compute unit.
+int server_streams(hipStream_t* lean, hipStream_t* self) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex lock;
+    static hipStream_t pairs[kMaxDevices][2] = {};
+    int dev = 0, cus = 0;
+    RP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(RP_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> guard(lock);
+    if (!pairs[dev][0]) {
+        RP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        std::vector<uint32_t> mask(size_t((cus + 31) / 32), 0xffffffffu);
+        if (cus % 32) mask.back() = (1u << (cus % 32)) - 1u;
+        hipStream_t a = nullptr, b = nullptr;
+        RP_HIP(hipExtStreamCreateWithCUMask(&a, uint32_t(mask.size()), mask.data()));
+        RP_HIP(hipExtStreamCreateWithCUMask(&b, uint32_t(mask.size()), mask.data()));
+        pairs[dev][0] = a;
+        pairs[dev][1] = b;
+    }
+    *lean = pairs[dev][0];
+    *self = pairs[dev][1];
+    return RP_OK;
+}
 
 void release_ranges(rp_server_impl& s, const Request& q) {
     for (int k = 0; k < kNumPools; ++k) s.pools[k].give(q.start[k], q.count[k]);
@@ -269,8 +376,7 @@ int rp_server_create(const rp_server_desc* desc, void** out_server) {
     const long long cap[kNumPools] = {segs, segs, segs, segs, rods, 2 * rods, 18 * rods};
     for (int k = 0; k < kNumPools; ++k) s->pools[k].reset(cap[k]);
 
-    RP_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-    RP_HIP(hipStreamCreateWithFlags(&s->selfStream, hipStreamNonBlocking));
+    if ((rc = server_streams(&s->stream, &s->selfStream))) return rc;
     RP_HIP(hipEventCreateWithFlags(&s->evReady, hipEventDisableTiming));
     RP_HIP(hipEventCreateWithFlags(&s->evSelfDone, hipEventDisableTiming));
     DeviceArrays& d = s->d;
@@ -394,41 +500,43 @@ int rp_server_submit(void* server, const rp_batch_desc* req, const rp_request_st
     }
     std::vector<int> zerosR(R, 0), limit(R, steps);
     const DeviceArrays& d = s.d;
-    if ((rc = s.put(d.x, p0 * 3, st->x, P * 3)) ||
-        (rc = s.put(d.v, p0 * 3, st->v ? st->v : zeros3.data(), P * 3)) ||
-        (rc = s.put(const_cast<float*>(d.force), p0 * 3, st->force ? st->force : zeros3.data(), P * 3)) ||
-        (rc = s.put(const_cast<float*>(d.kinVel), p0 * 3, st->kin_vel ? st->kin_vel : zeros3.data(), P * 3)) ||
-        (rc = s.put(const_cast<float*>(d.invMass), p0, rec.invMass.data(), P)) ||
-        (rc = s.put(d.q, f0 * 4, st->q, F * 4)) ||
-        (rc = s.put(d.omega, f0 * 3, st->omega ? st->omega : zeros3.data(), F * 3)) ||
-        (rc = s.put(const_cast<float*>(d.torque), f0 * 3, st->torque ? st->torque : zeros3.data(), F * 3)) ||
-        (rc = s.put(reinterpret_cast<float4*>(const_cast<float*>(d.invInertia4)), f0, rec.iI4.data(), F)) ||
-        (rc = s.put(reinterpret_cast<float4*>(const_cast<float*>(d.inertia4)), f0, rec.I4.data(), F)) ||
-        (rc = s.put(reinterpret_cast<int4*>(const_cast<int*>(d.sIdx)), s0, rec.sIdx.data(), rec.S)) ||
-        (rc = s.put(reinterpret_cast<float4*>(const_cast<float*>(d.sPar)), s0, rec.sPar.data(), rec.S)) ||
-        (rc = s.put(const_cast<int*>(d.segEnd), 2 * s0, rec.segEnd.data(), 2LL * rec.S)) ||
-        (rc = s.put(reinterpret_cast<int4*>(const_cast<int*>(d.bIdx)), b0, rec.bIdx.data(), rec.B)) ||
-        (rc = s.put(reinterpret_cast<float4*>(const_cast<float*>(d.bPar)), b0, rec.bPar.data(), rec.B)) ||
-        (rc = s.put(reinterpret_cast<float4*>(const_cast<float*>(d.bRest4)), b0, rec.bRest4.data(), rec.B)) ||
-        (rc = s.put(const_cast<Prim*>(d.prims), prim0, rec.prims.data(), (long long)rec.prims.size())) ||
-        (rc = s.put(const_cast<int*>(d.colStart), c0, rec.colStart.data(), (long long)rec.colStart.size())) ||
-        (rc = s.put(const_cast<RodInfo*>(d.rods), r0, recs.data(), (long long)R)) ||
-        (rc = s.put(d.complianceScale, r0, comp.data(), (long long)R)) ||
-        (rc = s.put(d.frictionScale, r0, fric.data(), (long long)R)) ||
-        (rc = s.put(d.selfOverflow, r0, zerosR.data(), (long long)R)) ||
-        (rc = s.put(d.clock, r0, zerosR.data(), (long long)R)) ||
-        (rc = s.put(s.stepLimit, r0, limit.data(), (long long)R))) {
+    // Staged through pinned memory and queued: the request's first launch, on
+    // the same stream, starts after them, and submit does not wait.
+    Upload up;
+    up.add(d.x, p0 * 3, st->x, P * 3);
+    up.add(d.v, p0 * 3, st->v ? st->v : zeros3.data(), P * 3);
+    up.add(const_cast<float*>(d.force), p0 * 3, st->force ? st->force : zeros3.data(), P * 3);
+    up.add(const_cast<float*>(d.kinVel), p0 * 3, st->kin_vel ? st->kin_vel : zeros3.data(), P * 3);
+    up.add(const_cast<float*>(d.invMass), p0, rec.invMass.data(), P);
+    up.add(d.q, f0 * 4, st->q, F * 4);
+    up.add(d.omega, f0 * 3, st->omega ? st->omega : zeros3.data(), F * 3);
+    up.add(const_cast<float*>(d.torque), f0 * 3, st->torque ? st->torque : zeros3.data(), F * 3);
+    up.add(reinterpret_cast<float4*>(const_cast<float*>(d.invInertia4)), f0, rec.iI4.data(), F);
+    up.add(reinterpret_cast<float4*>(const_cast<float*>(d.inertia4)), f0, rec.I4.data(), F);
+    up.add(reinterpret_cast<int4*>(const_cast<int*>(d.sIdx)), s0, rec.sIdx.data(), rec.S);
+    up.add(reinterpret_cast<float4*>(const_cast<float*>(d.sPar)), s0, rec.sPar.data(), rec.S);
+    up.add(const_cast<int*>(d.segEnd), 2 * s0, rec.segEnd.data(), 2LL * rec.S);
+    up.add(reinterpret_cast<int4*>(const_cast<int*>(d.bIdx)), b0, rec.bIdx.data(), rec.B);
+    up.add(reinterpret_cast<float4*>(const_cast<float*>(d.bPar)), b0, rec.bPar.data(), rec.B);
+    up.add(reinterpret_cast<float4*>(const_cast<float*>(d.bRest4)), b0, rec.bRest4.data(), rec.B);
+    up.add(const_cast<Prim*>(d.prims), prim0, rec.prims.data(), (long long)rec.prims.size());
+    up.add(const_cast<int*>(d.colStart), c0, rec.colStart.data(), (long long)rec.colStart.size());
+    up.add(const_cast<RodInfo*>(d.rods), r0, recs.data(), (long long)R);
+    up.add(d.complianceScale, r0, comp.data(), (long long)R);
+    up.add(d.frictionScale, r0, fric.data(), (long long)R);
+    up.add(d.selfOverflow, r0, zerosR.data(), (long long)R);
+    up.add(d.clock, r0, zerosR.data(), (long long)R);
+    up.add(s.stepLimit, r0, limit.data(), (long long)R);
+    if ((rc = up.queue(s.staging, s.stream))) {
         release_ranges(s, q);
         return rc;
     }
-    // The host vectors above go out of scope: wait for the copies.
-    RP_HIP(hipStreamSynchronize(s.stream));
+    q.id = s.nextId++;
     for (size_t r = 0; r < R; ++r) {
         s.rods[size_t(r0) + r] = recs[r];
         s.selfOf[size_t(r0) + r] = recs[r].self ? (recs[r].selfOrder ? 2 : 1) : 0;
     }
     std::copy(rec.colStart.begin(), rec.colStart.end(), s.colStart.begin() + c0);
-    q.id = s.nextId++;
     q.steps = steps;
     s.requests[q.id] = q;
     s.dirty = true;
@@ -455,8 +563,9 @@ int rp_server_step(void* server, int* num_finished) {
     if (lean.empty() && self.empty()) return RP_OK;
     const long long R = s.pools[kRods].capacity();
     if (s.dirty) {
-        // Two workgroups per compute unit when every rod fits that budget.
         Packing* packs[2] = {&s.packLean, &s.packSelf};
+        Upload up;
+        // Two workgroups per compute unit when every rod fits that budget.
         const std::vector<int>* ids[2] = {&lean, &self};
         const int cfgs[2] = {s.configLean, s.configSelf};
         for (int k = 0; k < 2; ++k) {
@@ -466,13 +575,12 @@ int rp_server_step(void* server, int* num_finished) {
             int rc = pack_rods(s.rods, *ids[k], s.colStart, cfgs[k], lds_budget(s.lim, 2), *packs[k]);
             if (rc) rc = pack_rods(s.rods, *ids[k], s.colStart, cfgs[k], lds_budget(s.lim, 1), *packs[k]);
             if (rc) return fail(rc, "resident rods do not pack");
-            RP_HIP(hipMemcpyAsync(s.dTiles + k * R, packs[k]->tiles.data(), packs[k]->tiles.size() * sizeof(Tile),
-                                  hipMemcpyHostToDevice, s.stream));
-            RP_HIP(hipMemcpyAsync(s.dSlots + k * R, packs[k]->slots.data(),
-                                  packs[k]->slots.size() * sizeof(TileSlot), hipMemcpyHostToDevice, s.stream));
+            up.add(s.dTiles + k * R, 0, packs[k]->tiles.data(), (long long)packs[k]->tiles.size());
+            up.add(s.dSlots + k * R, 0, packs[k]->slots.data(), (long long)packs[k]->slots.size());
         }
-        RP_HIP(hipStreamSynchronize(s.stream));
         s.dirty = false;
+        // Staged and queued: the launches below follow on the same stream.
+        if (int rc = up.queue(s.staging, s.stream)) return rc;
     }
     // Chunked: stepsPerLaunch steps. Time-sliced: up to the furthest horizon
     // (at most 8192 steps), the slice ending the launch.
@@ -586,9 +694,9 @@ int rp_server_release(void* server, int id) {
     if (const int rc = settle(s)) return rc;
     auto it = s.requests.find(id);
     if (it == s.requests.end()) return fail(RP_ERR_INVALID, "unknown request " + std::to_string(id));
-    // The ranges may be reused by the next submission: let queued work on
-    // them finish first.
-    RP_HIP(hipStreamSynchronize(s.stream));
+    // The ranges may be reused by the next submission, whose uploads queue on
+    // the stream after every launch that touches them (the second stream's
+    // launch is joined to it), so nothing waits here.
     release_ranges(s, it->second);
     s.finished.erase(std::remove(s.finished.begin(), s.finished.end(), id), s.finished.end());
     if (it->second.stepsDone < it->second.steps) s.dirty = true;

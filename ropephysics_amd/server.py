@@ -62,7 +62,8 @@ class Server:
     def submit(self, rods: list, world, steps: int, youngs_scale=None, friction_scale=None, x=None,
                world_of_rod=None) -> int:
         """Admit a request; returns its id. ``x`` (packed positions) overrides
-        the rods' own. Raises ServerFull when it does not fit now."""
+        the rods' own. Raises ServerFull when it does not fit now. The uploads
+        are queued behind the launch in flight, not waited for."""
         desc, keep, starts = describe(list(rods), world, world_of_rod)
         R = len(rods)
 

@@ -10,8 +10,10 @@ under the rods; self-collision on in half of the requests. Per-rod stiffness
 ``run_stream`` plays requests against a ``Server`` at given arrival times:
 arrived requests are admitted in arrival order while the budget has room,
 every running request advances in each launch, and finished results are read
-back as soon as their launch completes. Latency is arrival to result on the
-host, wall clock.
+back as soon as their launch completes. Admission runs while a launch is in
+flight (submission only queues its uploads behind it), so the host's work
+overlaps the device's. Latency is arrival to result on the host, wall
+clock.
 """
 from __future__ import annotations
 
@@ -110,7 +112,8 @@ def run_stream(server: Server, requests: list, arrivals: np.ndarray, keep_result
     done = 0
     launches = 0
     max_resident = 0
-    while done < n:
+    def admit():
+        nonlocal nxt
         now = time.perf_counter() - t0
         while nxt < n and arrivals[order[nxt]] <= now:
             queue.append(order[nxt])
@@ -124,6 +127,9 @@ def run_stream(server: Server, requests: list, arrivals: np.ndarray, keep_result
                 break
             ids[rid] = k
             queue.pop(0)
+
+    while done < n:
+        admit()
         st = server.stats()
         max_resident = max(max_resident, st["resident_requests"])
         if st["running_requests"] == 0:
@@ -132,6 +138,9 @@ def run_stream(server: Server, requests: list, arrivals: np.ndarray, keep_result
             continue
         server.step()
         launches += 1
+        # Requests arriving during the launch are admitted while it runs;
+        # they join the next one.
+        admit()
         server.synchronize()
         for rid in server.finished():
             k = ids.pop(rid)
