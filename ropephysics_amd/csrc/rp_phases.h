@@ -803,9 +803,10 @@ constexpr float kSkinRadii = 1.0f;
 // D, and a pair's centre distance fell and its cull radius grew by at most 2 D
 // and 2.002 D (the cull radius carries the 1.001 margin). A segment outside
 // the list was farther than its cull radius plus the skin, so while 4.01 D
-// stays below half the skin (less an absolute 1e-6 of the coordinate scale
-// for rounding) it is still beyond the cull radius and the count pass would
-// have dropped it: the lists give the scan's counts and contacts exactly. A
+// stays below the skin (less an absolute 1e-6 of the coordinate scale for
+// rounding; 4.01 over 4.002 covers the rounding of the sums) it is still
+// beyond the cull radius and the count pass would have dropped it: the lists
+// give the scan's counts and contacts exactly. A
 // lane whose sum passes that bound clears rmisc[7]; the lists are rebuilt at
 // a launch's first regeneration, at the first regeneration after a lane
 // cleared the flag, and after the bucket tables (whose arrays hold them) were
@@ -978,13 +979,13 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
 // Whether this lane's displacement bound, `moved` (summed since the rod's
 // lists were rebuilt), may let a segment outside the lists reach its cull
 // radius (self_regen_group): 4.01 moved plus 1e-6 of the coordinate scale
-// reaches half the skin. The coordinate scale is the largest centre
-// coordinate at the last full regeneration (rmisc[6]) plus `moved`, a bound on
+// reaches the skin. The coordinate scale is the largest centre
+// coordinate at the last rebuild (rmisc[6]) plus `moved`, a bound on
 // how far any of this lane's particles moved since. When every lane of the
 // rod says no, the rod's largest sum D passes the bound there too.
 __device__ __forceinline__ bool self_lists_stale(const SelfLds& s, real moved, real radius) {
     const real scale = bits_real(s.rmisc[6]) + moved;
-    return !(real(4.01) * moved + real(1e-6) * (real(1.0) + scale) <= real(0.5) * real(kSkinRadii) * radius);
+    return !(real(4.01) * moved + real(1e-6) * (real(1.0) + scale) <= real(kSkinRadii) * radius);
 }
 
 // Before a regeneration, with no barrier: whether the workgroup must run the
