@@ -586,6 +586,11 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     // Whether some rod of the workgroup had self contacts at the last
     // regeneration (the same on every lane).
     bool wgContacts = false;
+    // Per particle of the lane, its world contact slots that hold a contact
+    // (bit k for primitive k), set at every regeneration.
+    unsigned wAct[TP];
+#pragma unroll
+    for (int t = 0; t < TP; ++t) wAct[t] = 0u;
     // A rod whose clock has reached its step limit stops changing; its lanes
     // still take part in every barrier.
     const int limit = (active && d.stepLimit) ? d.stepLimit[rod] : 0x7fffffff;
@@ -629,8 +634,8 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                 for (int t = 0; t < TP; ++t) {
                     const int i = l + G * t;
                     if (live && i < nP)
-                        world_contacts_begin(cont4 + i, nP, prims, nPrim, f4v(xs[i]), wP[t],
-                                             ri.radius, ri.margin, fscale, regen);
+                        wAct[t] = world_contacts_begin_masked(cont4 + i, nP, prims, nPrim, f4v(xs[i]), wP[t],
+                                                              ri.radius, ri.margin, fscale, regen, wAct[t]);
                 }
             }
             // Self contacts: each lane first checks, with no barrier, whether its
@@ -730,7 +735,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                             const int i = l + G * t;
                             if (live && i < nP && wP[t] > real(0.0)) {
                                 v3 x = f4v(x4[i]);
-                                world_contacts_project(cont4 + i, nP, nPrim, x, xpv[t], wP[t]);
+                                world_contacts_project_masked(cont4 + i, nP, wAct[t], x, xpv[t], wP[t]);
                                 x4[i] = vf4(x, wP[t]);
                             }
                         }

@@ -176,6 +176,64 @@ __device__ __forceinline__ void world_contacts_project(real4* c4, int stride, in
     }
 }
 
+// The same two passes with the particle's active slots kept by the caller in
+// a bit mask (bit k: slot k holds a contact; nPrim <= 32): a regeneration
+// returns the new mask and writes only the slots that hold a contact, and the
+// reset and the projection touch only those. An inactive slot's words are
+// never read, so the arithmetic, the order over k and every result are
+// world_contacts_begin's and world_contacts_project's.
+static_assert(kMaxPrims <= 32, "one mask bit per primitive");
+__device__ __forceinline__ unsigned world_contacts_begin_masked(real4* c4, int stride, const real* prims, int nPrim,
+                                                                v3 x, real w, real radius, real margin, real fscale,
+                                                                bool regen, unsigned act) {
+    if (!regen) {
+        for (unsigned m = act; m != 0u; m &= m - 1u) {
+            real4* B = c4 + (2 * __builtin_ctz(m) + 1) * stride;
+            real4 b = *B;
+            b.y = real(0.0);  // lamN
+            b.z = real(0.0);  // applied
+            *B = b;
+        }
+        return act;
+    }
+    unsigned out = 0u;
+    for (int k = 0; k < nPrim; ++k) {
+        wcontact c;
+        if (prim_far(prims + k * kPrimFloats, x, radius, margin))
+            c.active = 0;
+        else
+            world_contact_generate(c, load_prim(prims + k * kPrimFloats), x, w, radius, margin, fscale);
+        if (c.active) {
+            real4* A = c4 + 2 * k * stride;
+            A[0] = make_real4(c.n.x, c.n.y, c.n.z, c.offset);
+            A[stride] = make_real4(c.friction, real(0.0), real(0.0), int_slot(1));
+            out |= 1u << k;
+        }
+    }
+    return out;
+}
+
+__device__ __forceinline__ void world_contacts_project_masked(real4* c4, int stride, unsigned act, v3& x, v3 xPrev,
+                                                              real w) {
+    for (; act != 0u; act &= act - 1u) {
+        const int k = __builtin_ctz(act);
+        const real4 a = c4[2 * k * stride];
+        real4* B = c4 + (2 * k + 1) * stride;
+        real4 b = *B;
+        wcontact c;
+        c.active = 1;
+        c.n = mk(a.x, a.y, a.z);
+        c.offset = a.w;
+        c.friction = b.x;
+        c.lamN = b.y;
+        c.applied = b.z;
+        world_contact_project(c, x, xPrev, w);
+        b.y = c.lamN;
+        b.z = c.applied;
+        *B = b;
+    }
+}
+
 // ------------------------------------------------------------------ drivers and observers
 
 __device__ inline void path_point(const PathRec& p, double t, double& x, double& y) {
@@ -980,7 +1038,7 @@ __device__ inline bool self_regen_group(const SelfLds& s, const real4* x4, const
 // lists were rebuilt), may let a segment outside the lists reach its cull
 // radius (self_regen_group): 4.01 moved plus 1e-6 of the coordinate scale
 // reaches the skin. The coordinate scale is the largest centre
-// coordinate at the last rebuild (rmisc[6]) plus `moved`, a bound on
+// coordinate at the last full regeneration (rmisc[6]) plus `moved`, a bound on
 // how far any of this lane's particles moved since. When every lane of the
 // rod says no, the rod's largest sum D passes the bound there too.
 __device__ __forceinline__ bool self_lists_stale(const SelfLds& s, real moved, real radius) {
