@@ -147,7 +147,7 @@ __device__ __forceinline__ void plain_rod_steps(const DeviceArrays& d, const Ste
         iI4[j] = reinterpret_cast<const real4*>(d.invInertia4)[f0 + j];
         I4[j] = reinterpret_cast<const real4*>(d.inertia4)[f0 + j];
     }
-    for (int k = tid; k < nPrim; k += nt) stage_prim(prims + k * kPrimFloats, d.prims[ri.primOff + k]);
+    for (int k = tid; k < nPrim; k += nt) stage_prim(prims + k * kPrimFloats, d.prims[ri.primOff + k], ri.radius, ri.margin);
     if (ri.self && tid == 0) {
         sl.misc[0] = 0;
         sl.misc[1] = 0;
@@ -504,7 +504,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
             tq4[j] = vf4(ld3(d.torque, f0 + j), real(0.0));
         }
     }
-    for (int k = l; active && k < nPrim; k += G) stage_prim(prims + k * kPrimFloats, d.prims[ri.primOff + k]);
+    for (int k = l; active && k < nPrim; k += G) stage_prim(prims + k * kPrimFloats, d.prims[ri.primOff + k], ri.radius, ri.margin);
 
     // Stretch constraints: colour c, trip t -> position colS[c] + l + G t.
     // Each constraint's fixed quotients and scaled compliance are formed here

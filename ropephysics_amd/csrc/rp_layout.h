@@ -18,7 +18,7 @@ namespace rp {
 constexpr int kMaxColours = 8;     // per family, PLAIN path
 constexpr int kTiledColours = 2;   // per family, TILED path
 constexpr int kMaxPrims = 8;       // per world
-constexpr int kPrimFloats = 20;    // one primitive staged in LDS
+constexpr int kPrimFloats = 24;    // one primitive staged in LDS
 constexpr int kContactFloats = 8;  // one world-contact slot
 constexpr int kSelfFloats = 16;    // one self-contact slot
 

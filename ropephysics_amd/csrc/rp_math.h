@@ -51,6 +51,7 @@ RP_HD real r_max(real a, real b) { return fmaxf(a, b); }
 __device__ __forceinline__ rbits real_bits(real a) { return __float_as_int(a); }
 __device__ __forceinline__ real bits_real(rbits b) { return __int_as_float(b); }
 #endif
+RP_HD real r_inf() { return real(__builtin_huge_val()); }
 // An int kept in a real slot of LDS (its bits; exact in either precision).
 __device__ __forceinline__ real int_slot(int i) { return bits_real(rbits(i)); }
 __device__ __forceinline__ int slot_int(real r) { return int(real_bits(r)); }

@@ -82,7 +82,7 @@ __device__ __forceinline__ prim load_prim(const real* s) {
     p.friction = s[18];
     return p;
 }
-__device__ __forceinline__ void stage_prim(real* s, const Prim& g) {
+__device__ __forceinline__ void stage_prim(real* s, const Prim& g, real radius, real margin) {
     s[0] = int_slot(g.type);
     for (int c = 0; c < 3; ++c) {
         s[1 + c] = g.a[c];
@@ -105,6 +105,15 @@ __device__ __forceinline__ void stage_prim(real* s, const Prim& g) {
         bound = norm(mk(g.half[0], g.half[1], g.half[2]));
     }
     s[19] = bound;
+    // The rod's far-test record (prim_far_staged): the bounding centre and
+    // the part of the cull radius that does not depend on the particle,
+    // formed by prim_far's operations; -inf for a shape with no bound.
+    v3 c = mk(s[1], s[2], s[3]);
+    if (g.type == kCapsule) c = scale(add(c, mk(s[4], s[5], s[6])), real(0.5));
+    s[20] = c.x;
+    s[21] = c.y;
+    s[22] = c.z;
+    s[23] = bound < real(0.0) ? -r_inf() : real(1.001) * (bound + radius + r_max(margin, real(0.0)));
 }
 
 // A particle farther from a convex primitive's bounding sphere than radius +
@@ -120,6 +129,17 @@ __device__ __forceinline__ bool prim_far(const real* s, v3 x, real radius, real 
     if (type == kCapsule) c = scale(add(c, mk(s[4], s[5], s[6])), real(0.5));
     const real lim = real(1.001) * (bound + radius + r_max(margin, real(0.0))) +
                      real(1e-6) * (real(1.0) + r_max(abs_max3v(x), abs_max3v(c)));
+    return norm2(sub(x, c)) > lim * lim;
+}
+
+// prim_far for the particles of the rod whose radius and margin the record
+// was staged with: one 16-byte read of the staged centre and radius part,
+// then prim_far's remaining operations on the same operands.
+__device__ __forceinline__ bool prim_far_staged(const real* s, v3 x) {
+    const real4 f = *reinterpret_cast<const real4*>(s + 20);
+    if (f.w == -r_inf()) return false;
+    const v3 c = mk(f.x, f.y, f.z);
+    const real lim = f.w + real(1e-6) * (real(1.0) + r_max(abs_max3v(x), abs_max3v(c)));
     return norm2(sub(x, c)) > lim * lim;
 }
 
@@ -199,7 +219,7 @@ __device__ __forceinline__ unsigned world_contacts_begin_masked(real4* c4, int s
     unsigned out = 0u;
     for (int k = 0; k < nPrim; ++k) {
         wcontact c;
-        if (prim_far(prims + k * kPrimFloats, x, radius, margin))
+        if (prim_far_staged(prims + k * kPrimFloats, x))
             c.active = 0;
         else
             world_contact_generate(c, load_prim(prims + k * kPrimFloats), x, w, radius, margin, fscale);
