@@ -629,6 +629,20 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                     sLam[c][t] = mk(real(0.), real(0.), real(0.));
                     bLam[c][t] = mk(real(0.), real(0.), real(0.));
                 }
+            // Self contacts: each lane first checks, with no barrier, whether its
+            // segments need the full regeneration (self_quick_check); the vote
+            // at the barrier after predict decides whether the workgroup runs
+            // it, on the substep-start positions. The check comes before the
+            // world contacts, so its LDS reads do not wait behind their slot
+            // writes.
+            bool needFull = false;
+            if (anySelf) {
+                if (regen)
+                    needFull = self_quick_check<TP>(sl, xs, segEnd, nS, ri.selfGap, ri.radius, ri.self != 0 && live,
+                                                    stale, l, G);
+                else if (ri.self && l == 0 && live)
+                    self_reset(sl);
+            }
             if (anyPrim && nPrim > 0) {
 #pragma unroll
                 for (int t = 0; t < TP; ++t) {
@@ -637,18 +651,6 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                         wAct[t] = world_contacts_begin_masked(cont4 + i, nP, prims, nPrim, f4v(xs[i]), wP[t],
                                                               ri.radius, ri.margin, fscale, regen, wAct[t]);
                 }
-            }
-            // Self contacts: each lane first checks, with no barrier, whether its
-            // segments need the full regeneration (self_quick_check); the vote
-            // at the barrier after predict decides whether the workgroup runs
-            // it, on the substep-start positions.
-            bool needFull = false;
-            if (anySelf) {
-                if (regen)
-                    needFull = self_quick_check<TP>(sl, xs, segEnd, nS, ri.selfGap, ri.radius, ri.self != 0 && live,
-                                                    stale, l, G);
-                else if (ri.self && l == 0 && live)
-                    self_reset(sl);
             }
             // ---- predict
 #pragma unroll

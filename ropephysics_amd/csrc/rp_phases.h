@@ -107,9 +107,11 @@ __device__ __forceinline__ void stage_prim(real* s, const Prim& g, real radius, 
     s[19] = bound;
     // The rod's far-test record (prim_far_staged): the bounding centre and
     // the part of the cull radius that does not depend on the particle,
-    // formed by prim_far's operations; -inf for a shape with no bound.
+    // formed by prim_far's operations. A shape with no bound has -inf there
+    // and its normal in place of the centre.
     v3 c = mk(s[1], s[2], s[3]);
     if (g.type == kCapsule) c = scale(add(c, mk(s[4], s[5], s[6])), real(0.5));
+    if (bound < real(0.0)) c = mk(s[7], s[8], s[9]);
     s[20] = c.x;
     s[21] = c.y;
     s[22] = c.z;
@@ -133,10 +135,9 @@ __device__ __forceinline__ bool prim_far(const real* s, v3 x, real radius, real 
 }
 
 // prim_far for the particles of the rod whose radius and margin the record
-// was staged with: one 16-byte read of the staged centre and radius part,
-// then prim_far's remaining operations on the same operands.
-__device__ __forceinline__ bool prim_far_staged(const real* s, v3 x) {
-    const real4 f = *reinterpret_cast<const real4*>(s + 20);
+// was staged with, from the record's far-test word f = s[20..23]: prim_far's
+// remaining operations on the same operands.
+__device__ __forceinline__ bool prim_far_staged(real4 f, v3 x) {
     if (f.w == -r_inf()) return false;
     const v3 c = mk(f.x, f.y, f.z);
     const real lim = f.w + real(1e-6) * (real(1.0) + r_max(abs_max3v(x), abs_max3v(c)));
@@ -218,11 +219,28 @@ __device__ __forceinline__ unsigned world_contacts_begin_masked(real4* c4, int s
     }
     unsigned out = 0u;
     for (int k = 0; k < nPrim; ++k) {
+        const real* sk = prims + k * kPrimFloats;
+        const real4 f = *reinterpret_cast<const real4*>(sk + 20);
         wcontact c;
-        if (prim_far_staged(prims + k * kPrimFloats, x))
-            c.active = 0;
-        else
-            world_contact_generate(c, load_prim(prims + k * kPrimFloats), x, w, radius, margin, fscale);
+        c.active = 0;
+        if (f.w == -r_inf()) {
+            // A plane reads its type and point in one word, its normal from
+            // the far-test word and its friction: the fields the plane case
+            // of world_contact_generate uses.
+            const real4 ta = *reinterpret_cast<const real4*>(sk);
+            if (slot_int(ta.x) == kPlane) {
+                prim pr;
+                pr.type = kPlane;
+                pr.a = mk(ta.y, ta.z, ta.w);
+                pr.normal = mk(f.x, f.y, f.z);
+                pr.friction = sk[18];
+                world_contact_generate(c, pr, x, w, radius, margin, fscale);
+            } else {
+                world_contact_generate(c, load_prim(sk), x, w, radius, margin, fscale);
+            }
+        } else if (!prim_far_staged(f, x)) {
+            world_contact_generate(c, load_prim(sk), x, w, radius, margin, fscale);
+        }
         if (c.active) {
             real4* A = c4 + 2 * k * stride;
             A[0] = make_real4(c.n.x, c.n.y, c.n.z, c.offset);
