@@ -591,6 +591,9 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
     unsigned wAct[TP];
 #pragma unroll
     for (int t = 0; t < TP; ++t) wAct[t] = 0u;
+    // One iteration with contacts regenerated at every substep: the world
+    // contacts are formed and projected in the sweep, with no slots.
+    const bool fusedWorld = sa.iterations == 1 && sa.contactInterval == 1;
     // A rod whose clock has reached its step limit stops changing; its lanes
     // still take part in every barrier.
     const int limit = (active && d.stepLimit) ? d.stepLimit[rod] : 0x7fffffff;
@@ -643,7 +646,7 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                 else if (ri.self && l == 0 && live)
                     self_reset(sl);
             }
-            if (anyPrim && nPrim > 0) {
+            if (anyPrim && nPrim > 0 && !fusedWorld) {
 #pragma unroll
                 for (int t = 0; t < TP; ++t) {
                     const int i = l + G * t;
@@ -737,7 +740,10 @@ k_tiled_step(DeviceArrays d, StepArgs sa, const Tile* tiles, const TileSlot* slo
                             const int i = l + G * t;
                             if (live && i < nP && wP[t] > real(0.0)) {
                                 v3 x = f4v(x4[i]);
-                                world_contacts_project_masked(cont4 + i, nP, wAct[t], x, xpv[t], wP[t]);
+                                if (fusedWorld)
+                                    world_contacts_fused(prims, nPrim, x, xpv[t], wP[t], ri.radius, ri.margin, fscale);
+                                else
+                                    world_contacts_project_masked(cont4 + i, nP, wAct[t], x, xpv[t], wP[t]);
                                 x4[i] = vf4(x, wP[t]);
                             }
                         }

@@ -204,6 +204,49 @@ __device__ __forceinline__ void world_contacts_project(real4* c4, int stride, in
 // never read, so the arithmetic, the order over k and every result are
 // world_contacts_begin's and world_contacts_project's.
 static_assert(kMaxPrims <= 32, "one mask bit per primitive");
+// world_contact_generate for the staged record sk of a rod's primitive,
+// after its far test: c.active = 0 for a primitive that is far. A plane
+// reads its type and point in one word, its normal from the far-test word and
+// its friction: the fields the plane case of world_contact_generate uses.
+__device__ __forceinline__ void world_contact_staged(wcontact& c, const real* sk, v3 x, real w, real radius,
+                                                     real margin, real fscale) {
+    const real4 f = *reinterpret_cast<const real4*>(sk + 20);
+    c.active = 0;
+    if (f.w == -r_inf()) {
+        const real4 ta = *reinterpret_cast<const real4*>(sk);
+        if (slot_int(ta.x) == kPlane) {
+            prim pr;
+            pr.type = kPlane;
+            pr.a = mk(ta.y, ta.z, ta.w);
+            pr.normal = mk(f.x, f.y, f.z);
+            pr.friction = sk[18];
+            world_contact_generate(c, pr, x, w, radius, margin, fscale);
+        } else {
+            world_contact_generate(c, load_prim(sk), x, w, radius, margin, fscale);
+        }
+    } else if (!prim_far_staged(f, x)) {
+        world_contact_generate(c, load_prim(sk), x, w, radius, margin, fscale);
+    }
+}
+
+// The world contacts of one particle in a substep of one iteration whose
+// contacts are regenerated: each slot would be regenerated from the
+// substep-start position xPrev and projected once with lamN = applied = 0,
+// so the contact is formed here and projected at once, in the same order
+// over k, and nothing is kept.
+__device__ __forceinline__ void world_contacts_fused(const real* prims, int nPrim, v3& x, v3 xPrev, real w,
+                                                     real radius, real margin, real fscale) {
+    for (int k = 0; k < nPrim; ++k) {
+        wcontact c;
+        world_contact_staged(c, prims + k * kPrimFloats, xPrev, w, radius, margin, fscale);
+        if (c.active) {
+            c.lamN = real(0.0);
+            c.applied = real(0.0);
+            world_contact_project(c, x, xPrev, w);
+        }
+    }
+}
+
 __device__ __forceinline__ unsigned world_contacts_begin_masked(real4* c4, int stride, const real* prims, int nPrim,
                                                                 v3 x, real w, real radius, real margin, real fscale,
                                                                 bool regen, unsigned act) {
@@ -219,28 +262,8 @@ __device__ __forceinline__ unsigned world_contacts_begin_masked(real4* c4, int s
     }
     unsigned out = 0u;
     for (int k = 0; k < nPrim; ++k) {
-        const real* sk = prims + k * kPrimFloats;
-        const real4 f = *reinterpret_cast<const real4*>(sk + 20);
         wcontact c;
-        c.active = 0;
-        if (f.w == -r_inf()) {
-            // A plane reads its type and point in one word, its normal from
-            // the far-test word and its friction: the fields the plane case
-            // of world_contact_generate uses.
-            const real4 ta = *reinterpret_cast<const real4*>(sk);
-            if (slot_int(ta.x) == kPlane) {
-                prim pr;
-                pr.type = kPlane;
-                pr.a = mk(ta.y, ta.z, ta.w);
-                pr.normal = mk(f.x, f.y, f.z);
-                pr.friction = sk[18];
-                world_contact_generate(c, pr, x, w, radius, margin, fscale);
-            } else {
-                world_contact_generate(c, load_prim(sk), x, w, radius, margin, fscale);
-            }
-        } else if (!prim_far_staged(f, x)) {
-            world_contact_generate(c, load_prim(sk), x, w, radius, margin, fscale);
-        }
+        world_contact_staged(c, prims + k * kPrimFloats, x, w, radius, margin, fscale);
         if (c.active) {
             real4* A = c4 + 2 * k * stride;
             A[0] = make_real4(c.n.x, c.n.y, c.n.z, c.offset);
