@@ -141,6 +141,21 @@ def test_all_primitive_types(strategy):
 
 
 @pytest.mark.parametrize("strategy", STRATEGIES)
+@pytest.mark.parametrize("iterations", [1, 2])
+def test_contacts_kept_between_regenerations(strategy, iterations):
+    """Contact interval 3 (src/core/solver.cpp:282-284: contacts regenerated at every
+    third substep, their multipliers reset at the others): the world-contact
+    slots live across substeps, with one and with two iterations. With one
+    iteration and an interval of 1 the TILED path keeps no slots at all; this
+    case runs the one that keeps them."""
+    params = rp.SolverParams(dt=1e-3, substeps=8, iterations=iterations, gravity=(0.0, 0.0, -9.81),
+                             contact_interval=3)
+    diffs, _, _ = run_pair([benchmark_rod(40), benchmark_rod(64)], ALL_PRIMS_WORLD, params, 120, strategy,
+                           checkpoints=[10, 120])
+    assert_bitwise(diffs)
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
 def test_per_rod_stiffness_and_friction_scales(strategy):
     """setMaterialScales: stiffness on a swinging cantilever, friction on an
     incline that holds at mu = 0.5 and slides at half (gpu_cases.cpp:316-419)."""
