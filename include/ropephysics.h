@@ -442,6 +442,8 @@ int rp_batch_precision(void* batch);
  * request's horizon. One step definition
  * (rp_step_params) serves every request. A request's result is bitwise the
  * one it gets alone: the per-rod arithmetic does not depend on its neighbours.
+ * Every server of a process uses the same two streams per device, created
+ * once with a full CU mask so that each has a hardware queue of its own. */
  *
  * Use: rp_server_sync; read and release what rp_server_finished lists; submit
  * what has arrived; rp_server_step; repeat. Not thread-safe. */
@@ -497,7 +499,9 @@ typedef struct rp_server_stats {
 int rp_server_create(const rp_server_desc* desc, void** out_server);
 void rp_server_destroy(void* server);
 
-/* Admit a request of `steps` steps. RP_ERR_CAPACITY: no room now (release
+/* Admit a request of `steps` steps. The host arrays are copied into pinned
+ * staging memory before the call returns; their upload is queued behind the
+ * launch in flight, not waited for. RP_ERR_CAPACITY: no room now (release
  * finished requests and retry) or never (larger than the budget, or a rod no
  * tile shape takes; rp_last_error says which). */
 int rp_server_submit(void* server, const rp_batch_desc* request, const rp_request_state* state, int steps,
