@@ -443,7 +443,7 @@ int rp_batch_precision(void* batch);
  * (rp_step_params) serves every request. A request's result is bitwise the
  * one it gets alone: the per-rod arithmetic does not depend on its neighbours.
  * Every server of a process uses the same two streams per device, created
- * once with a full CU mask so that each has a hardware queue of its own. */
+ * once with a full CU mask so that each has a hardware queue of its own.
  *
  * Use: rp_server_sync; read and release what rp_server_finished lists; submit
  * what has arrived; rp_server_step; repeat. Not thread-safe. */
