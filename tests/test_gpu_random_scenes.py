@@ -6,12 +6,14 @@ PLAIN and TILED at several tile shapes. Each rod's final state must equal the
 FP32 oracle's bit for bit (the reference's step, src/core/solver.cpp, stepped
 per rod in the GPU's colour order). The scenes cross paths the fixed
 scenarios take one at a time: world contacts kept in slots or formed in the
-sweep, far tests of every primitive type, self contacts on candidate lists."""
+sweep, far tests of every primitive type, self contacts on candidate lists.
+Half of the seeds run again as FP64 batches against the FP64 oracle."""
 import numpy as np
 import pytest
 
 import ropephysics_amd as rp
 from helpers import compare_bitwise, compliance_scale_f32, oracle_params, oracle_state, oracle_twin, oracle_world
+from test_gpu_fp64 import FP64_TILE_CONFIGS, diff_words
 
 pytestmark = pytest.mark.gpu
 
@@ -74,22 +76,29 @@ def random_scene(seed: int):
     return rods, world, params, scales, shape
 
 
-def run_scene(seed: int, strategy: str, steps: int = 60):
+def run_scene(seed: int, strategy: str, steps: int = 60, precision: int = 32):
     rods, world, params, (ys, fs), shape = random_scene(seed)
+    if precision == 64 and shape not in FP64_TILE_CONFIGS:
+        shape = 0
     try:
-        b = rp.Batch(rods, strategy=strategy, world=world, tile_config=shape if strategy == "tiled" else 0)
+        b = rp.Batch(rods, strategy=strategy, world=world, tile_config=shape if strategy == "tiled" else 0,
+                     precision=precision)
     except rp.RopePhysicsError:
-        b = rp.Batch(rods, strategy=strategy, world=world)  # that shape does not fit these rods
+        b = rp.Batch(rods, strategy=strategy, world=world, precision=precision)  # the shape does not fit
     b.set_material_scales(ys, fs)
     b.step(params, steps)
     b.synchronize()
     ow = oracle_world(world)
     diffs = {"x": 0, "v": 0, "q": 0, "omega": 0}
     for i, r in enumerate(rods):
-        o = oracle_twin(r, 32)
-        o.step(oracle_params(params, compliance_scale=compliance_scale_f32(ys[i]),
-                             friction_scale=float(np.float32(fs[i]))), ow, steps)
-        d = compare_bitwise(b.download(i), oracle_state(o))
+        o = oracle_twin(r, precision)
+        if precision == 32:
+            op = oracle_params(params, compliance_scale=compliance_scale_f32(ys[i]),
+                               friction_scale=float(np.float32(fs[i])))
+        else:
+            op = oracle_params(params, compliance_scale=1.0 / float(ys[i]), friction_scale=float(fs[i]))
+        o.step(op, ow, steps)
+        d = (compare_bitwise if precision == 32 else diff_words)(b.download(i), oracle_state(o))
         for k in diffs:
             diffs[k] += d[k]
     moved = max(float(np.max(np.abs(b.download(i)["x"] - r.x))) for i, r in enumerate(rods))
@@ -103,3 +112,11 @@ def test_random_scene_bitwise(seed, strategy):
     diffs, moved = run_scene(seed, strategy)
     assert all(v == 0 for v in diffs.values()), (seed, strategy, diffs)
     assert moved > 1e-4  # the rods really moved
+
+
+@pytest.mark.parametrize("strategy", ["plain", "tiled"])
+@pytest.mark.parametrize("seed", SEEDS[::2])
+def test_random_scene_bitwise_fp64(seed, strategy):
+    diffs, moved = run_scene(seed, strategy, precision=64)
+    assert all(v == 0 for v in diffs.values()), (seed, strategy, diffs)
+    assert moved > 1e-4
