@@ -424,7 +424,8 @@ def global_leg(num_rods: int, warmup: int, segments: int = 2000, steps: int = 20
 
 # ------------------------------------------------------------------ serving stream (latency)
 
-def serving_leg(num: int, load: float, budget_gib: float, slice_us: int, capacity_num: int = 40) -> dict:
+def serving_leg(num: int, load: float, budget_gib: float, slice_us: int, capacity_num: int = 40,
+                tune_num: int = 16, tune_steps: int = 512) -> dict:
     """The serving workload through the server in its two modes: chunked
     steps (64 steps per server step, every resident request in lockstep) and
     time-sliced steps (`slice_us` per server step, each workgroup running the
@@ -433,7 +434,9 @@ def serving_leg(num: int, load: float, budget_gib: float, slice_us: int, capacit
     a Poisson stream at `load` times the time-sliced mode's capacity through
     the time-sliced server (the headline p50/p99), and the first
     `capacity_num` of the same stream at the same rate through the chunked one.
-    Latency is from arrival to result on the host."""
+    Latency is from arrival to result on the host. Each mode's tile shapes
+    come from `serving.autotune_shapes` on the first `tune_num` requests cut
+    to `tune_steps` steps (0 requests: the server's defaults)."""
     from ropephysics_amd import serving
 
     reqs = serving.make_requests(num)
@@ -449,8 +452,20 @@ def serving_leg(num: int, load: float, budget_gib: float, slice_us: int, capacit
            "rate_basis": "load x the time-sliced mode's saturated capacity"}
     modes = (("chunked", {}), ("time_sliced", {"time_slice_us": slice_us}))
     servers, capacity = {}, {}
+    budget = int(budget_gib * (1 << 30))
     for mode, kw in modes:
-        server = rp.Server(serving.PARAMS, hbm_budget_bytes=int(budget_gib * (1 << 30)), **kw)
+        tune = None
+        if tune_num > 0:
+            best, trials = serving.autotune_shapes(head[:tune_num], budget, time_slice_us=kw.get("time_slice_us", 0),
+                                                   steps=tune_steps)
+            name = rp.tile_configs()
+            tune = {"sample_requests": min(tune_num, len(head)), "steps": tune_steps,
+                    "trials_s": [[name[a], name[b], t] for (a, b), t in trials],
+                    "chosen": [name[best[0]], name[best[1]]]}
+            kw = dict(kw, tile_config=best[0], self_tile_config=best[1])
+            log(f"serving {mode}: shapes {name[best[0]]} / {name[best[1]]} (autotuned on {tune['sample_requests']} "
+                f"requests of {tune_steps} steps)")
+        server = rp.Server(serving.PARAMS, hbm_budget_bytes=budget, **kw)
         serving.run_stream(server, serving.make_requests(2, seed=5), np.zeros(2))  # code objects, first launches
         sat = serving.run_stream(server, head, np.zeros(len(head)))
         capacity[mode] = len(head) / sat["wall"]
@@ -459,8 +474,10 @@ def serving_leg(num: int, load: float, budget_gib: float, slice_us: int, capacit
         out[mode] = {"saturated": {"requests": len(head), "wall_s": sat["wall"], "value": work_sat / sat["wall"],
                                    "unit": "segment-substeps/s", "requests_per_s": capacity[mode],
                                    "launches": sat["launches"]}}
-        if kw:
+        if "time_slice_us" in kw:
             out[mode]["time_slice_us"] = slice_us
+        if tune is not None:
+            out[mode]["autotune"] = tune
         log(f"serving {mode}: saturated on {len(head)} requests {work_sat / sat['wall'] / 1e9:.3f} G "
             f"seg-substeps/s ({capacity[mode]:.2f} req/s)")
     rate = load * capacity["time_sliced"]
@@ -713,6 +730,8 @@ def main():
                     help="Poisson rate as a fraction of the time-sliced server's capacity")
     ap.add_argument("--serve-slice-us", type=int, default=50000, help="time slice of the time-sliced server mode")
     ap.add_argument("--serve-budget-gib", type=float, default=4.0, help="server HBM budget")
+    ap.add_argument("--serve-tune-requests", type=int, default=16,
+                    help="requests of each serving mode's tile-shape autotuning sample (0: the server's defaults)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-traffic kernel for "
                          "scaling hardware byte counters)")
@@ -762,7 +781,7 @@ def main():
     glob = global_leg(args.global_rods, W) if args.global_rods > 0 else None
     cem = cem_leg(args.cem_iterations, args.cem_population, not args.no_cpu) if args.cem_iterations > 0 else None
     serving_out = serving_leg(args.serve_requests, args.serve_load, args.serve_budget_gib, args.serve_slice_us,
-                              min(args.serve_capacity_requests, args.serve_requests)) \
+                              min(args.serve_capacity_requests, args.serve_requests), args.serve_tune_requests) \
         if args.serve_requests > 0 else None
 
     cpu = None

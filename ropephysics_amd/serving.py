@@ -7,6 +7,9 @@ U[0.3, 0.9] and, each with probability 1/2, a sphere, a capsule and a box
 under the rods; self-collision on in half of the requests. Per-rod stiffness
 0.4-4x (log-uniform) and friction 0.6-1.4x.
 
+``autotune_shapes`` picks a server's tile shapes by serving a sample of
+requests once per candidate pair.
+
 ``run_stream`` plays requests against a ``Server`` at given arrival times:
 arrived requests are admitted in arrival order while the budget has room,
 every running request advances in each launch, and finished results are read
@@ -18,7 +21,7 @@ clock.
 from __future__ import annotations
 
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -154,3 +157,76 @@ def run_stream(server: Server, requests: list, arrivals: np.ndarray, keep_result
     wall = time.perf_counter() - t0 - float(np.min(arrivals))
     return {"latency": latency, "wall": wall, "launches": launches, "max_resident": max_resident,
             "results": results}
+
+
+# Tile-shape pairs (rods without self-collision, rods with it) that
+# autotune_shapes tries, by configuration id: tp3-tc1-w4 / tp1-tc1-w8 (the
+# chunked default), tp2-tc1-w4 / tp1-tc1-w8, tp3-tc1-w2 / tp1-tc1-w8,
+# tp3-tc1-w4 / tp2-tc1-w4 (the time-sliced default), tp2-tc1-w4 for both and
+# tp3-tc1-w4 for both.
+SHAPE_PAIRS = ((10, 13), (9, 13), (6, 13), (10, 9), (9, 9), (10, 10))
+
+
+def autotune_shapes(requests: list, hbm_budget_bytes: int, time_slice_us: int = 0, steps: int = 64,
+                    pairs=SHAPE_PAIRS, params: SolverParams = PARAMS, max_launches: int = 10000):
+    """Choose a server's tile shapes on a sample of its requests.
+
+    Each (lean, self-colliding) pair in `pairs` serves `requests`, all
+    arriving at once and their horizons cut to `steps` steps, in a fresh
+    server of the given budget and mode, after a warm-up on the first two; the
+    pair with the shortest wall time wins. A pair under which some request is
+    refused by an empty server (a rod that shape cannot take) is skipped.
+    Returns (best pair, [(pair, seconds or None), ...]). The shapes change the
+    time only: every request's result is the same bit for bit.
+
+    There is no counterpart in the reference, which has no server; the batch
+    path's autotuner (run inside rp_batch_step) plays the same role for one
+    batch, but a server's resident set changes at every admission, and trials
+    on it would stall the running requests, so the server's shapes are chosen
+    before it serves."""
+    sample = [replace(q, steps=min(q.steps, steps)) for q in requests]
+    trials = []
+    for lean, selfc in pairs:
+        server = Server(params, hbm_budget_bytes, tile_config=lean, self_tile_config=selfc,
+                        time_slice_us=time_slice_us)
+        try:
+            t = _serve_at_once(server, sample[:2], max_launches)  # code objects, first launches
+            if t is not None:
+                t = _serve_at_once(server, sample, max_launches)
+        finally:
+            server.destroy()
+        trials.append(((lean, selfc), t))
+    timed = [(p, t) for p, t in trials if t is not None]
+    if not timed:
+        raise RuntimeError("no tile-shape pair serves these requests")
+    return min(timed, key=lambda e: e[1])[0], trials
+
+
+def _serve_at_once(server: Server, requests: list, max_launches: int):
+    """Wall time to serve `requests` arriving at once (admitted in order as
+    the budget frees), or None when an empty server refuses one of them."""
+    t0 = time.perf_counter()
+    pending = list(requests)
+    running = 0
+    launches = 0
+    while pending or running:
+        while pending:
+            q = pending[0]
+            try:
+                server.submit(q.rods, q.world, q.steps, q.youngs, q.friction, x=q.x)
+            except ServerFull:
+                if running == 0:
+                    return None
+                break
+            pending.pop(0)
+            running += 1
+        server.step()
+        server.synchronize()
+        launches += 1
+        for rid in server.finished():
+            server.result(rid)
+            server.release(rid)
+            running -= 1
+        if launches > max_launches:
+            raise RuntimeError("the sample did not finish within max_launches server steps")
+    return time.perf_counter() - t0

@@ -212,3 +212,34 @@ def test_admission_and_batch_share_the_packer_lds_bound():
     server.synchronize()
     assert server.finished() == [rid]
     assert np.all(np.isfinite(server.result(rid)["x"]))
+
+
+def test_serving_autotune_picks_a_pair_and_results_do_not_depend_on_it():
+    """serving.autotune_shapes serves a sample once per shape pair and returns
+    the fastest; a request's result is the same bit for bit under the chosen
+    shapes and the defaults, in both server modes."""
+    from ropephysics_amd import serving
+    from dataclasses import replace
+
+    reqs = [replace(q, steps=min(q.steps, 24)) for q in serving.make_requests(5, seed=11)]
+    pairs = serving.SHAPE_PAIRS[:3]
+    for slice_us in (0, 300):
+        best, trials = serving.autotune_shapes(reqs[:4], 1 << 30, time_slice_us=slice_us, steps=24, pairs=pairs)
+        assert best in pairs
+        assert [p for p, _ in trials] == list(pairs)
+        assert all(t is not None and t > 0 for _, t in trials)
+        got = {}
+        for key, kw in (("default", {}), ("tuned", {"tile_config": best[0], "self_tile_config": best[1]})):
+            server = rp.Server(serving.PARAMS, hbm_budget_bytes=1 << 30, time_slice_us=slice_us, **kw)
+            q = reqs[4]
+            rid = server.submit(q.rods, q.world, q.steps, q.youngs, q.friction, x=q.x)
+            rounds = 0
+            while not server.finished():
+                server.step()
+                server.synchronize()
+                rounds += 1
+                assert rounds < 1000
+            got[key] = server.result(rid)
+            server.destroy()
+        for k in got["default"]:
+            assert np.array_equal(bits(got["default"][k]), bits(got["tuned"][k])), k
