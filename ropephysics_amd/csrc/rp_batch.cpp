@@ -208,7 +208,9 @@ int pack_tiles(BatchT<Real>& b, int configId, int ldsCap, PackedTiles& out) {
         if (!b.tileable[r]) return RP_ERR_CAPACITY;
         ids[r] = r;
     }
-    return pack_rods(b.rods, ids, b.colStart, configId, ldsCap, out);
+    const int rc = pack_rods(b.rods, ids, b.colStart, configId, ldsCap, out);
+    if (rc == RP_OK) place_tiles(out, b.computeUnits);
+    return rc;
 }
 
 int upload_tiles(PackedTiles& t) {

@@ -173,6 +173,10 @@ struct Packing {
 template <typename Real>
 int pack_rods(const std::vector<RodInfoT<Real>>& rods, const std::vector<int>& ids, const std::vector<int>& colStart,
               int configId, int ldsCap, Packing& out);
+// Reorders the tiles of a packing whose launch pairs workgroups b and b + units
+// on a compute unit (pair_units): the costliest take the units they hold
+// alone, the rest pair costliest with cheapest. No-op for other launches.
+void place_tiles(Packing& p, int units);
 
 template <typename Real>
 StepArgsT<Real> step_args(const typename Prec<Real>::StepParams& p, int steps);
