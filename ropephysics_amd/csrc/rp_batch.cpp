@@ -209,8 +209,13 @@ int pack_tiles(BatchT<Real>& b, int configId, int ldsCap, PackedTiles& out) {
         ids[r] = r;
     }
     const int rc = pack_rods(b.rods, ids, b.colStart, configId, ldsCap, out);
-    if (rc == RP_OK) place_tiles(out, b.computeUnits);
-    return rc;
+    if (rc != RP_OK) return rc;
+    const TileConfig& cfg = tile_config(configId);
+    const int selfMode = b.anySelf ? (b.anyColoured ? 2 : 1) : 0;
+    const int resident = Prec<Real>::tiled_resident(cfg.tp, cfg.tc, selfMode, 64 * cfg.waves, size_t(out.maxLds));
+    out.pairUnits = pair_units(int(out.tiles.size()), b.computeUnits, resident);
+    place_tiles(out, out.pairUnits);
+    return RP_OK;
 }
 
 int upload_tiles(PackedTiles& t) {
@@ -255,7 +260,7 @@ int launch_steps(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int 
         const int n = std::min(left, b.stepsPerLaunch);
         StepArgsT<Real> a = step_args<Real>(p, n);
         a.progress = static_cast<int*>(t->dProgress.p);
-        a.pairUnits = pair_units(int(t->tiles.size()), b.computeUnits);
+        a.pairUnits = t->pairUnits;
         const int selfMode = b.anySelf ? (b.anyColoured ? 2 : 1) : 0;
         RP_HIP(Prec<Real>::launch_tiled(cfg.tp, cfg.tc, selfMode, b.d, a, static_cast<const Tile*>(t->dTiles.p),
                                         static_cast<const TileSlot*>(t->dSlots.p), int(t->tiles.size()),

@@ -212,9 +212,9 @@ int pack_rods(const std::vector<RodInfoT<Real>>& rods, const std::vector<int>& i
     return RP_OK;
 }
 
-// A launch of n tiles on `units` compute units with units < n <= 2 units runs
-// workgroups b and b + units on one unit for b < n - units, and workgroups
-// n - units .. units - 1 alone. The launch lasts as long as its slowest unit.
+// A launch of n tiles on `units` compute units with units < n <= 2 units, two
+// workgroups fitting on a unit, runs workgroups b and b + units on one unit
+// for b < n - units, and workgroups n - units .. units - 1 alone. The launch lasts as long as its slowest unit.
 // A tile's cost is estimated as the sum over its rods of lanes squared: a
 // rod's self-contact search (the candidate scan over the hash cells its
 // segments reach, and the lists it keeps) grows with the square of its length
@@ -226,8 +226,8 @@ int pack_rods(const std::vector<RodInfoT<Real>>& rods, const std::vector<int>& i
 // index, so every result is the same bit for bit.
 void place_tiles(Packing& p, int units) {
     const int n = int(p.tiles.size());
-    const int pairs = pair_units(n, units) ? n - units : 0;
-    if (pairs == 0) return;
+    if (units <= 0 || n <= units || n > 2 * units) return;
+    const int pairs = n - units;
     std::vector<long long> cost(size_t(n), 0);
     std::vector<int> byCost(static_cast<size_t>(n));
     for (int i = 0; i < n; ++i) {

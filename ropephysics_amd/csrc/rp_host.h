@@ -28,6 +28,7 @@ namespace rp {
                             const Tile* tiles, const TileSlot* slots, int numTiles, int threads, size_t lds,   \
                             hipStream_t s);                                                                    \
     bool tiled_shape_available(int tp, int tc, int threads);                                                   \
+    int tiled_resident(int tp, int tc, int self, int threads, size_t lds);                                     \
     hipError_t launch_fixed_frames(R* q, const R* src, const R* invInertia4, int f0, int count, hipStream_t s);
 namespace f32 {
 RP_KERNEL_API(float)
@@ -61,6 +62,7 @@ struct Prec<float> {
     static constexpr auto global_max_threads = &f32::global_max_threads;
     static constexpr auto launch_tiled = &f32::launch_tiled;
     static constexpr auto tiled_shape_available = &f32::tiled_shape_available;
+    static constexpr auto tiled_resident = &f32::tiled_resident;
     static constexpr auto launch_fixed_frames = &f32::launch_fixed_frames;
     static Vec4 make4(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
 };
@@ -81,6 +83,7 @@ struct Prec<double> {
     static constexpr auto global_max_threads = &f64::global_max_threads;
     static constexpr auto launch_tiled = &f64::launch_tiled;
     static constexpr auto tiled_shape_available = &f64::tiled_shape_available;
+    static constexpr auto tiled_resident = &f64::tiled_resident;
     static constexpr auto launch_fixed_frames = &f64::launch_fixed_frames;
     static Vec4 make4(double a, double b, double c, double d) { return make_double4(a, b, c, d); }
 };
@@ -149,11 +152,13 @@ int lds_limits(LdsLimits& out);
 // Dynamic LDS budget of a workgroup when `perCU` of them share a CU.
 int lds_budget(const LdsLimits& l, int perCU);
 // Workgroups of a launch of `numTiles` pair up on `units` compute units when
-// there are more than one and at most two per unit: dispatch gives every unit
-// one workgroup, then a second, so b and b + units share a unit. Returns that
-// distance, or 0.
-inline int pair_units(int numTiles, int units) {
-    return units > 0 && numTiles > units && numTiles <= 2 * units ? units : 0;
+// there are more than one and at most two per unit and at least two fit on a
+// unit at once (`resident`, from the kernel's registers and LDS): dispatch
+// gives every unit one workgroup, then a second, so b and b + units share a
+// unit. Returns that distance, or 0. With one resident workgroup per unit the
+// launch runs in two rounds instead, and nothing pairs.
+inline int pair_units(int numTiles, int units, int resident) {
+    return resident >= 2 && units > 0 && numTiles > units && numTiles <= 2 * units ? units : 0;
 }
 
 template <typename Real>
@@ -165,6 +170,7 @@ int rod_tiled_lds(const RodInfoT<Real>& r);
 // `ldsCap` bytes of LDS (best fit, widest rods first).
 struct Packing {
     int config = 0;
+    int pairUnits = 0;  // pair_units of the packing's launch
     std::vector<Tile> tiles;
     std::vector<TileSlot> slots;
     int maxLds = 0;
@@ -174,8 +180,8 @@ template <typename Real>
 int pack_rods(const std::vector<RodInfoT<Real>>& rods, const std::vector<int>& ids, const std::vector<int>& colStart,
               int configId, int ldsCap, Packing& out);
 // Reorders the tiles of a packing whose launch pairs workgroups b and b + units
-// on a compute unit (pair_units): the costliest take the units they hold
-// alone, the rest pair costliest with cheapest. No-op for other launches.
+// on a compute unit (`units` from pair_units): the costliest take the units
+// they hold alone, the rest pair costliest with cheapest. No-op for units 0.
 void place_tiles(Packing& p, int units);
 
 template <typename Real>

@@ -941,6 +941,29 @@ bool tiled_shape_available(int tp, int tc, int threads) {
     return !RP_FP64 && tc == 2 && (tp == 3 || tp == 5);
 }
 
+template <int TP, int TC>
+static int tiled_resident_t(int self, int threads, size_t lds) {
+    const void* k = self == 2   ? reinterpret_cast<const void*>(&k_tiled_step<TP, TC, 2>)
+                    : self == 1 ? reinterpret_cast<const void*>(&k_tiled_step<TP, TC, 1>)
+                                : reinterpret_cast<const void*>(&k_tiled_step<TP, TC, 0>);
+    int n = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, lds) == hipSuccess ? n : 0;
+}
+
+// Workgroups of a TILED launch that fit on one compute unit at once, as its
+// registers and `lds` bytes of dynamic LDS allow; 0 when unknown.
+int tiled_resident(int tp, int tc, int self, int threads, size_t lds) {
+    if (!tiled_shape_available(tp, tc, threads) || self < 0 || self > 2) return 0;
+    if (tp == 1 && tc == 1) return tiled_resident_t<1, 1>(self, threads, lds);
+    if (tp == 2 && tc == 1) return tiled_resident_t<2, 1>(self, threads, lds);
+    if (tp == 3 && tc == 1) return tiled_resident_t<3, 1>(self, threads, lds);
+#if !RP_FP64
+    if (tp == 3 && tc == 2) return tiled_resident_t<3, 2>(self, threads, lds);
+    if (tp == 5 && tc == 2) return tiled_resident_t<5, 2>(self, threads, lds);
+#endif
+    return 0;
+}
+
 #if !RP_FP64
 // The working arrays of GLOBAL steps taken through rp_batch_step_noalloc, for
 // callers that cannot allocate device memory: a fixed region of the code

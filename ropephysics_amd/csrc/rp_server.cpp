@@ -571,10 +571,16 @@ int rp_server_step(void* server, int* num_finished) {
         for (int k = 0; k < 2; ++k) {
             packs[k]->tiles.clear();
             packs[k]->slots.clear();
+            packs[k]->pairUnits = 0;
             if (ids[k]->empty()) continue;
             int rc = pack_rods(s.rods, *ids[k], s.colStart, cfgs[k], lds_budget(s.lim, 2), *packs[k]);
             if (rc) rc = pack_rods(s.rods, *ids[k], s.colStart, cfgs[k], lds_budget(s.lim, 1), *packs[k]);
             if (rc) return fail(rc, "resident rods do not pack");
+            const TileConfig& pc = tile_config(cfgs[k]);
+            packs[k]->pairUnits =
+                pair_units(int(packs[k]->tiles.size()), s.lim.computeUnits,
+                           tiled_resident(pc.tp, pc.tc, k == 0 ? 0 : (coloured ? 2 : 1), 64 * pc.waves,
+                                          size_t(packs[k]->maxLds)));
             up.add(s.dTiles + k * R, 0, packs[k]->tiles.data(), (long long)packs[k]->tiles.size());
             up.add(s.dSlots + k * R, 0, packs[k]->slots.data(), (long long)packs[k]->slots.size());
         }
@@ -603,7 +609,7 @@ int rp_server_step(void* server, int* num_finished) {
         RP_HIP(hipEventRecord(s.evReady, s.stream));
         RP_HIP(hipStreamWaitEvent(s.selfStream, s.evReady, 0));
         a.progress = s.dProgress + R;
-        a.pairUnits = oneLaunch ? pair_units(int(s.packSelf.tiles.size()), s.lim.computeUnits) : 0;
+        a.pairUnits = oneLaunch ? s.packSelf.pairUnits : 0;
         RP_HIP(launch_tiled(cfg.tp, cfg.tc, coloured ? 2 : 1, s.d, a, s.dTiles + R, s.dSlots + R,
                             int(s.packSelf.tiles.size()), 64 * cfg.waves, size_t(s.packSelf.maxLds), s.selfStream));
         RP_HIP(hipEventRecord(s.evSelfDone, s.selfStream));
@@ -612,7 +618,7 @@ int rp_server_step(void* server, int* num_finished) {
     if (!lean.empty()) {
         const TileConfig& cfg = tile_config(s.configLean);
         a.progress = s.dProgress;
-        a.pairUnits = oneLaunch ? pair_units(int(s.packLean.tiles.size()), s.lim.computeUnits) : 0;
+        a.pairUnits = oneLaunch ? s.packLean.pairUnits : 0;
         RP_HIP(launch_tiled(cfg.tp, cfg.tc, 0, s.d, a, s.dTiles, s.dSlots, int(s.packLean.tiles.size()),
                             64 * cfg.waves, size_t(s.packLean.maxLds), s.stream));
         ++s.launches;
