@@ -327,7 +327,7 @@ int autotune(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int only
     const int trialSteps = 8;
     b.tuneLog.clear();
     const int first = onlyConfig > 0 ? onlyConfig : 1, last = onlyConfig > 0 ? onlyConfig : num_tile_configs();
-    for (int id = first; id <= last; ++id)
+    for (int id = first; id <= last && rc == RP_OK; ++id)
         for (int perCU = 1, lastTiles = -1; perCU <= 4; ++perCU) {
             auto t = std::make_unique<PackedTiles>();
             if (pack_tiles(b, id, lds_budget(b, perCU), *t) != RP_OK) continue;
@@ -335,10 +335,13 @@ int autotune(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int only
             if (int(t->tiles.size()) == lastTiles) continue;
             lastTiles = int(t->tiles.size());
             if (upload_tiles(*t) != RP_OK) continue;
-            // A warm-up launch (code object load), then the best of two timed ones.
+            // A warm-up launch (code object load), then the best of two timed
+            // ones. Every timed launch starts from the saved state, so each
+            // shape is timed over the same steps.
             if (launch_steps(b, p, 1, RP_STRATEGY_TILED, t.get()) != RP_OK) continue;
             float ms = 1e30f;
-            for (int rep = 0; rep < 2; ++rep) {
+            for (int rep = 0; rep < 2 && rc == RP_OK; ++rep) {
+                if ((rc = backup_state(b, true))) break;
                 RP_HIP(hipEventRecord(e0, b.stream));
                 if (launch_steps(b, p, trialSteps, RP_STRATEGY_TILED, t.get()) != RP_OK) break;
                 RP_HIP(hipEventRecord(e1, b.stream));
@@ -347,6 +350,7 @@ int autotune(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int only
                 RP_HIP(hipEventElapsedTime(&m, e0, e1));
                 ms = std::min(ms, m);
             }
+            if (rc) break;
             b.tuneLog.push_back(std::make_pair(id, ms));
             if (ms < best) {
                 second = best;
@@ -360,12 +364,13 @@ int autotune(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int only
         }
     // Two shapes within 10% of each other in the short trials: time both over
     // four times as many steps (best of two) and keep the faster.
-    if (bestTiles && secondTiles && second < 1.1 * best) {
+    if (rc == RP_OK && bestTiles && secondTiles && second < 1.1 * best) {
         float ms[2] = {1e30f, 1e30f};
         PackedTiles* cand[2] = {bestTiles.get(), secondTiles.get()};
         for (int c = 0; c < 2; ++c) {
             if (upload_tiles(*cand[c]) != RP_OK) continue;
-            for (int rep = 0; rep < 2; ++rep) {
+            for (int rep = 0; rep < 2 && rc == RP_OK; ++rep) {
+                if ((rc = backup_state(b, true))) break;
                 RP_HIP(hipEventRecord(e0, b.stream));
                 if (launch_steps(b, p, 4 * trialSteps, RP_STRATEGY_TILED, cand[c]) != RP_OK) break;
                 RP_HIP(hipEventRecord(e1, b.stream));
@@ -382,6 +387,7 @@ int autotune(BatchT<Real>& b, const typename Prec<Real>::StepParams& p, int only
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b.d.rec = recording;
+    if (rc) return rc;
     rc = backup_state(b, true);
     if (rc) return rc;
     if (!bestTiles) return fail(RP_ERR_CAPACITY, "no tile configuration fits this batch");
