@@ -480,10 +480,12 @@ def test_calls_longer_than_one_launch():
 def test_paired_workgroups_give_the_same_bits():
     """A TILED launch with between one and two workgroups per compute unit,
     where workgroups b and b + units trade issue priority by step count at
-    each of the first 64 steps and every 64 after: routing cables of 130-160
-    segments (self-collision, world contacts, gripper paths), one per 256-lane
-    workgroup whatever the LDS budget, 1.5 per compute unit, 300 steps in one
-    launch, equal PLAIN bit for bit."""
+    each of the first 64 steps and every 64 after, and the tiles are placed by
+    estimated cost (the costliest alone on a unit, the rest paired costliest
+    with cheapest, so the launch order differs from the packing order): routing
+    cables of 130-160 segments (self-collision, world contacts, gripper paths),
+    one per 256-lane workgroup whatever the LDS budget, 1.5 per compute unit,
+    300 steps in one launch, equal PLAIN bit for bit."""
     from ropephysics_amd import _abi, scenes
 
     units = _abi.device_info()["compute_units"]
